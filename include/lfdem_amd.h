@@ -1,0 +1,145 @@
+/*
+ * lfdem_amd.h -- C ABI of the MI355X (gfx950) Stokes-resistance library.
+ *
+ * Two entry families:
+ *
+ *  1. lfd_stokes_*  -- drop-in for LF_DEM's StokesSolver (StokesSolver.h:31-286):
+ *     the same fill -> factorize -> RHS -> solve protocol and packed 6x6 block
+ *     formats (MatrixBlocks.h:5-84), solved on the GPU.
+ *  2. lfd_system_*  -- the device-resident step loop of LF_DEM's System
+ *     (System.cpp:888-983): cell list, pair lifecycle, assembly, solve and the
+ *     predictor-corrector move, all kept in HBM between steps.
+ *
+ * Conventions: plain pointers and sizes, no exceptions; every call returns an
+ * lfd_status (0 = success); lfd_last_error() describes the last failure of the
+ * calling thread.  Handles are not thread safe; use one per host thread.
+ * RHS / solution vectors are interleaved per particle as [F|U xyz, T|W xyz]
+ * (StokesSolver.h:38-40), FP64 at the boundary.
+ */
+#ifndef LFDEM_AMD_H
+#define LFDEM_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lfd_status {
+	LFD_OK = 0,
+	LFD_ERR_ARG = -1,            /* bad argument or call order */
+	LFD_ERR_HIP = -2,            /* HIP runtime failure */
+	LFD_ERR_NOT_SPD = -3,        /* a diagonal block is not positive definite */
+	LFD_ERR_NOT_CONVERGED = -4,  /* Krylov / refinement limit reached */
+	LFD_ERR_CAPACITY = -5,       /* pair or per-row capacity exceeded */
+	LFD_ERR_BOX = -6,            /* position outside the cell grid (BoxSet.cpp:775-778) */
+	LFD_ERR_UNSUPPORTED = -7     /* feature outside this library's scope */
+} lfd_status;
+
+/* Physical parameters, hydrodynamic units (gamma-dot = 1); defaults follow
+ * SimulationInit.cpp:941-1014 with the configuration of the benchmark
+ * workloads (kn=1e4, kt=5e3, mu=1, tau_n=1e-3, delta=1e-3, gap 0.5). */
+typedef struct lfd_params {
+	double shear_rate, theta_shear, lub_max_gap, lub_reduce, kn, kt;
+	double mu_static, mu_dynamic, tau_n, tau_t, sd_coeff, repulsion;
+	double repulsive_length, disp_max, dt_max, dt_min;
+	int friction_model;     /* 0 frictionless, 1 Coulomb (Contact.cpp:255-291) */
+	int lubrication_model;  /* 1 normal, 2 tangential */
+} lfd_params;
+
+typedef enum lfd_solver_kind {
+	LFD_SOLVER_MIXED = 0,   /* FP64 refinement around persistent FP32 block-Jacobi PCG */
+	LFD_SOLVER_PLAIN = 1    /* FP64 block-Jacobi PCG, one kernel per operation */
+} lfd_solver_kind;
+
+typedef struct lfd_solve_opts {
+	double tol;             /* target ||b - R x|| / ||b|| in FP64 */
+	double inner_tol;       /* residual drop per FP32 inner solve */
+	int max_outer, max_inner;
+	int lanes_per_row;      /* SpMV lanes per 6x6 block row: 1, 2, 4, 8 */
+	int groups;             /* workgroups of the persistent solver, 0 = auto */
+	int kind;               /* lfd_solver_kind */
+} lfd_solve_opts;
+
+typedef struct lfd_solve_stats {
+	int iters_inner, iters_outer;
+	double res_norm, rhs_norm;
+} lfd_solve_stats;
+
+const char *lfd_last_error(void);
+void lfd_default_params(lfd_params *p);
+void lfd_default_solve_opts(lfd_solve_opts *o);
+int lfd_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* StokesSolver drop-in                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct lfd_stokes lfd_stokes;
+
+/* StokesSolver::init (StokesSolver.cpp:77-106).  Only np_mobile == np_total
+ * is supported (the mixed mobile/fixed problem is out of scope). */
+int lfd_stokes_create(int np_total, int np_mobile, lfd_stokes **out);
+void lfd_stokes_destroy(lfd_stokes *s);
+int lfd_stokes_set_opts(lfd_stokes *s, const lfd_solve_opts *o);
+/* StokesSolver::resetResistanceMatrix (StokesSolver.cpp:474-511); drag holds
+ * np_total DBlocks (15 doubles each); nb_blocks_hint may overcount. */
+int lfd_stokes_reset(lfd_stokes *s, int nb_blocks_hint, const double *drag_dblocks, int pattern_changed);
+/* StokesSolver::startNewColumn (StokesSolver.cpp:921-935) */
+int lfd_stokes_start_new_column(lfd_stokes *s);
+/* StokesSolver::addResistanceBlocks (StokesSolver.cpp:121-129): i is the
+ * current column, j > i ascending within the column; dbi/dbj DBlocks (15),
+ * odb the ODBlock (18) at (row j, column i). */
+int lfd_stokes_add_blocks(lfd_stokes *s, int i, int j, const double *dbi, const double *dbj, const double *odb);
+/* StokesSolver::matrixFillingDone (StokesSolver.cpp:937-948): uploads the
+ * matrix and builds the preconditioner (the "factorization"). */
+int lfd_stokes_fill_done(lfd_stokes *s);
+/* RHS (StokesSolver.cpp:513-624) */
+int lfd_stokes_reset_rhs(lfd_stokes *s);
+int lfd_stokes_reset_rhs_torque(lfd_stokes *s);
+int lfd_stokes_add_rhs_force(lfd_stokes *s, int i, const double f[3]);
+int lfd_stokes_add_rhs_torque(lfd_stokes *s, int i, const double t[3]);
+int lfd_stokes_set_rhs_force(lfd_stokes *s, int i, const double f[3]);
+int lfd_stokes_set_rhs_torque(lfd_stokes *s, int i, const double t[3]);
+int lfd_stokes_set_rhs(lfd_stokes *s, const double *ft);
+int lfd_stokes_add_rhs(lfd_stokes *s, const double *ft);
+/* StokesSolver::solve (StokesSolver.cpp:671-703): uw has room for
+ * uw_len doubles; LFD_ERR_ARG when uw_len < 6*np_mobile (the reference throws). */
+int lfd_stokes_solve(lfd_stokes *s, double *uw, int uw_len, lfd_solve_stats *stats);
+/* StokesSolver::solvingIsDone (StokesSolver.cpp:847-851) */
+int lfd_stokes_solving_done(lfd_stokes *s);
+/* y = R x with the current matrix (StokesSolver::multiplySolutionByResMat's role) */
+int lfd_stokes_multiply(lfd_stokes *s, const double *x, double *y);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident System step loop                                     */
+/* ------------------------------------------------------------------ */
+typedef struct lfd_system lfd_system;
+
+/* System::setupConfiguration (System.cpp:468-521, 546-611): cell list,
+ * initial pairs, drag seed.  pos: np x 3 in [0, L); radius: np. */
+int lfd_system_create(int np, const double *pos, const double *radius, const double L[3],
+                      const double shear_disp[3], const lfd_params *p, const lfd_solve_opts *o,
+                      lfd_system **out);
+void lfd_system_destroy(lfd_system *s);
+/* System::timeEvolution first-call block (System.cpp:1219-1228) */
+int lfd_system_first_update(lfd_system *s);
+/* n predictor-corrector steps (System.cpp:1241-1256, 888-983); strain_end < 0
+ * disables the end-of-interval dt clamp.  Enqueued; errors are checked at the end. */
+int lfd_system_step(lfd_system *s, int nsteps, double strain_end);
+int lfd_system_sync(lfd_system *s);
+/* individual phases (tests) */
+int lfd_system_check_new_interactions(lfd_system *s);
+int lfd_system_update_interactions(lfd_system *s, int in_predictor, double dt);
+int lfd_system_assemble(lfd_system *s);
+int lfd_system_compute_velocities(lfd_system *s, int in_predictor);
+/* state access */
+int lfd_system_num_pairs(lfd_system *s);
+int lfd_system_get_pairs(lfd_system *s, int cap, int *p0, int *p1, int *state, double *geom5, double *disp_tan);
+int lfd_system_get_state(lfd_system *s, double *pos, double *vel6, double *na6, double *scalars8);
+int lfd_system_get_cells(lfd_system *s, int *dims4, int *labels);
+int lfd_system_get_blocks(lfd_system *s, int cap, double *dblocks, double *odblocks, int *p0, int *p1);
+int lfd_system_get_rhs(lfd_system *s, double *rhs);
+int lfd_system_get_solve_stats(lfd_system *s, lfd_solve_stats *st, long long *total_inner);
+int lfd_system_set_opts(lfd_system *s, const lfd_solve_opts *o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+"""ctypes binding of liblfdem_amd.so (the gfx950 library) for tests and benchmarks.
+
+The classes mirror LF_DEM's own interfaces: ``StokesSolver`` keeps the method
+names and call protocol of StokesSolver.h:198-285; ``System`` exposes the
+device-resident step loop of System.cpp:888-983.  There is no CPU fallback:
+if the library is missing or no GPU is present the calls fail loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "liblfdem_amd.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_lib = None
+
+STATUS = {0: "LFD_OK", -1: "LFD_ERR_ARG", -2: "LFD_ERR_HIP", -3: "LFD_ERR_NOT_SPD",
+          -4: "LFD_ERR_NOT_CONVERGED", -5: "LFD_ERR_CAPACITY", -6: "LFD_ERR_BOX",
+          -7: "LFD_ERR_UNSUPPORTED"}
+
+# every function of include/lfdem_amd.h with its argument types
+SIGNATURES = {
+    "lfd_last_error": ([], ctypes.c_char_p),
+    "lfd_default_params": ([_vp], None),
+    "lfd_default_solve_opts": ([_vp], None),
+    "lfd_device_count": ([], ctypes.c_int),
+    "lfd_stokes_create": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lfd_stokes_destroy": ([_vp], None),
+    "lfd_stokes_set_opts": ([_vp, _vp], ctypes.c_int),
+    "lfd_stokes_reset": ([_vp, ctypes.c_int, _dp, ctypes.c_int], ctypes.c_int),
+    "lfd_stokes_start_new_column": ([_vp], ctypes.c_int),
+    "lfd_stokes_add_blocks": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+    "lfd_stokes_fill_done": ([_vp], ctypes.c_int),
+    "lfd_stokes_reset_rhs": ([_vp], ctypes.c_int),
+    "lfd_stokes_reset_rhs_torque": ([_vp], ctypes.c_int),
+    "lfd_stokes_add_rhs_force": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_stokes_add_rhs_torque": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_stokes_set_rhs_force": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_stokes_set_rhs_torque": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_stokes_set_rhs": ([_vp, _dp], ctypes.c_int),
+    "lfd_stokes_add_rhs": ([_vp, _dp], ctypes.c_int),
+    "lfd_stokes_solve": ([_vp, _dp, ctypes.c_int, _vp], ctypes.c_int),
+    "lfd_stokes_solving_done": ([_vp], ctypes.c_int),
+    "lfd_stokes_multiply": ([_vp, _dp, _dp], ctypes.c_int),
+    "lfd_system_create": ([ctypes.c_int, _dp, _dp, _dp, _dp, _vp, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lfd_system_destroy": ([_vp], None),
+    "lfd_system_first_update": ([_vp], ctypes.c_int),
+    "lfd_system_step": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "lfd_system_sync": ([_vp], ctypes.c_int),
+    "lfd_system_check_new_interactions": ([_vp], ctypes.c_int),
+    "lfd_system_update_interactions": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "lfd_system_assemble": ([_vp], ctypes.c_int),
+    "lfd_system_compute_velocities": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_system_num_pairs": ([_vp], ctypes.c_int),
+    "lfd_system_get_pairs": ([_vp, ctypes.c_int, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
+    "lfd_system_get_state": ([_vp, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "lfd_system_get_cells": ([_vp, _ip, _ip], ctypes.c_int),
+    "lfd_system_get_blocks": ([_vp, ctypes.c_int, _dp, _dp, _ip, _ip], ctypes.c_int),
+    "lfd_system_get_rhs": ([_vp, _dp], ctypes.c_int),
+    "lfd_system_get_solve_stats": ([_vp, _vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "lfd_system_set_opts": ([_vp, _vp], ctypes.c_int),
+}
+
+
+class Params(ctypes.Structure):
+    """lfd_params (include/lfdem_amd.h)."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "shear_rate", "theta_shear", "lub_max_gap", "lub_reduce", "kn", "kt",
+        "mu_static", "mu_dynamic", "tau_n", "tau_t", "sd_coeff", "repulsion",
+        "repulsive_length", "disp_max", "dt_max", "dt_min")] + [
+        ("friction_model", ctypes.c_int), ("lubrication_model", ctypes.c_int)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+                ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int),
+                ("lanes_per_row", ctypes.c_int), ("groups", ctypes.c_int), ("kind", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iters_inner", ctypes.c_int), ("iters_outer", ctypes.c_int),
+                ("res_norm", ctypes.c_double), ("rhs_norm", ctypes.c_double)]
+
+
+class LfdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("liblfdem_amd.so is not built (run __graft_entry__.build()): %s" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise LfdError(rc, lib().lfd_last_error().decode())
+    return rc
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().lfd_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().lfd_default_solve_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class StokesSolver:
+    """GPU drop-in for LF_DEM's StokesSolver (StokesSolver.h:198-285)."""
+
+    def __init__(self, np_total, np_mobile=None, opts=None):
+        h = _vp()
+        _check(lib().lfd_stokes_create(np_total, np_total if np_mobile is None else np_mobile,
+                                       ctypes.byref(h)))
+        self.h = h
+        self.np = np_total
+        if opts is not None:
+            _check(lib().lfd_stokes_set_opts(self.h, ctypes.byref(opts)))
+        self.last_stats = SolveStats()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lfd_stokes_destroy(self.h)
+            self.h = None
+
+    def resetResistanceMatrix(self, nb_blocks_hint, drag_dblocks, pattern_changed=True):
+        d = np.ascontiguousarray(drag_dblocks, dtype=np.float64)
+        _check(lib().lfd_stokes_reset(self.h, int(nb_blocks_hint), _d(d), int(pattern_changed)))
+
+    def startNewColumn(self):
+        _check(lib().lfd_stokes_start_new_column(self.h))
+
+    def addResistanceBlocks(self, i, j, dbi, dbj, odb):
+        a = np.ascontiguousarray(dbi, dtype=np.float64)
+        b = np.ascontiguousarray(dbj, dtype=np.float64)
+        c = np.ascontiguousarray(odb, dtype=np.float64)
+        _check(lib().lfd_stokes_add_blocks(self.h, int(i), int(j), _d(a), _d(b), _d(c)))
+
+    def matrixFillingDone(self):
+        _check(lib().lfd_stokes_fill_done(self.h))
+
+    def resetRHS(self):
+        _check(lib().lfd_stokes_reset_rhs(self.h))
+
+    def resetRHStorque(self):
+        _check(lib().lfd_stokes_reset_rhs_torque(self.h))
+
+    def addToRHSForce(self, i, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        _check(lib().lfd_stokes_add_rhs_force(self.h, int(i), _d(f)))
+
+    def addToRHSTorque(self, i, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        _check(lib().lfd_stokes_add_rhs_torque(self.h, int(i), _d(t)))
+
+    def setRHSForce(self, i, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        _check(lib().lfd_stokes_set_rhs_force(self.h, int(i), _d(f)))
+
+    def setRHSTorque(self, i, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        _check(lib().lfd_stokes_set_rhs_torque(self.h, int(i), _d(t)))
+
+    def setRHS(self, ft):
+        ft = np.ascontiguousarray(ft, dtype=np.float64).ravel()
+        _check(lib().lfd_stokes_set_rhs(self.h, _d(ft)))
+
+    def addToRHS(self, ft):
+        ft = np.ascontiguousarray(ft, dtype=np.float64).ravel()
+        _check(lib().lfd_stokes_add_rhs(self.h, _d(ft)))
+
+    def solve(self, out_len=None):
+        """Returns (velocity (np,3), ang_velocity (np,3))."""
+        n = 6 * self.np if out_len is None else out_len
+        uw = np.zeros(max(n, 1))
+        _check(lib().lfd_stokes_solve(self.h, _d(uw), int(n), ctypes.byref(self.last_stats)))
+        u = uw[:6 * self.np].reshape(self.np, 6)
+        return u[:, :3].copy(), u[:, 3:].copy()
+
+    def solvingIsDone(self):
+        _check(lib().lfd_stokes_solving_done(self.h))
+
+    def multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.zeros_like(x)
+        _check(lib().lfd_stokes_multiply(self.h, _d(x), _d(y)))
+        return y
+
+
+class System:
+    """Device-resident LF_DEM System (setupConfiguration + step loop)."""
+
+    def __init__(self, pos, radius, L, shear_disp=(0.0, 0.0, 0.0), params=None, opts=None):
+        self.params = params if params is not None else default_params()
+        self.opts = opts if opts is not None else default_opts()
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rad = np.ascontiguousarray(radius, dtype=np.float64)
+        self.np = len(rad)
+        self.L = np.ascontiguousarray(L, dtype=np.float64)
+        sd = np.ascontiguousarray(shear_disp, dtype=np.float64)
+        h = _vp()
+        _check(lib().lfd_system_create(self.np, _d(pos), _d(rad), _d(self.L), _d(sd),
+                                       ctypes.byref(self.params), ctypes.byref(self.opts),
+                                       ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lfd_system_destroy(self.h)
+            self.h = None
+
+    def set_opts(self, opts):
+        self.opts = opts
+        _check(lib().lfd_system_set_opts(self.h, ctypes.byref(opts)))
+
+    def first_update(self):
+        _check(lib().lfd_system_first_update(self.h))
+
+    def step(self, nsteps=1, strain_end=-1.0):
+        _check(lib().lfd_system_step(self.h, int(nsteps), float(strain_end)))
+
+    def sync(self):
+        _check(lib().lfd_system_sync(self.h))
+
+    def check_new_interactions(self):
+        _check(lib().lfd_system_check_new_interactions(self.h))
+
+    def update_interactions(self, in_predictor, dt):
+        _check(lib().lfd_system_update_interactions(self.h, int(in_predictor), float(dt)))
+
+    def assemble(self):
+        _check(lib().lfd_system_assemble(self.h))
+
+    def compute_velocities(self, in_predictor=True):
+        _check(lib().lfd_system_compute_velocities(self.h, int(in_predictor)))
+
+    @property
+    def npairs(self):
+        return _check(lib().lfd_system_num_pairs(self.h))
+
+    def pairs(self):
+        n = self.npairs
+        cap = max(n, 1)
+        p0 = np.zeros(cap, dtype=np.int32)
+        p1 = np.zeros(cap, dtype=np.int32)
+        st = np.zeros(cap, dtype=np.int32)
+        geom = np.zeros((cap, 5))
+        dt = np.zeros((cap, 3))
+        n = _check(lib().lfd_system_get_pairs(self.h, cap, _i(p0), _i(p1), _i(st), _d(geom), _d(dt)))
+        return dict(p0=p0[:n], p1=p1[:n], state=st[:n], nvec=geom[:n, :3], r=geom[:n, 3],
+                    gap=geom[:n, 4], disp_tan=dt[:n])
+
+    def state(self):
+        pos = np.zeros((self.np, 3))
+        vel6 = np.zeros((self.np, 6))
+        na6 = np.zeros((self.np, 6))
+        sc = np.zeros(8)
+        _check(lib().lfd_system_get_state(self.h, _d(pos), _d(vel6), _d(na6), _d(sc)))
+        return dict(pos=pos, vel=vel6[:, :3].copy(), ang_vel=vel6[:, 3:].copy(), na_vel=na6[:, :3].copy(),
+                    na_ang_vel=na6[:, 3:].copy(), shear_disp=sc[:3].copy(), dt=sc[3], time=sc[4],
+                    strain=sc[5], max_velocity=sc[6], max_sliding_velocity=sc[7])
+
+    def cells(self):
+        dims = np.zeros(4, dtype=np.int32)
+        labels = np.zeros(self.np, dtype=np.int32)
+        nbox = _check(lib().lfd_system_get_cells(self.h, _i(dims), _i(labels)))
+        return dict(nx=int(dims[0]), ny=int(dims[1]), nz=int(dims[2]), boxed=bool(dims[3]), nbox=nbox,
+                    labels=labels)
+
+    def blocks(self):
+        n = self.npairs
+        cap = max(n, 1)
+        db = np.zeros((self.np, 15))
+        od = np.zeros((cap, 18))
+        p0 = np.zeros(cap, dtype=np.int32)
+        p1 = np.zeros(cap, dtype=np.int32)
+        n = _check(lib().lfd_system_get_blocks(self.h, cap, _d(db), _d(od), _i(p0), _i(p1)))
+        return db, od[:n], p0[:n], p1[:n]
+
+    def rhs(self):
+        r = np.zeros(6 * self.np)
+        _check(lib().lfd_system_get_rhs(self.h, _d(r)))
+        return r
+
+    def solve_stats(self):
+        st = SolveStats()
+        tot = ctypes.c_longlong(0)
+        _check(lib().lfd_system_get_solve_stats(self.h, ctypes.byref(st), ctypes.byref(tot)))
+        return dict(iters_inner=st.iters_inner, iters_outer=st.iters_outer, res_norm=st.res_norm,
+                    rhs_norm=st.rhs_norm, total_inner=tot.value)

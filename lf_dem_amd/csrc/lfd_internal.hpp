@@ -1,0 +1,131 @@
+// lfd_internal.hpp -- device data layout and kernel launchers of the
+// MI355X Stokes-resistance library (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include "lfd_physics.hpp"
+
+namespace lfd {
+
+// Error bits raised by kernels into DevScalars::error
+enum : int {
+    E_BOX_OOB = 1,        // a position fell outside the cell grid (BoxSet.cpp:775-778)
+    E_FOUND_OVERFLOW = 2, // more new pairs for one particle than the per-row slot count
+    E_PAIR_OVERFLOW = 4,  // pair capacity exceeded
+    E_NOT_SPD = 8,        // a 6x6 diagonal block is not positive definite
+    E_NOT_CONVERGED = 16, // Krylov iteration limit reached before the tolerance
+};
+
+// Per-instance scalars kept on the device so a step needs no host round trip.
+struct DevScalars {
+    double dt, time, strain;
+    double shear_disp[3];
+    unsigned long long max_vel_bits;    // max |U_na| (bit pattern of a non-negative double)
+    unsigned long long max_slide_bits;  // max sliding velocity squared
+    double res_norm, rhs_norm;          // last solve: ||b - A x||, ||b||
+    int npairs, npairs_tmp;
+    int error;
+    int in_predictor;
+    int iters_inner, iters_outer;       // last solve
+    int total_inner;                    // accumulated over the instance's life
+    int pad;
+};
+
+// 6x6-block sparse symmetric matrix.  Off-diagonal blocks are stored once per
+// pair (p0 < p1) in LF_DEM's packed ODBlock layout, located at (row p1, col p0);
+// row p0 uses the transpose.  Upper incidence of row k = pairs [row_ptr[k], row_ptr[k+1])
+// (pairs sorted by (p0, p1)); lower incidence = tidx[tptr[k] .. tptr[k+1]).
+struct BsrView {
+    int n;                 // block rows (particles)
+    const int* npairs;     // device count
+    const int* p0;         // [P] (strided view into PairRec is avoided: compact arrays)
+    const int* p1;
+    const int* row_ptr;    // [n+1]
+    const int* tptr;       // [n+1]
+    const int* tidx;       // [P]
+    const double* diag64;  // [n*15]
+    const float* diag32;   // [n*15]
+    const double* od64;    // [P*18]
+    const float* od32;     // [P*18]
+    const float* minv32;   // [n*36] block-Jacobi inverse
+    const double* minv64;  // [n*36]
+};
+
+struct SolverWork {
+    // FP64
+    double* r64; double* x64;
+    // FP32 Krylov vectors
+    float *r, *z, *p, *q, *w, *d;
+    // FP64 vectors for the plain form
+    double *pr, *pz, *pp, *pq;
+    double* partials;      // [max_groups * 8]
+    int max_groups;
+};
+
+struct SolveOpts {
+    double tol;            // relative residual target ||b - A x|| / ||b|| (FP64)
+    double inner_tol;      // relative residual drop per FP32 inner solve
+    int max_outer, max_inner;
+    int lanes_per_row;     // 1, 2, 4 or 8 (SpMV lanes per block row)
+    int block;             // workgroup size
+    int groups;            // workgroups of the persistent kernel (0 = auto)
+};
+
+// ---------------- launchers (lfd_kernels.hip) ----------------
+void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
+void launch_fill_double(double* p, double v, size_t n, hipStream_t s);
+void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s);
+
+// cell list
+struct CellGrid {
+    int nbox, nx, ny, nz, yz, boxed;
+    double bx, by, bz;
+    int* nb;       // [nbox*64] neighbour cells incl. self, deduplicated
+    int* nnb;      // [nbox]
+    int* nstatic;  // [nbox] count of self+static entries (moving ones follow)
+    int* ctype;    // [nbox] 0 bulk, 1 bottom, 2 top, 3 top-bottom
+    double probe_top[16][3], probe_bot[16][3];
+};
+void launch_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc, hipStream_t s);
+void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int* mem, int* cursor,
+                int* scan_tmp, hipStream_t s);
+void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, DevScalars* err, hipStream_t s);
+void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_start, const int* home_mem,
+                       const int* snap_start, const int* snap_mem, const DevScalars* sc, Consts C,
+                       int* found, int* found_cnt, int K, DevScalars* err, hipStream_t s);
+void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
+                        const int* found_cnt, int K, int* cnt, int* new_row_ptr, PairRec* new_pairs,
+                        const double4* pos, Consts C, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
+void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, Consts C, DevScalars* sc,
+                         int* keep, int pcap, int in_predictor, double dt_override, hipStream_t s);
+void launch_compact_pairs(const PairRec* in, PairRec* out, const int* keep, int* keep_scan, int n,
+                          int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, DevScalars* sc,
+                          int pcap, int* scan_tmp, hipStream_t s);
+void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
+                       int* tcnt, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
+void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
+                     const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
+                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, int pcap,
+                     DevScalars* err, hipStream_t s);
+void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float* diag32, const double* od64,
+                           float* od32, double* minv64, float* minv32, int pcap, DevScalars* err, hipStream_t s);
+void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
+                            DevScalars* sc, int in_predictor, hipStream_t s);
+void launch_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time, hipStream_t s);
+void launch_boxing(DevScalars* sc, Consts C, hipStream_t s);
+void launch_displace(double4* pos, double* vel6, const double* pred6, int n, Consts C, DevScalars* sc,
+                     CellGrid g, int* box_of, int mode, hipStream_t s);
+void launch_copy_double(double* dst, const double* src, size_t n, hipStream_t s);
+
+// solver
+int persistent_groups(const SolveOpts& o);
+hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                       const SolveOpts& o, hipStream_t s);
+hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                       const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
+void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);
+void launch_spmv32(const BsrView& A, const float* x, float* y, int lanes_per_row, hipStream_t s);
+
+}  // namespace lfd

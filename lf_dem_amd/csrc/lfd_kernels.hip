@@ -1,0 +1,1307 @@
+// lfd_kernels.hip -- HIP kernels for gfx950 (MI355X) of the Stokes-resistance
+// step: cell binning, LDS-staged neighbour search with wavefront ballot
+// compaction, pair lifecycle, deterministic 6x6-block assembly, block SpMV and
+// the Krylov solvers.  Wavefronts are 64 lanes wide.
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+#include "lfd_internal.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace lfd {
+
+static constexpr int WAVE = 64;
+
+__device__ inline void raise_error(DevScalars* sc, int bit) { atomicOr(&sc->error, bit); }
+
+__device__ inline unsigned lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ inline unsigned popc_below(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ inline void atomic_max_nonneg(unsigned long long* addr, double v)
+{
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// ------------------------------------------------------------------ fills
+__global__ void k_fill_int(int* p, int v, size_t n)
+{
+    size_t i = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_double(double* p, double v, size_t n)
+{
+    size_t i = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_copy_double(double* d, const double* s, size_t n)
+{
+    size_t i = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+void launch_fill_int(int* p, int v, size_t n, hipStream_t s)
+{
+    if (n) k_fill_int<<<(unsigned)((n+255)/256), 256, 0, s>>>(p, v, n);
+}
+void launch_fill_double(double* p, double v, size_t n, hipStream_t s)
+{
+    if (n) k_fill_double<<<(unsigned)((n+255)/256), 256, 0, s>>>(p, v, n);
+}
+void launch_copy_double(double* d, const double* src, size_t n, hipStream_t s)
+{
+    if (n) k_copy_double<<<(unsigned)((n+255)/256), 256, 0, s>>>(d, src, n);
+}
+
+// ------------------------------------------------------------------ scan
+// Exclusive scan of in[0..count) into out[0..count], out[count] = total.
+static constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T*SCAN_I;
+
+__device__ inline int block_exclusive_scan(int v, int* lds, int& total)
+{
+    // wave-inclusive scan by shuffles, then scan of the wave totals in LDS
+    unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
+    int x = v;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int y = __shfl_up(x, o, WAVE);
+        if ((int)lane >= o) x += y;
+    }
+    if (lane == WAVE-1) lds[wid] = x;
+    __syncthreads();
+    int nw = blockDim.x/WAVE;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < nw; k++) { int t = lds[k]; lds[k] = acc; acc += t; }
+        lds[nw] = acc;
+    }
+    __syncthreads();
+    int r = x-v+lds[wid];
+    total = lds[nw];
+    __syncthreads();
+    return r;
+}
+
+__global__ void k_scan_reduce(const int* in, const int* count_dev, int cap, int* sums)
+{
+    int count = count_dev ? min(*count_dev, cap) : cap;
+    int base = blockIdx.x*SCAN_TILE+threadIdx.x*SCAN_I;
+    int acc = 0;
+    for (int k = 0; k < SCAN_I; k++) { int i = base+k; if (i < count) acc += in[i]; }
+    __shared__ int lds[SCAN_T/WAVE+1];
+    int total;
+    block_exclusive_scan(acc, lds, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+__global__ void k_scan_top(int* sums, int nb)
+{
+    __shared__ int lds[SCAN_T/WAVE+1];
+    int carry = 0;
+    for (int base = 0; base < nb; base += SCAN_T) {
+        int i = base+threadIdx.x;
+        int v = i < nb ? sums[i] : 0;
+        int total;
+        int e = block_exclusive_scan(v, lds, total);
+        if (i < nb) sums[i] = e+carry;
+        carry += total;
+    }
+    if (threadIdx.x == 0) sums[nb] = carry;
+}
+__global__ void k_scan_down(const int* in, int* out, const int* count_dev, int cap, const int* sums)
+{
+    int count = count_dev ? min(*count_dev, cap) : cap;
+    int base = blockIdx.x*SCAN_TILE+threadIdx.x*SCAN_I;
+    int v[SCAN_I], acc = 0;
+    for (int k = 0; k < SCAN_I; k++) { int i = base+k; v[k] = i < count ? in[i] : 0; acc += v[k]; }
+    __shared__ int lds[SCAN_T/WAVE+1];
+    int total;
+    int e = block_exclusive_scan(acc, lds, total)+sums[blockIdx.x];
+    for (int k = 0; k < SCAN_I; k++) {
+        int i = base+k;
+        if (i < count) out[i] = e;
+        e += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[count] = sums[gridDim.x];
+}
+void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s)
+{
+    int nb = (cap+SCAN_TILE-1)/SCAN_TILE;
+    if (nb < 1) nb = 1;
+    k_scan_reduce<<<nb, SCAN_T, 0, s>>>(in, count_dev, cap, tmp);
+    k_scan_top<<<1, SCAN_T, 0, s>>>(tmp, nb);
+    k_scan_down<<<nb, SCAN_T, 0, s>>>(in, out, count_dev, cap, tmp);
+}
+
+// ------------------------------------------------------------------ cells
+__global__ void k_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    double4 q = pos[i];
+    double p[3] = {q.x, q.y, q.z};
+    int l = which_box(p, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
+    if (l < 0) { raise_error(sc, E_BOX_OOB); l = 0; }
+    box_of[i] = l;
+}
+void launch_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc, hipStream_t s)
+{
+    k_box_labels<<<(n+255)/256, 256, 0, s>>>(pos, n, g, box_of, sc);
+}
+
+__global__ void k_bin_count(const int* box_of, int n, int* count)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n) atomicAdd(&count[box_of[i]], 1);
+}
+__global__ void k_bin_scatter(const int* box_of, int n, int* cursor, int* mem)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n) mem[atomicAdd(&cursor[box_of[i]], 1)] = i;
+}
+// members of a cell in ascending particle order (the order of Box's std::set<int>)
+__global__ void k_bin_sort(const int* start, int nbox, int* mem)
+{
+    int b = blockIdx.x*blockDim.x+threadIdx.x;
+    if (b >= nbox) return;
+    int s0 = start[b], s1 = start[b+1];
+    for (int a = s0+1; a < s1; a++) {
+        int x = mem[a], c = a-1;
+        while (c >= s0 && mem[c] > x) { mem[c+1] = mem[c]; c--; }
+        mem[c+1] = x;
+    }
+}
+__global__ void k_copy_int(int* d, const int* s, int n)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int* mem, int* cursor,
+                int* scan_tmp, hipStream_t s)
+{
+    launch_fill_int(count, 0, nbox, s);
+    k_bin_count<<<(n+255)/256, 256, 0, s>>>(box_of, n, count);
+    exclusive_scan_int(count, start, nullptr, nbox, scan_tmp, s);
+    k_copy_int<<<(nbox+255)/256, 256, 0, s>>>(cursor, start, nbox);
+    k_bin_scatter<<<(n+255)/256, 256, 0, s>>>(box_of, n, cursor, mem);
+    k_bin_sort<<<(nbox+255)/256, 256, 0, s>>>(start, nbox, mem);
+}
+
+// BoxSet::updateNeighbors (BoxSet.cpp:408-442): moving neighbours of top and
+// bottom cells from the 16 probing positions, with the current LE shift.
+__global__ void k_moving_neighbors(CellGrid g, const DevScalars* sc, Consts C, DevScalars* err)
+{
+    int b = blockIdx.x*blockDim.x+threadIdx.x;
+    if (b >= g.nbox) return;
+    int t = g.ctype[b];
+    int cnt = g.nstatic[b];
+    if (t == 0 || !g.boxed) { g.nnb[b] = cnt; return; }
+    unsigned ix = b/g.yz, iy = (b%g.yz)/g.nz, iz = b%g.nz;
+    double c[3] = {g.bx*(ix+0.5), g.by*(iy+0.5), g.bz*(iz+0.5)};
+    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    int* nb = g.nb+64*b;
+    for (int side = 0; side < 2; side++) {
+        if (side == 0 && !(t == 2 || t == 3)) continue;
+        if (side == 1 && !(t == 1 || t == 3)) continue;
+        for (int k = 0; k < 16; k++) {
+            const double* pr = side == 0 ? g.probe_top[k] : g.probe_bot[k];
+            double q[3] = {c[0]+pr[0], c[1]+pr[1], c[2]+pr[2]};
+            periodize(C.L, sd, q);
+            int l = which_box(q, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
+            if (l < 0) { raise_error(err, E_BOX_OOB); continue; }
+            bool dup = false;
+            for (int m = 0; m < cnt; m++) if (nb[m] == l) { dup = true; break; }
+            if (!dup && cnt < 64) nb[cnt++] = l;
+        }
+    }
+    g.nnb[b] = cnt;
+}
+void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, DevScalars* err, hipStream_t s)
+{
+    Consts C{};
+    C.L[0] = L[0]; C.L[1] = L[1]; C.L[2] = L[2];
+    k_moving_neighbors<<<(g.nbox+255)/256, 256, 0, s>>>(g, sc, C, err);
+}
+
+// ------------------------------------------------------------------ neighbour search
+// One workgroup per home cell.  Candidates (snapshot members of the cell's
+// neighbourhood, current positions) are staged in LDS in chunks; each wave
+// takes one home particle, its 64 lanes test 64 candidates at a time, and hits
+// are compacted with a ballot + masked popcount (System::checkNewInteraction,
+// System.cpp:1328-1353, over BoxSet::neighborhood, Box.cpp:45-87).
+static constexpr int FP_BLOCK = 256, FP_CHUNK = 512;
+
+__global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restrict__ pos, CellGrid g,
+    const int* __restrict__ home_start, const int* __restrict__ home_mem, const int* __restrict__ snap_start,
+    const int* __restrict__ snap_mem, const DevScalars* sc, Consts C, int* __restrict__ found,
+    int* __restrict__ found_cnt, int K, DevScalars* err)
+{
+    __shared__ int cells[64];
+    __shared__ int coff[65];
+    __shared__ int ncell;
+    __shared__ int cidx[FP_CHUNK];
+    __shared__ double4 cpos[FP_CHUNK];
+    const int cell = blockIdx.x;
+    const int h0 = home_start[cell], nh = home_start[cell+1]-h0;
+    if (nh == 0) return;
+    const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE, nwave = FP_BLOCK/WAVE;
+    if (threadIdx.x == 0) {
+        int nc = g.nnb[cell], acc = 0;
+        for (int c = 0; c < nc; c++) {
+            int b = g.nb[64*cell+c];
+            cells[c] = b;
+            coff[c] = acc;
+            acc += snap_start[b+1]-snap_start[b];
+        }
+        coff[nc] = acc;
+        ncell = nc;
+    }
+    __syncthreads();
+    const int nc = ncell;
+    const int total = coff[nc];
+    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    for (int base = 0; base < total; base += FP_CHUNK) {
+        int nchunk = min(FP_CHUNK, total-base);
+        for (int t = threadIdx.x; t < nchunk; t += FP_BLOCK) {
+            int gi = base+t, c = 0;
+            while (c+1 < nc && coff[c+1] <= gi) c++;
+            int j = snap_mem[snap_start[cells[c]]+(gi-coff[c])];
+            cidx[t] = j;
+            cpos[t] = pos[j];
+        }
+        __syncthreads();
+        for (int h = wid; h < nh; h += nwave) {
+            const int i = home_mem[h0+h];
+            const double4 xi = pos[i];
+            int cnt = found_cnt[i];
+            for (int t0 = 0; t0 < nchunk; t0 += WAVE) {
+                int t = t0+(int)lane;
+                bool hit = false;
+                int j = -1;
+                if (t < nchunk) {
+                    j = cidx[t];
+                    if (j > i) {
+                        double4 xj = cpos[t];
+                        double d[3] = {xj.x-xi.x, xj.y-xi.y, xj.z-xi.z};
+                        periodize_diff(C.L, C.half, sd, d);
+                        double sq = d[0]*d[0]+d[1]*d[1]+d[2]*d[2];
+                        double range = lub_range(xi.w, xj.w, C.lub_max_gap);
+                        hit = sq < range*range;
+                    }
+                }
+                unsigned long long mask = __ballot(hit);
+                if (hit) {
+                    int slot = cnt+(int)popc_below(mask);
+                    if (slot < K) found[(size_t)i*K+slot] = j;
+                }
+                cnt += __popcll(mask);
+            }
+            if (lane == 0) {
+                found_cnt[i] = cnt;
+                if (cnt > K) raise_error(err, E_FOUND_OVERFLOW);
+            }
+        }
+        __syncthreads();
+    }
+}
+void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_start, const int* home_mem,
+                       const int* snap_start, const int* snap_mem, const DevScalars* sc, Consts C,
+                       int* found, int* found_cnt, int K, DevScalars* err, hipStream_t s)
+{
+    launch_fill_int(found_cnt, 0, n, s);
+    k_find_pairs<<<g.nbox, FP_BLOCK, 0, s>>>(pos, g, home_start, home_mem, snap_start, snap_mem, sc, C,
+                                             found, found_cnt, K, err);
+}
+
+// ------------------------------------------------------------------ merge new pairs
+// Row i: union of the existing pairs (i, j) and the pairs found this pass.
+__global__ void k_merge_count(int n, const PairRec* old_pairs, const int* old_row_ptr, int* found,
+                              const int* found_cnt, int K, int* cnt)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    int nf = min(found_cnt[i], K);
+    int* f = found+(size_t)i*K;
+    for (int a = 1; a < nf; a++) {            // insertion sort of the found partners
+        int x = f[a], c = a-1;
+        while (c >= 0 && f[c] > x) { f[c+1] = f[c]; c--; }
+        f[c+1] = x;
+    }
+    int o = old_row_ptr[i], oe = old_row_ptr[i+1], a = 0, m = 0;
+    while (o < oe || a < nf) {
+        if (a >= nf) { o++; }
+        else if (o >= oe) { a++; }
+        else {
+            int jo = old_pairs[o].p1, jf = f[a];
+            if (jo == jf) { o++; a++; }
+            else if (jo < jf) o++;
+            else a++;
+        }
+        m++;
+    }
+    cnt[i] = m;
+}
+__global__ void k_merge_write(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
+                              const int* found_cnt, int K, const int* new_row_ptr, PairRec* new_pairs,
+                              const double4* pos, Consts C, const DevScalars* sc, int pcap, DevScalars* err)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    int nf = min(found_cnt[i], K);
+    const int* f = found+(size_t)i*K;
+    int o = old_row_ptr[i], oe = old_row_ptr[i+1], a = 0;
+    int w = new_row_ptr[i];
+    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    double4 xi = pos[i];
+    while (o < oe || a < nf) {
+        int take_old;
+        if (a >= nf) take_old = 1;
+        else if (o >= oe) take_old = 0;
+        else {
+            int jo = old_pairs[o].p1, jf = f[a];
+            if (jo == jf) { take_old = 1; a++; }
+            else take_old = jo < jf;
+        }
+        if (w >= pcap) { raise_error(err, E_PAIR_OVERFLOW); return; }
+        if (take_old) {
+            new_pairs[w] = old_pairs[o];
+            o++;
+        } else {
+            int j = f[a++];
+            double4 xj = pos[j];
+            double pi[3] = {xi.x, xi.y, xi.z}, pj[3] = {xj.x, xj.y, xj.z};
+            PairRec q;
+            pair_create(q, i, j, pi, pj, xi.w, xj.w, C, sd);
+            new_pairs[w] = q;
+        }
+        w++;
+    }
+}
+__global__ void k_set_npairs_from(DevScalars* sc, const int* total, int pcap)
+{
+    int t = *total;
+    if (t > pcap) { atomicOr(&sc->error, E_PAIR_OVERFLOW); t = pcap; }
+    sc->npairs = t;
+}
+void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
+                        const int* found_cnt, int K, int* cnt, int* new_row_ptr, PairRec* new_pairs,
+                        const double4* pos, Consts C, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
+{
+    k_merge_count<<<(n+255)/256, 256, 0, s>>>(n, old_pairs, old_row_ptr, const_cast<int*>(found), found_cnt, K, cnt);
+    exclusive_scan_int(cnt, new_row_ptr, nullptr, n, scan_tmp, s);
+    k_merge_write<<<(n+255)/256, 256, 0, s>>>(n, old_pairs, old_row_ptr, found, found_cnt, K, new_row_ptr,
+                                              new_pairs, pos, C, sc, pcap, sc);
+    k_set_npairs_from<<<1, 1, 0, s>>>(sc, new_row_ptr+n, pcap);
+}
+
+// ------------------------------------------------------------------ pair update
+// System::updateInteractions (System.cpp:1375-1403) without the swap-with-last
+// removal: every pair is updated once and removals are compacted afterwards.
+__global__ void k_update_pairs(PairRec* pairs, const double4* __restrict__ pos, const double* __restrict__ vel6,
+                               Consts C, DevScalars* sc, int* keep, int pcap, int in_predictor, double dt_override)
+{
+    int k = blockIdx.x*blockDim.x+threadIdx.x;
+    int np = sc->npairs;
+    if (k >= np || k >= pcap) return;
+    PairRec q = pairs[k];
+    double4 x0 = pos[q.p0], x1 = pos[q.p1];
+    double a[3] = {x0.x, x0.y, x0.z}, b[3] = {x1.x, x1.y, x1.z};
+    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    double dt = dt_override >= 0 ? dt_override : sc->dt;
+    int drop = pair_update(q, a, b, x0.w, x1.w, vel6+6*(size_t)q.p0, vel6+6*(size_t)q.p1, C, sd, dt,
+                           in_predictor);
+    if (C.friction && (q.flags & F_CONTACT)) atomic_max_nonneg(&sc->max_slide_bits, q.rsv_sq);
+    pairs[k] = q;
+    keep[k] = drop ? 0 : 1;
+}
+__global__ void k_reset_slide(DevScalars* sc) { sc->max_slide_bits = 0; }
+void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, Consts C, DevScalars* sc,
+                         int* keep, int pcap, int in_predictor, double dt_override, hipStream_t s)
+{
+    k_reset_slide<<<1, 1, 0, s>>>(sc);
+    k_update_pairs<<<(pcap+255)/256, 256, 0, s>>>(pairs, pos, vel6, C, sc, keep, pcap, in_predictor, dt_override);
+}
+
+__global__ void k_compact(const PairRec* in, PairRec* out, const int* keep, const int* ks, const DevScalars* sc)
+{
+    int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= sc->npairs) return;
+    if (keep[k]) out[ks[k]] = in[k];
+}
+__global__ void k_set_npairs(DevScalars* sc, const int* ks)
+{
+    sc->npairs = ks[sc->npairs];
+}
+void launch_compact_pairs(const PairRec* in, PairRec* out, const int* keep, int* keep_scan, int n,
+                          int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, DevScalars* sc,
+                          int pcap, int* scan_tmp, hipStream_t s)
+{
+    exclusive_scan_int(keep, keep_scan, &sc->npairs, pcap, scan_tmp, s);
+    k_compact<<<(pcap+255)/256, 256, 0, s>>>(in, out, keep, keep_scan, sc);
+    k_set_npairs<<<1, 1, 0, s>>>(sc, keep_scan);
+    launch_pair_index(out, n, row_ptr, p0, p1, tptr, tidx, tcnt, sc, pcap, scan_tmp, s);
+}
+
+// ------------------------------------------------------------------ pair index
+__global__ void k_pair_arrays(const PairRec* pairs, int* p0, int* p1, int* tcnt, const DevScalars* sc)
+{
+    int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= sc->npairs) return;
+    int a = pairs[k].p0, b = pairs[k].p1;
+    p0[k] = a; p1[k] = b;
+    atomicAdd(&tcnt[b], 1);
+}
+__global__ void k_row_ptr(const int* p0, int n, int* row_ptr, const DevScalars* sc)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i > n) return;
+    int lo = 0, hi = sc->npairs;          // lower_bound of i in the sorted p0
+    while (lo < hi) { int m = (lo+hi) >> 1; if (p0[m] < i) lo = m+1; else hi = m; }
+    row_ptr[i] = lo;
+}
+__global__ void k_tfill(const int* p1, int* cursor, int* tidx, const DevScalars* sc)
+{
+    int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= sc->npairs) return;
+    tidx[atomicAdd(&cursor[p1[k]], 1)] = k;
+}
+// lower incidence lists in ascending pair index, i.e. ascending p0
+__global__ void k_tsort(const int* tptr, int n, int* tidx)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    int s0 = tptr[i], s1 = tptr[i+1];
+    for (int a = s0+1; a < s1; a++) {
+        int x = tidx[a], c = a-1;
+        while (c >= s0 && tidx[c] > x) { tidx[c+1] = tidx[c]; c--; }
+        tidx[c+1] = x;
+    }
+}
+void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
+                       int* tcnt, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
+{
+    launch_fill_int(tcnt, 0, n, s);
+    k_pair_arrays<<<(pcap+255)/256, 256, 0, s>>>(pairs, p0, p1, tcnt, sc);
+    k_row_ptr<<<(n+1+255)/256, 256, 0, s>>>(p0, n, row_ptr, sc);
+    exclusive_scan_int(tcnt, tptr, nullptr, n, scan_tmp, s);
+    k_copy_int<<<(n+255)/256, 256, 0, s>>>(tcnt, tptr, n);
+    k_tfill<<<(pcap+255)/256, 256, 0, s>>>(p1, tcnt, tidx, sc);
+    k_tsort<<<(n+255)/256, 256, 0, s>>>(tptr, n, tidx);
+}
+
+// ------------------------------------------------------------------ assembly
+__device__ inline void dense_from_dblock(const double* b, double M[6][6])
+{
+    // lower triangle per StokesSolver::insertDBlockValues (StokesSolver.cpp:154-184)
+    double L[6][6] = {};
+    L[0][0] = b[0]; L[1][0] = b[1]; L[2][0] = b[2]; L[4][0] = b[3]; L[5][0] = b[4];
+    L[1][1] = b[5]; L[2][1] = b[6]; L[3][1] = -b[3]; L[5][1] = b[7];
+    L[2][2] = b[8]; L[3][2] = -b[4]; L[4][2] = -b[7];
+    L[3][3] = b[9]; L[4][3] = b[10]; L[5][3] = b[11];
+    L[4][4] = b[12]; L[5][4] = b[13];
+    L[5][5] = b[14];
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) M[r][c] = r >= c ? L[r][c] : L[c][r];
+}
+
+// inverse of an SPD 6x6 block through its Cholesky factor; returns false if not SPD
+__device__ inline bool spd_inverse6(double M[6][6], double Inv[6][6])
+{
+    double Lc[6][6] = {};
+    for (int j = 0; j < 6; j++) {
+        double d = M[j][j];
+        for (int k = 0; k < j; k++) d -= Lc[j][k]*Lc[j][k];
+        if (!(d > 0)) return false;
+        d = sqrt(d);
+        Lc[j][j] = d;
+        for (int i = j+1; i < 6; i++) {
+            double v = M[i][j];
+            for (int k = 0; k < j; k++) v -= Lc[i][k]*Lc[j][k];
+            Lc[i][j] = v/d;
+        }
+    }
+    // inverse column by column: solve L L^T x = e_c
+    for (int c = 0; c < 6; c++) {
+        double y[6];
+        for (int r = 0; r < 6; r++) {
+            double v = (r == c) ? 1.0 : 0.0;
+            for (int k = 0; k < r; k++) v -= Lc[r][k]*y[k];
+            y[r] = v/Lc[r][r];
+        }
+        for (int r = 5; r >= 0; r--) {
+            double v = y[r];
+            for (int k = r+1; k < 6; k++) v -= Lc[k][r]*y[k];
+            y[r] = v/Lc[r][r];
+        }
+        for (int r = 0; r < 6; r++) Inv[r][c] = y[r];
+    }
+    return true;
+}
+
+__device__ inline void uinf(const Consts& C, const double* x, double* u)
+{
+    // System::computeUInf (System.cpp:2357-2367)
+    double e[3], w[3];
+    symdot(C.E, x, e);
+    cross3(C.omega, x, w);
+    u[0] = e[0]+w[0]; u[1] = e[1]+w[1]; u[2] = e[2]+w[2];
+}
+
+// Right-hand-side pieces of one pair for one of its particles (side 0 = p0,
+// side 1 = p1), accumulated in LF_DEM's component order: contact
+// (Contact.cpp:411-437), dashpot R_FU^dash.U_inf (ContactDashpot.cpp:342-441),
+// hydro R_FE:E_inf (Lubrication.cpp:332-403), repulsion (RepulsiveForce.cpp:101-105).
+struct RhsAcc { double Fc[3], Tc[3], Fd[3], Td[3], Fh[3], Th[3], Fr[3]; };
+
+__device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, double a1, const double* x0,
+                                     const double* x1, const Consts& C, RhsAcc& R)
+{
+    const double* n = q.n;
+    if (q.flags & F_CONTACT) {
+        if (side == 0) for (int k = 0; k < 3; k++) R.Fc[k] += q.f_total[k];
+        else for (int k = 0; k < 3; k++) R.Fc[k] -= q.f_total[k];
+        if (pair_state(q.flags) >= 2) {
+            double t[3];
+            cross3(n, q.f_tan, t);
+            double a = side == 0 ? a0 : a1;
+            for (int k = 0; k < 3; k++) R.Tc[k] += a*t[k];
+        }
+        if (C.shear_rate != 0) {
+            DashCoef c;
+            dashpot_coeffs(a0, a1, C.normal_coeff, C.tangential_coeff, c);
+            double vi[3], vj[3], t[3], f[3];
+            uinf(C, x0, vi);
+            uinf(C, x1, vj);
+            double zo = (double)q.zoff;
+            for (int k = 0; k < 3; k++) vj[k] += zo*C.vel_diff[k];
+            for (int k = 0; k < 3; k++) t[k] = c.XA[0]*vi[k]+c.XA[1]*vj[k];
+            double d = -dot3(t, n);
+            for (int k = 0; k < 3; k++) f[k] = d*n[k];
+            if (C.tangential_coeff > 0) {
+                double di = dot3(n, vi), dj = dot3(n, vj), ci[3];
+                for (int k = 0; k < 3; k++) {
+                    double av = vi[k]-n[k]*di, bv = vj[k]-n[k]*dj;
+                    f[k] += -c.YA[0]*av-c.YA[1]*bv;
+                }
+                cross3(n, C.omega, ci);
+                for (int k = 0; k < 3; k++) f[k] += -c.YB[0]*ci[k]-c.YB[2]*ci[k];
+            }
+            if (side == 0) for (int k = 0; k < 3; k++) R.Fd[k] += f[k];
+            else for (int k = 0; k < 3; k++) R.Fd[k] -= f[k];
+            if (C.tangential_coeff > 0) {
+                double tq[3], t0[3];
+                cross3(n, f, tq);
+                for (int k = 0; k < 3; k++) t0[k] = a0*tq[k];
+                if (side == 0) for (int k = 0; k < 3; k++) R.Td[k] += t0[k];
+                else {
+                    double ratio = a1/a0;
+                    for (int k = 0; k < 3; k++) R.Td[k] += ratio*t0[k];
+                }
+            }
+        }
+    }
+    if ((q.flags & F_LUB) && C.shear_rate != 0) {
+        LubConst c;
+        lub_constants(a0, a1, c);
+        double lc, llc;
+        lub_coeffs(q.gap, C.lub_reduce, lc, llc);
+        double XG[4], YG[4], YH[4];
+        for (int j = 0; j < 4; j++) { XG[j] = c.XG[j]*lc; YG[j] = c.YG[j]*llc; YH[j] = c.YH[j]*llc; }
+        double En[3];
+        symdot(C.E, n, En);
+        double nnE = dot3(n, En);
+        if (C.tangential) {
+            double YGa = side == 0 ? YG[0]+YG[2] : YG[1]+YG[3];
+            double cGE = side == 0 ? (XG[0]+XG[2]-2*YGa)*nnE : (XG[1]+XG[3]-2*YGa)*nnE;
+            double cHE = side == 0 ? YH[0]+YH[2] : YH[1]+YH[3];
+            double gy = YGa*2, x[3];
+            for (int k = 0; k < 3; k++) R.Fh[k] += cGE*n[k]+gy*En[k];
+            cross3(En, n, x);
+            double h = -2*cHE;
+            for (int k = 0; k < 3; k++) R.Th[k] += h*x[k];
+        } else {
+            double cGE = side == 0 ? (XG[0]+XG[2])*nnE : (XG[1]+XG[3])*nnE;
+            for (int k = 0; k < 3; k++) R.Fh[k] += cGE*n[k];
+        }
+    }
+    if (C.repulsive) {
+        for (int k = 0; k < 3; k++) {
+            double rf = -q.rep_norm*n[k];
+            if (side == 0) R.Fr[k] += rf; else R.Fr[k] -= rf;
+        }
+    }
+}
+
+// Off-diagonal blocks, one thread per pair.
+__global__ void k_assemble_od(const PairRec* __restrict__ pairs, const double4* __restrict__ pos, Consts C,
+                              const DevScalars* sc, double* __restrict__ od64, float* __restrict__ od32, int pcap)
+{
+    int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= sc->npairs || k >= pcap) return;
+    PairRec q = pairs[k];
+    double db0[15], db1[15], od[18];
+    pair_blocks(q, pos[q.p0].w, pos[q.p1].w, C, db0, db1, od);
+    for (int m = 0; m < 18; m++) { od64[18*(size_t)k+m] = od[m]; od32[18*(size_t)k+m] = (float)od[m]; }
+}
+
+// Diagonal blocks (drag seed, System.cpp:558-575, then pair blocks in
+// ascending partner order, System.cpp:1459-1471), block-Jacobi inverse and
+// the right-hand side (StokesSolver.cpp:513-549), one thread per particle.
+__global__ void k_assemble_diag_rhs(const PairRec* __restrict__ pairs, const double4* __restrict__ pos, int n,
+                                    Consts C, const int* __restrict__ row_ptr, const int* __restrict__ tptr,
+                                    const int* __restrict__ tidx, double* __restrict__ diag64,
+                                    float* __restrict__ diag32, double* __restrict__ minv64, float* __restrict__ minv32,
+                                    double* __restrict__ rhs, DevScalars* err)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    double4 xi4 = pos[i];
+    double a = xi4.w;
+    double D[15];
+    for (int m = 0; m < 15; m++) D[m] = 0;
+    double FU = C.sd_coeff*a;
+    double TW = C.sd_coeff*(a*a*a)*4.0/3;
+    D[0] = FU; D[5] = FU; D[8] = FU; D[9] = TW; D[12] = TW; D[14] = TW;
+    RhsAcc R;
+    for (int k = 0; k < 3; k++) {
+        R.Fc[k] = 0; R.Tc[k] = 0; R.Fd[k] = 0; R.Td[k] = 0; R.Fh[k] = 0; R.Th[k] = 0; R.Fr[k] = 0;
+    }
+    double xi[3] = {xi4.x, xi4.y, xi4.z};
+    // pairs (j, i), j < i: this particle is p1
+    for (int t = tptr[i]; t < tptr[i+1]; t++) {
+        const PairRec q = pairs[tidx[t]];
+        double4 x04 = pos[q.p0];
+        double x0[3] = {x04.x, x04.y, x04.z};
+        double db0[15], db1[15], od[18];
+        if (pair_blocks(q, x04.w, a, C, db0, db1, od)) for (int m = 0; m < 15; m++) D[m] += db1[m];
+        pair_rhs_side(q, 1, x04.w, a, x0, xi, C, R);
+    }
+    // pairs (i, j), j > i: this particle is p0
+    for (int k = row_ptr[i]; k < row_ptr[i+1]; k++) {
+        const PairRec q = pairs[k];
+        double4 x14 = pos[q.p1];
+        double x1[3] = {x14.x, x14.y, x14.z};
+        double db0[15], db1[15], od[18];
+        if (pair_blocks(q, a, x14.w, C, db0, db1, od)) for (int m = 0; m < 15; m++) D[m] += db0[m];
+        pair_rhs_side(q, 0, a, x14.w, xi, x1, C, R);
+    }
+    for (int m = 0; m < 15; m++) { diag64[15*(size_t)i+m] = D[m]; diag32[15*(size_t)i+m] = (float)D[m]; }
+    double M[6][6], Inv[6][6];
+    dense_from_dblock(D, M);
+    if (!spd_inverse6(M, Inv)) raise_error(err, E_NOT_SPD);
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) {
+            minv64[36*(size_t)i+6*r+c] = Inv[r][c];
+            minv32[36*(size_t)i+6*r+c] = (float)Inv[r][c];
+        }
+    double F[3], T[3];
+    for (int k = 0; k < 3; k++) {
+        F[k] = 0; T[k] = 0;
+        F[k] += R.Fc[k];
+        if (C.friction) T[k] += R.Tc[k];
+        F[k] += R.Fd[k];
+        T[k] += R.Td[k];
+        F[k] += R.Fh[k];
+        if (C.tangential) T[k] += R.Th[k];
+        if (C.repulsive) F[k] += R.Fr[k];
+    }
+    for (int k = 0; k < 3; k++) { rhs[6*(size_t)i+k] = F[k]; rhs[6*(size_t)i+3+k] = T[k]; }
+}
+void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
+                     const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
+                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, int pcap,
+                     DevScalars* err, hipStream_t s)
+{
+    k_assemble_od<<<(pcap+255)/256, 256, 0, s>>>(pairs, pos, C, sc, od64, od32, pcap);
+    k_assemble_diag_rhs<<<(n+127)/128, 128, 0, s>>>(pairs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
+                                                    minv64, minv32, rhs, err);
+}
+
+// Matrix handed over through the StokesSolver staging protocol: FP32 copy and
+// block-Jacobi inverse of host-provided blocks.
+__global__ void k_prepare_od(const double* od64, float* od32, const int* npairs, int pcap)
+{
+    size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    size_t m = (size_t)min(*npairs, pcap)*18;
+    if (k < m) od32[k] = (float)od64[k];
+}
+__global__ void k_prepare_diag(int n, const double* diag64, float* diag32, double* minv64, float* minv32,
+                               DevScalars* err)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    for (int m = 0; m < 15; m++) diag32[15*(size_t)i+m] = (float)diag64[15*(size_t)i+m];
+    double M[6][6], Inv[6][6];
+    dense_from_dblock(diag64+15*(size_t)i, M);
+    if (!spd_inverse6(M, Inv)) raise_error(err, E_NOT_SPD);
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) {
+            minv64[36*(size_t)i+6*r+c] = Inv[r][c];
+            minv32[36*(size_t)i+6*r+c] = (float)Inv[r][c];
+        }
+}
+void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float* diag32, const double* od64,
+                           float* od32, double* minv64, float* minv32, int pcap, DevScalars* err, hipStream_t s)
+{
+    size_t m = (size_t)pcap*18;
+    if (m) k_prepare_od<<<(unsigned)((m+255)/256), 256, 0, s>>>(od64, od32, npairs, pcap);
+    k_prepare_diag<<<(n+127)/128, 128, 0, s>>>(n, diag64, diag32, minv64, minv32, err);
+}
+
+// ------------------------------------------------------------------ velocities, time step, moves
+__global__ void k_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
+                                  DevScalars* sc, int in_predictor)
+{
+    // System::computeMaxNAVelocity + adjustVelocityPeriodicBoundary (System.cpp:1887-1902, 2369-2392)
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    double4 x4 = pos[i];
+    double x[3] = {x4.x, x4.y, x4.z}, u[3];
+    uinf(C, x, u);
+    const double* v = na6+6*(size_t)i;
+    if (in_predictor) atomic_max_nonneg(&sc->max_vel_bits, v[0]*v[0]+v[1]*v[1]+v[2]*v[2]);
+    for (int k = 0; k < 3; k++) {
+        vel6[6*(size_t)i+k] = v[k]+u[k];
+        vel6[6*(size_t)i+3+k] = v[3+k]+C.omega[k];
+    }
+}
+__global__ void k_reset_maxvel(DevScalars* sc) { sc->max_vel_bits = 0; }
+void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
+                            DevScalars* sc, int in_predictor, hipStream_t s)
+{
+    if (in_predictor) k_reset_maxvel<<<1, 1, 0, s>>>(sc);
+    k_finish_velocity<<<(n+255)/256, 256, 0, s>>>(n, pos, na6, vel6, C, sc, in_predictor);
+}
+
+// System::adaptTimeStep (System.cpp:1013-1066), time_end disabled
+__global__ void k_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time)
+{
+    double rate = P.shear_rate;
+    double mv = sqrt(bits_to_double(sc->max_vel_bits));
+    double ms = sqrt(bits_to_double(sc->max_slide_bits));
+    double dt;
+    if (mv > 0 || ms > 0) dt = mv > ms ? P.disp_max/mv : P.disp_max/ms;
+    else dt = P.disp_max/rate;
+    if (dt*rate > P.disp_max) dt = P.disp_max/rate;
+    if (P.dt_max > 0 && dt > P.dt_max) dt = P.dt_max;
+    if (P.dt_min > 0 && dt < P.dt_min) dt = P.dt_min;
+    if (strain_end >= 0) {
+        if (fabs(dt*rate) > strain_end-sc->strain) dt = fabs((strain_end-sc->strain)/rate);
+    }
+    sc->dt = dt;
+    if (advance_time) sc->time += dt;
+}
+void launch_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time, hipStream_t s)
+{
+    k_adapt_dt<<<1, 1, 0, s>>>(sc, P, strain_end, advance_time);
+}
+
+// System::timeStepBoxing (System.cpp:672-717), simple shear
+__global__ void k_boxing(DevScalars* sc, Consts C)
+{
+    double dt = sc->dt;
+    sc->strain += C.shear_rate*dt;
+    double z[3] = {0, 0, 1}, e[3];
+    symdot(C.E, z, e);
+    double inc[3] = {2*e[0]*dt, 2*e[1]*dt, 2*e[2]*dt};
+    for (int k = 0; k < 3; k++) sc->shear_disp[k] += inc[k]*C.L[2];
+    int m = (int)(sc->shear_disp[0]/C.L[0]);
+    if (sc->shear_disp[0] < 0) m--;
+    sc->shear_disp[0] = sc->shear_disp[0]-m*C.L[0];
+    m = (int)(sc->shear_disp[1]/C.L[1]);
+    if (sc->shear_disp[1] < 0) m--;
+    sc->shear_disp[1] = sc->shear_disp[1]-m*C.L[1];
+}
+void launch_boxing(DevScalars* sc, Consts C, hipStream_t s) { k_boxing<<<1, 1, 0, s>>>(sc, C); }
+
+// System::displacement (System.cpp:2418-2443).  mode 0: predictor move by
+// U dt; mode 1: corrector (velocity average, System.cpp:1164-1187).
+__global__ void k_displace(double4* pos, double* vel6, const double* pred6, int n, Consts C, DevScalars* sc,
+                           CellGrid g, int* box_of, int mode)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= n) return;
+    double dt = sc->dt;
+    double* v = vel6+6*(size_t)i;
+    double dr[3];
+    if (mode == 0) {
+        for (int k = 0; k < 3; k++) dr[k] = v[k]*dt;
+    } else {
+        const double* vp = pred6+6*(size_t)i;
+        for (int k = 0; k < 6; k++) v[k] = 0.5*(v[k]+vp[k]);
+        for (int k = 0; k < 3; k++) dr[k] = (v[k]-vp[k])*dt;
+    }
+    double4 x4 = pos[i];
+    double x[3] = {x4.x+dr[0], x4.y+dr[1], x4.z+dr[2]};
+    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    int zs = periodize(C.L, sd, x);
+    if (zs != 0) {
+        double f = (double)zs;
+        for (int k = 0; k < 3; k++) v[k] += f*C.vel_diff[k];
+    }
+    pos[i] = make_double4(x[0], x[1], x[2], x4.w);
+    int l = which_box(x, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
+    if (l < 0) { raise_error(sc, E_BOX_OOB); l = 0; }
+    box_of[i] = l;
+}
+void launch_displace(double4* pos, double* vel6, const double* pred6, int n, Consts C, DevScalars* sc,
+                     CellGrid g, int* box_of, int mode, hipStream_t s)
+{
+    k_displace<<<(n+255)/256, 256, 0, s>>>(pos, vel6, pred6, n, C, sc, g, box_of, mode);
+}
+
+// ------------------------------------------------------------------ block SpMV
+// y = D x, y = M x, y = M^T x for LF_DEM's packed blocks (dense layout as in
+// StokesSolver::insertDBlockValues / insertODBlockValues).
+template <typename T, typename A>
+__device__ inline void dmul(const T* d, const A* x, A* y)
+{
+    y[0] += (A)d[0]*x[0]+(A)d[1]*x[1]+(A)d[2]*x[2]+(A)d[3]*x[4]+(A)d[4]*x[5];
+    y[1] += (A)d[1]*x[0]+(A)d[5]*x[1]+(A)d[6]*x[2]-(A)d[3]*x[3]+(A)d[7]*x[5];
+    y[2] += (A)d[2]*x[0]+(A)d[6]*x[1]+(A)d[8]*x[2]-(A)d[4]*x[3]-(A)d[7]*x[4];
+    y[3] += -(A)d[3]*x[1]-(A)d[4]*x[2]+(A)d[9]*x[3]+(A)d[10]*x[4]+(A)d[11]*x[5];
+    y[4] += (A)d[3]*x[0]-(A)d[7]*x[2]+(A)d[10]*x[3]+(A)d[12]*x[4]+(A)d[13]*x[5];
+    y[5] += (A)d[4]*x[0]+(A)d[7]*x[1]+(A)d[11]*x[3]+(A)d[13]*x[4]+(A)d[14]*x[5];
+}
+template <typename T, typename A>
+__device__ inline void omul(const T* b, const A* x, A* y)
+{
+    y[0] += (A)b[0]*x[0]+(A)b[1]*x[1]+(A)b[2]*x[2]-(A)b[9]*x[4]-(A)b[10]*x[5];
+    y[1] += (A)b[1]*x[0]+(A)b[5]*x[1]+(A)b[6]*x[2]+(A)b[9]*x[3]-(A)b[14]*x[5];
+    y[2] += (A)b[2]*x[0]+(A)b[6]*x[1]+(A)b[8]*x[2]+(A)b[10]*x[3]+(A)b[14]*x[4];
+    y[3] += -(A)b[3]*x[1]-(A)b[4]*x[2]+(A)b[11]*x[3]+(A)b[12]*x[4]+(A)b[13]*x[5];
+    y[4] += (A)b[3]*x[0]-(A)b[7]*x[2]+(A)b[12]*x[3]+(A)b[15]*x[4]+(A)b[16]*x[5];
+    y[5] += (A)b[4]*x[0]+(A)b[7]*x[1]+(A)b[13]*x[3]+(A)b[16]*x[4]+(A)b[17]*x[5];
+}
+template <typename T, typename A>
+__device__ inline void omul_t(const T* b, const A* x, A* y)
+{
+    y[0] += (A)b[0]*x[0]+(A)b[1]*x[1]+(A)b[2]*x[2]+(A)b[3]*x[4]+(A)b[4]*x[5];
+    y[1] += (A)b[1]*x[0]+(A)b[5]*x[1]+(A)b[6]*x[2]-(A)b[3]*x[3]+(A)b[7]*x[5];
+    y[2] += (A)b[2]*x[0]+(A)b[6]*x[1]+(A)b[8]*x[2]-(A)b[4]*x[3]-(A)b[7]*x[4];
+    y[3] += (A)b[9]*x[1]+(A)b[10]*x[2]+(A)b[11]*x[3]+(A)b[12]*x[4]+(A)b[13]*x[5];
+    y[4] += -(A)b[9]*x[0]+(A)b[14]*x[2]+(A)b[12]*x[3]+(A)b[15]*x[4]+(A)b[16]*x[5];
+    y[5] += -(A)b[10]*x[0]-(A)b[14]*x[1]+(A)b[13]*x[3]+(A)b[16]*x[4]+(A)b[17]*x[5];
+}
+
+// Row i of A x using LPR lanes: the lanes split row i's incident blocks and
+// reduce with cross-lane shuffles.  Every lane of the group ends with y.
+template <typename T, typename V, int LPR>
+__device__ inline void spmv_row(const BsrView& A, const T* od, const T* diag, const V* x, int i, int sub, V* y)
+{
+    for (int c = 0; c < 6; c++) y[c] = 0;
+    const int u0 = A.row_ptr[i], u1 = A.row_ptr[i+1];
+    const int l0 = A.tptr[i], l1 = A.tptr[i+1];
+    const int nl = l1-l0, nt = nl+(u1-u0);
+    for (int e = sub; e < nt; e += LPR) {
+        if (e < nl) {
+            int k = A.tidx[l0+e];
+            int j = A.p0[k];
+            V xj[6];
+            for (int c = 0; c < 6; c++) xj[c] = x[6*(size_t)j+c];
+            omul<T, V>(od+18*(size_t)k, xj, y);       // block (row i = p1, col p0)
+        } else {
+            int k = u0+(e-nl);
+            int j = A.p1[k];
+            V xj[6];
+            for (int c = 0; c < 6; c++) xj[c] = x[6*(size_t)j+c];
+            omul_t<T, V>(od+18*(size_t)k, xj, y);     // transpose (row i = p0, col p1)
+        }
+    }
+    if (sub == 0 && diag) {
+        V xi[6];
+        for (int c = 0; c < 6; c++) xi[c] = x[6*(size_t)i+c];
+        dmul<T, V>(diag, xi, y);
+    }
+    for (int o = 1; o < LPR; o <<= 1)
+        for (int c = 0; c < 6; c++) y[c] += __shfl_xor(y[c], o, LPR);
+}
+
+template <int LPR>
+__global__ void k_spmv64(BsrView A, const double* x, double* y)
+{
+    int g = blockIdx.x*blockDim.x+threadIdx.x;
+    int i = g/LPR, sub = g%LPR;
+    if (i >= A.n) return;
+    double v[6];
+    spmv_row<double, double, LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, sub, v);
+    if (sub == 0) for (int c = 0; c < 6; c++) y[6*(size_t)i+c] = v[c];
+}
+void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s)
+{
+    int th = A.n*4;
+    k_spmv64<4><<<(th+255)/256, 256, 0, s>>>(A, x, y);
+}
+// FP32 SpMV with the FP32 diagonal taken from the FP64 one on the fly
+template <int LPR>
+__global__ void k_spmv32(BsrView A, const float* x, float* y)
+{
+    int g = blockIdx.x*blockDim.x+threadIdx.x;
+    int i = g/LPR, sub = g%LPR;
+    if (i >= A.n) return;
+    float v[6];
+    spmv_row<float, float, LPR>(A, A.od32, (const float*)nullptr, x, i, sub, v);
+    if (sub == 0) {
+        float xi[6];
+        for (int c = 0; c < 6; c++) xi[c] = x[6*(size_t)i+c];
+        dmul<float, float>(A.diag32+15*(size_t)i, xi, v);
+        for (int c = 0; c < 6; c++) y[6*(size_t)i+c] = v[c];
+    }
+}
+void launch_spmv32(const BsrView& A, const float* x, float* y, int lpr, hipStream_t s)
+{
+    int th = A.n*lpr;
+    switch (lpr) {
+    case 1: k_spmv32<1><<<(th+255)/256, 256, 0, s>>>(A, x, y); break;
+    case 2: k_spmv32<2><<<(th+255)/256, 256, 0, s>>>(A, x, y); break;
+    case 8: k_spmv32<8><<<(th+255)/256, 256, 0, s>>>(A, x, y); break;
+    default: k_spmv32<4><<<(th+255)/256, 256, 0, s>>>(A, x, y); break;
+    }
+}
+
+// ------------------------------------------------------------------ persistent mixed-precision solver
+// FP64 iterative refinement around an FP32 block-Jacobi PCG, in one
+// cooperative launch.  Per inner iteration two grid barriers:
+//   phase A: d += a p, r -= a q, z = M^-1 r, partial (r.z, r.r)
+//   phase B: w = A z, p = z + b p, q = w + b q, partial p.q   (q = A p without a second SpMV)
+// Partial sums are reduced by every workgroup in the same fixed order, so all
+// workgroups take identical branch decisions.
+static constexpr int PS_BLOCK = 256;
+
+__device__ inline void block_sum2(double& a, double& b, double* lds)
+{
+    for (int o = WAVE/2; o > 0; o >>= 1) { a += __shfl_xor(a, o, WAVE); b += __shfl_xor(b, o, WAVE); }
+    unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
+    __syncthreads();
+    if (lane == 0) { lds[2*wid] = a; lds[2*wid+1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0, s1 = 0;
+        for (unsigned k = 0; k < blockDim.x/WAVE; k++) { s0 += lds[2*k]; s1 += lds[2*k+1]; }
+        lds[0] = s0; lds[1] = s1;
+    }
+    __syncthreads();
+    a = lds[0]; b = lds[1];
+    __syncthreads();
+}
+// every workgroup reduces the G partial pairs in the same order
+__device__ inline void grid_reduce2(const double* part, int G, double& a, double& b, double* lds)
+{
+    double s0 = 0, s1 = 0;
+    if (threadIdx.x < WAVE) {
+        for (int g = threadIdx.x; g < G; g += WAVE) { s0 += part[2*g]; s1 += part[2*g+1]; }
+        for (int o = WAVE/2; o > 0; o >>= 1) { s0 += __shfl_xor(s0, o, WAVE); s1 += __shfl_xor(s1, o, WAVE); }
+        if (threadIdx.x == 0) { lds[16] = s0; lds[17] = s1; }
+    }
+    __syncthreads();
+    a = lds[16]; b = lds[17];
+    __syncthreads();
+}
+
+// block sum, publish into one of two partial buffers, grid barrier, then the
+// same ordered reduction in every workgroup.  Alternating buffers means one
+// barrier per reduction suffices: a buffer is rewritten only after the next
+// barrier, by which time every workgroup has finished reading it.
+__device__ inline void grid_sum2(cg::grid_group& grid, double& a, double& b, double* part, int& buf, int G,
+                                 double* lds)
+{
+    block_sum2(a, b, lds);
+    double* pb = part+(size_t)buf*2*G;
+    if (threadIdx.x == 0) { pb[2*blockIdx.x] = a; pb[2*blockIdx.x+1] = b; }
+    grid.sync();
+    grid_reduce2(pb, G, a, b, lds);
+    buf ^= 1;
+}
+
+template <int LPR>
+__global__ __launch_bounds__(PS_BLOCK) void k_solve_mixed(BsrView A, const double* __restrict__ b,
+    double* __restrict__ x, SolverWork w, DevScalars* sc, SolveOpts o)
+{
+    cg::grid_group grid = cg::this_grid();
+    __shared__ double lds[32];
+    const int G = gridDim.x;
+    const int rows_per_block = PS_BLOCK/LPR;
+    const int sub = threadIdx.x%LPR;
+    const int stride = G*rows_per_block;
+    const int row0 = blockIdx.x*rows_per_block+threadIdx.x/LPR;
+    const int n = A.n;
+    double* part = w.partials;
+    int buf = 0;
+    double bb = 0, rr64 = 0;
+    int outer = 0, inner_total = 0;
+    bool converged = false;
+    for (outer = 0; outer <= o.max_outer; outer++) {
+        // residual r64 = b - A64 x (and ||b||^2)
+        double s_rr = 0, s_bb = 0;
+        for (int i = row0; i < n; i += stride) {
+            double y[6];
+            spmv_row<double, double, LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, sub, y);
+            if (sub == 0) {
+                for (int c = 0; c < 6; c++) {
+                    double bi = b[6*(size_t)i+c];
+                    double r = bi-y[c];
+                    w.r64[6*(size_t)i+c] = r;
+                    s_rr += r*r;
+                    s_bb += bi*bi;
+                }
+            }
+        }
+        grid_sum2(grid, s_rr, s_bb, part, buf, G, lds);
+        if (outer == 0) bb = s_bb;
+        rr64 = s_rr;
+        converged = bb == 0 || sqrt(rr64) <= o.tol*sqrt(bb);
+        if (converged || outer == o.max_outer) break;
+        // inner PCG on A32 d = r64 / ||r64|| ; d0 = 0, p = z = M^-1 r
+        const double scale = 1.0/sqrt(rr64);
+        double s_rz = 0, s_r2 = 0;
+        for (int i = row0; i < n; i += stride) {
+            if (sub != 0) continue;
+            float r[6];
+            for (int c = 0; c < 6; c++) r[c] = (float)(w.r64[6*(size_t)i+c]*scale);
+            const float* Mi = A.minv32+36*(size_t)i;
+            for (int a = 0; a < 6; a++) {
+                float v = 0;
+                for (int c = 0; c < 6; c++) v += Mi[6*a+c]*r[c];
+                size_t k = 6*(size_t)i+a;
+                w.r[k] = r[a]; w.z[k] = v; w.p[k] = v; w.d[k] = 0.f;
+                s_rz += (double)r[a]*v;
+                s_r2 += (double)r[a]*r[a];
+            }
+        }
+        grid_sum2(grid, s_rz, s_r2, part, buf, G, lds);
+        double rz = s_rz, r20 = s_r2;
+        // q = A p
+        double s_pq = 0, dummy = 0;
+        for (int i = row0; i < n; i += stride) {
+            float y[6];
+            spmv_row<float, float, LPR>(A, A.od32, (const float*)nullptr, w.p, i, sub, y);
+            if (sub == 0) {
+                float xi[6];
+                for (int c = 0; c < 6; c++) xi[c] = w.p[6*(size_t)i+c];
+                dmul<float, float>(A.diag32+15*(size_t)i, xi, y);
+                for (int c = 0; c < 6; c++) { w.q[6*(size_t)i+c] = y[c]; s_pq += (double)xi[c]*y[c]; }
+            }
+        }
+        grid_sum2(grid, s_pq, dummy, part, buf, G, lds);
+        double pq = s_pq;
+        const double stop2 = o.inner_tol*o.inner_tol*r20;
+        int it;
+        for (it = 0; it < o.max_inner; it++) {
+            if (!(pq > 0)) break;
+            const float alpha = (float)(rz/pq);
+            double s_rz2 = 0, s_rr2 = 0;
+            for (int i = row0; i < n; i += stride) {
+                if (sub != 0) continue;
+                float r[6];
+                for (int c = 0; c < 6; c++) {
+                    size_t k = 6*(size_t)i+c;
+                    w.d[k] += alpha*w.p[k];
+                    r[c] = w.r[k]-alpha*w.q[k];
+                    w.r[k] = r[c];
+                }
+                const float* Mi = A.minv32+36*(size_t)i;
+                for (int a = 0; a < 6; a++) {
+                    float v = 0;
+                    for (int c = 0; c < 6; c++) v += Mi[6*a+c]*r[c];
+                    w.z[6*(size_t)i+a] = v;
+                    s_rz2 += (double)r[a]*v;
+                    s_rr2 += (double)r[a]*r[a];
+                }
+            }
+            grid_sum2(grid, s_rz2, s_rr2, part, buf, G, lds);
+            if (s_rr2 <= stop2) { it++; break; }
+            const float beta = (float)(s_rz2/rz);
+            rz = s_rz2;
+            double s_pq2 = 0;
+            for (int i = row0; i < n; i += stride) {
+                float y[6];
+                spmv_row<float, float, LPR>(A, A.od32, (const float*)nullptr, w.z, i, sub, y);
+                if (sub == 0) {
+                    float zi[6];
+                    for (int c = 0; c < 6; c++) zi[c] = w.z[6*(size_t)i+c];
+                    dmul<float, float>(A.diag32+15*(size_t)i, zi, y);
+                    for (int c = 0; c < 6; c++) {
+                        size_t k = 6*(size_t)i+c;
+                        float pn = zi[c]+beta*w.p[k];
+                        float qn = y[c]+beta*w.q[k];
+                        w.p[k] = pn; w.q[k] = qn;
+                        s_pq2 += (double)pn*qn;
+                    }
+                }
+            }
+            grid_sum2(grid, s_pq2, dummy, part, buf, G, lds);
+            pq = s_pq2;
+        }
+        inner_total += it;
+        // x += d * ||r64|| (FP64); the next residual reads neighbours' x
+        const double inv = sqrt(rr64);
+        for (int i = row0; i < n; i += stride) {
+            if (sub != 0) continue;
+            for (int c = 0; c < 6; c++) {
+                size_t k = 6*(size_t)i+c;
+                x[k] += (double)w.d[k]*inv;
+            }
+        }
+        grid.sync();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->res_norm = sqrt(rr64);
+        sc->rhs_norm = sqrt(bb);
+        sc->iters_inner = inner_total;
+        sc->iters_outer = outer;
+        sc->total_inner += inner_total;
+        if (!converged) atomicOr(&sc->error, E_NOT_CONVERGED);
+    }
+}
+
+int persistent_groups(const SolveOpts& o)
+{
+    static int cached[9] = {0};
+    int lpr = o.lanes_per_row;
+    if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8) lpr = 4;
+    if (!cached[lpr]) {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        hipError_t e = hipErrorInvalidValue;
+        switch (lpr) {
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_mixed<1>, PS_BLOCK, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_mixed<2>, PS_BLOCK, 0); break;
+        case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_mixed<4>, PS_BLOCK, 0); break;
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_mixed<8>, PS_BLOCK, 0); break;
+        }
+        if (e != hipSuccess || per < 1) per = 1;
+        cached[lpr] = ncu*per;
+    }
+    return cached[lpr];
+}
+
+hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                       const SolveOpts& o, hipStream_t s)
+{
+    int lpr = o.lanes_per_row;
+    if (lpr != 1 && lpr != 2 && lpr != 4 && lpr != 8) lpr = 4;
+    int rows_per_block = PS_BLOCK/lpr;
+    int need = (A.n+rows_per_block-1)/rows_per_block;
+    int maxg = persistent_groups(o);
+    int G = o.groups > 0 ? o.groups : need;
+    if (G > maxg) G = maxg;
+    if (G > w.max_groups) G = w.max_groups;
+    if (G < 1) G = 1;
+    BsrView Av = A;
+    SolverWork wv = w;
+    SolveOpts ov = o;
+    void* args[] = {&Av, (void*)&b, (void*)&x, &wv, &sc, &ov};
+    const void* fn = nullptr;
+    switch (lpr) {
+    case 1: fn = (const void*)k_solve_mixed<1>; break;
+    case 2: fn = (const void*)k_solve_mixed<2>; break;
+    case 8: fn = (const void*)k_solve_mixed<8>; break;
+    default: fn = (const void*)k_solve_mixed<4>; break;
+    }
+    return hipLaunchCooperativeKernel(fn, dim3(G), dim3(PS_BLOCK), args, 0, s);
+}
+
+// ------------------------------------------------------------------ plain FP64 PCG, one kernel per operation
+__global__ void k_residual64(BsrView A, const double* b, const double* x, double* r)
+{
+    int g = blockIdx.x*blockDim.x+threadIdx.x;
+    int i = g/4, sub = g%4;
+    if (i >= A.n) return;
+    double y[6];
+    spmv_row<double, double, 4>(A, A.od64, A.diag64+15*(size_t)i, x, i, sub, y);
+    if (sub == 0) for (int c = 0; c < 6; c++) r[6*(size_t)i+c] = b[6*(size_t)i+c]-y[c];
+}
+__global__ void k_precond64(BsrView A, const double* r, double* z)
+{
+    int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= A.n) return;
+    const double* Mi = A.minv64+36*(size_t)i;
+    for (int a = 0; a < 6; a++) {
+        double v = 0;
+        for (int c = 0; c < 6; c++) v += Mi[6*a+c]*r[6*(size_t)i+c];
+        z[6*(size_t)i+a] = v;
+    }
+}
+__global__ void k_dot64_partial(const double* a, const double* b, size_t m, double* part)
+{
+    __shared__ double lds[32];
+    double s = 0, t = 0;
+    for (size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x; k < m; k += (size_t)gridDim.x*blockDim.x)
+        s += a[k]*b[k];
+    block_sum2(s, t, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void k_dot64_final(const double* part, int G, double* out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0;
+        for (int g = 0; g < G; g++) s += part[g];
+        *out = s;
+    }
+}
+__global__ void k_axpy64(double* y, const double* x, double a, size_t m)
+{
+    size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (k < m) y[k] += a*x[k];
+}
+__global__ void k_xpay64(double* p, const double* z, double beta, size_t m)
+{
+    size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (k < m) p[k] = z[k]+beta*p[k];
+}
+
+static double host_dot(const double* a, const double* b, size_t m, double* part, double* out_dev, hipStream_t s)
+{
+    int G = 64;
+    k_dot64_partial<<<G, 256, 0, s>>>(a, b, m, part);
+    k_dot64_final<<<1, 64, 0, s>>>(part, G, out_dev);
+    double v = 0;
+    (void)hipMemcpyAsync(&v, out_dev, sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    return v;
+}
+
+hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                       const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out)
+{
+    size_t m = 6*(size_t)A.n;
+    unsigned gm = (unsigned)((m+255)/256);
+    double* scal = w.partials+w.max_groups*8-1;
+    k_residual64<<<(A.n*4+255)/256, 256, 0, s>>>(A, b, x, w.pr);
+    k_precond64<<<(A.n+255)/256, 256, 0, s>>>(A, w.pr, w.pz);
+    (void)hipMemcpyAsync(w.pp, w.pz, m*sizeof(double), hipMemcpyDeviceToDevice, s);
+    double bb = host_dot(b, b, m, w.partials, scal, s);
+    double rz = host_dot(w.pr, w.pz, m, w.partials, scal, s);
+    double rr = host_dot(w.pr, w.pr, m, w.partials, scal, s);
+    double stop = o.tol*o.tol*bb;
+    int it = 0;
+    int maxit = o.max_inner*(o.max_outer > 0 ? o.max_outer : 1);
+    while (rr > stop && it < maxit) {
+        launch_spmv64(A, w.pp, w.pq, s);
+        double pq = host_dot(w.pp, w.pq, m, w.partials, scal, s);
+        if (!(pq > 0)) break;
+        double alpha = rz/pq;
+        k_axpy64<<<gm, 256, 0, s>>>(x, w.pp, alpha, m);
+        k_axpy64<<<gm, 256, 0, s>>>(w.pr, w.pq, -alpha, m);
+        rr = host_dot(w.pr, w.pr, m, w.partials, scal, s);
+        it++;
+        if (rr <= stop) break;
+        k_precond64<<<(A.n+255)/256, 256, 0, s>>>(A, w.pr, w.pz);
+        double rz_new = host_dot(w.pr, w.pz, m, w.partials, scal, s);
+        double beta = rz_new/rz;
+        rz = rz_new;
+        k_xpay64<<<gm, 256, 0, s>>>(w.pp, w.pz, beta, m);
+    }
+    // true residual
+    k_residual64<<<(A.n*4+255)/256, 256, 0, s>>>(A, b, x, w.pr);
+    double rt = host_dot(w.pr, w.pr, m, w.partials, scal, s);
+    if (iters_out) *iters_out = it;
+    if (res_out) *res_out = bb > 0 ? sqrt(rt/bb) : 0;
+    (void)sc;
+    return hipGetLastError();
+}
+
+}  // namespace lfd

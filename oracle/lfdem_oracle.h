@@ -1,0 +1,117 @@
+/*
+ * lfdem_oracle.h -- CPU FP64 restatement of LF_DEM's per-step Stokes-resistance
+ * path (cell list, pair lifecycle, lubrication / contact / dashpot / repulsion,
+ * resistance-matrix assembly, sparse Cholesky solve, predictor-corrector step).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  The product library never links or calls it.
+ *
+ * Every function cites the LF_DEM source (file:line, relative to the LF_DEM
+ * repository root) whose semantics it restates.  Arithmetic follows the
+ * reference's operation order; the library is built with -ffp-contract=off.
+ *
+ * Packed block formats (identical to LF_DEM's MatrixBlocks.h:5-46,58-84):
+ *   DBlock  (15 doubles): col0[5] col1[3] col2[1] col3[3] col4[2] col5[1]
+ *   ODBlock (18 doubles): col0[5] col1[3] col2[1] col3[5] col4[3] col5[1]
+ */
+#ifndef LFDEM_ORACLE_H
+#define LFDEM_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_DBLOCK 15
+#define ORC_ODBLOCK 18
+
+typedef struct orc_params {
+	double shear_rate;        /* gamma-dot (1 in hydrodynamic units) */
+	double theta_shear;       /* shear direction (System.cpp:2006-2015) */
+	double lub_max_gap;       /* lubrication cutoff gap (0.5) */
+	double lub_reduce;        /* regularisation delta (1e-3) */
+	double kn, kt;            /* spring constants */
+	double mu_static, mu_dynamic;
+	double tau_n, tau_t;      /* contact_relaxation_time, _tan */
+	double sd_coeff;          /* Stokes-drag floor coefficient */
+	double repulsion;         /* repulsive force scale F_R^0 */
+	double repulsive_length;  /* screening length lambda_D; <=0 disables */
+	double disp_max, dt_max, dt_min;
+	int friction_model;       /* 0 frictionless, 1 Coulomb */
+	int lubrication_model;    /* 1 normal, 2 tangential */
+} orc_params;
+
+void orc_default_params(orc_params *p);
+
+/* ---- geometry (System.cpp:2478-2548, 2665-2674) ---- */
+int orc_periodize(const double L[3], const double shear_disp[3], double pos[3]);
+int orc_periodize_diff(const double L[3], const double shear_disp[3], double d[3]);
+double orc_lub_range(double ri, double rj, double lub_max_gap);
+
+/* ---- pair physics on a single pair (unit tests) ---- */
+void orc_lub_constants(double a0, double a1, double out[36]);
+void orc_dashpot_coeffs(double a0, double a1, const orc_params *p, double out[16]);
+void orc_lub_blocks(double a0, double a1, double reduced_gap, const double nvec[3],
+                    const orc_params *p, double db0[15], double db1[15], double od[18]);
+void orc_dashpot_blocks(double a0, double a1, const double nvec[3], const orc_params *p,
+                        double db0[15], double db1[15], double od[18]);
+void orc_dblock_dense(const double b[15], double M[36]);
+void orc_odblock_dense(const double b[18], double M[36]);
+
+/* ---- CSC export, lower triangle (StokesSolver.cpp:334-379, 889-919) ---- */
+long orc_csc_nnz(int np, int nod);
+int orc_export_csc(int np, const double *dblocks, int nod, const int *od_col_ptr,
+                   const int *od_rows, const double *odblocks,
+                   int *colp, int *rowi, double *val);
+
+/* ---- brute-force pair set (all i<j with sq_dist < range^2) ---- */
+int orc_bruteforce_pairs(int np, const double *pos, const double *radius, const double L[3],
+                         const double shear_disp[3], double lub_max_gap,
+                         int *p0, int *p1, int cap);
+
+/* ---- system ---- */
+typedef struct orc_system orc_system;
+orc_system *orc_create(int np, const double *pos, const double *radius, const double L[3],
+                       const double shear_disp[3], const orc_params *p);
+void orc_destroy(orc_system *s);
+int orc_setup(orc_system *s);          /* initializeBoxing + checkNewInteraction + drag seed */
+int orc_first_update(orc_system *s);   /* timeEvolution first-call block */
+int orc_step(orc_system *s, double strain_end);   /* one predictor-corrector step */
+const char *orc_last_error(orc_system *s);
+
+/* phases, callable one at a time */
+int orc_boxset_update(orc_system *s);
+int orc_check_new_interaction(orc_system *s);
+int orc_update_interactions(orc_system *s);
+int orc_build_blocks(orc_system *s, double *dblocks, int *od_col_ptr, int *od_rows,
+                     double *odblocks, int cap);
+int orc_build_rhs(orc_system *s, double *rhs);
+int orc_solve_current(orc_system *s, const double *rhs, double *x);
+int orc_compute_velocities(orc_system *s);
+int orc_set_in_predictor(orc_system *s, int flag);
+int orc_set_dt(orc_system *s, double dt);
+
+/* state access */
+int orc_num_pairs(const orc_system *s);
+int orc_get_pairs(const orc_system *s, int *p0, int *p1, int *contact_state,
+                  double *geom /* P x 5: nx ny nz r gap */, double *disp_tan /* P x 3 */);
+int orc_set_pair_history(orc_system *s, int n, const int *p0, const int *p1,
+                         const int *state, const double *disp_tan, const double *prev_disp_tan);
+int orc_get_state(const orc_system *s, double *pos, double *vel, double *ang_vel,
+                  double *na_vel, double *na_ang, double *scalars /* 8 */);
+int orc_set_positions(orc_system *s, const double *pos, const double shear_disp[3]);
+int orc_get_box_info(const orc_system *s, int *dims /* 4: nx ny nz boxed */, int *labels);
+int orc_get_neighbor_cells(const orc_system *s, int cell, int *out, int cap);
+
+/* ---- sparse Cholesky on 6x6-block SPD matrices ---- */
+typedef struct orc_chol orc_chol;
+orc_chol *orc_chol_factor(int np, const double *dblocks, const int *od_col_ptr,
+                          const int *od_rows, const double *odblocks, int *status);
+void orc_chol_solve(const orc_chol *c, const double *b, double *x);
+long orc_chol_nnz_blocks(const orc_chol *c);
+double orc_chol_flops(const orc_chol *c);
+void orc_chol_free(orc_chol *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

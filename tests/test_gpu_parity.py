@@ -1,0 +1,167 @@
+"""Parity of the HIP path (through the C ABI) against the oracle.
+
+Bars: pair set and cell labels bit-exact; pair geometry and assembled 6x6
+blocks / RHS within a few FP64 ulps (the only differences allowed come from
+the device log/exp, which need not round like glibc's); velocities within
+1e-8 relative of the oracle's FP64 direct (Cholesky) solve.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import conf
+
+pytestmark = pytest.mark.gpu
+
+REL_VEL = 1e-8        # velocities vs FP64 direct solve
+REL_BLOCK = 1e-13     # FP64 assembly: ulp-level (device log/exp vs glibc)
+
+
+def _fixture_sys(golden):
+    return golden["dat_pos"], golden["dat_radius"], golden["dat_L"], golden["dat_disp"]
+
+
+def _bin_sys(golden):
+    return golden["bin_pos"], golden["bin_radius"], golden["bin_L"], golden["bin_disp"]
+
+
+def _c1():
+    pos, rad, L = conf(500, 0.5, 1)
+    return pos, rad, L, np.zeros(3)
+
+
+CASES = {"n200_fixture": _fixture_sys, "n200_le_shift": _bin_sys}
+
+
+def _make(case, golden):
+    if case == "c1_n500":
+        return _c1()
+    return CASES[case](golden)
+
+
+ALL = ["n200_fixture", "n200_le_shift", "c1_n500"]
+
+
+def _pairset(d):
+    return list(zip(d["p0"].tolist(), d["p1"].tolist()))
+
+
+@pytest.mark.parametrize("case", ALL)
+def test_pairs_and_cells_bitexact(case, golden, oracle, capi):
+    pos, rad, L, sd = _make(case, golden)
+    print("case", case, "np", len(rad), flush=True)
+    S = capi.System(pos, rad, L, sd)
+    R = oracle.System(pos, rad, L, sd)
+    g, o = S.pairs(), R.pairs()
+    assert _pairset(g) == _pairset(o)
+    assert np.array_equal(g["nvec"], o["nvec"])
+    assert np.array_equal(g["r"], o["r"])
+    assert np.array_equal(g["gap"], o["gap"])
+    assert np.array_equal(g["state"], o["state"])
+    c, bi = S.cells(), R.box_info()
+    assert (c["nx"], c["ny"], c["nz"], c["boxed"]) == (bi["nx"], bi["ny"], bi["nz"], bi["boxed"])
+    assert np.array_equal(c["labels"], bi["labels"])
+
+
+def test_golden_pair_set_on_gpu(golden, capi):
+    """The reference's own t=0 pair list (check/reference/int_*stress5r.dat)."""
+    pos, rad, L, sd = _fixture_sys(golden)
+    S = capi.System(pos, rad, L, sd)
+    g = S.pairs()
+    gi = golden["stress_int"]
+    assert set(_pairset(g)) == set(zip(gi[:, 0].astype(int).tolist(), gi[:, 1].astype(int).tolist()))
+    assert len(g["p0"]) == 745
+
+
+@pytest.mark.parametrize("case", ALL)
+def test_blocks_and_rhs(case, golden, oracle, capi):
+    pos, rad, L, sd = _make(case, golden)
+    print("case", case, flush=True)
+    S = capi.System(pos, rad, L, sd)
+    R = oracle.System(pos, rad, L, sd)
+    S.first_update()
+    R.first_update()
+    S.assemble()
+    db, od, p0, p1 = S.blocks()
+    rdb, rcp, rrows, rod = R.blocks()
+    # every pair of these configurations carries a resistance
+    assert len(rrows) == len(p1)
+    rp0 = np.repeat(np.arange(len(rad)), np.diff(rcp))
+    assert np.array_equal(rp0, p0) and np.array_equal(rrows, p1)
+    scale = np.abs(rdb).max()
+    assert np.abs(db - rdb).max() <= REL_BLOCK * scale
+    assert np.abs(od - rod).max() <= REL_BLOCK * np.abs(rod).max()
+    r_gpu, r_orc = S.rhs(), R.rhs()
+    assert np.abs(r_gpu - r_orc).max() <= REL_BLOCK * np.abs(r_orc).max()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("case", ALL)
+def test_velocities_vs_direct(case, kind, golden, oracle, capi):
+    pos, rad, L, sd = _make(case, golden)
+    print("case", case, "solver kind", kind, flush=True)
+    opts = capi.default_opts(kind=kind)
+    S = capi.System(pos, rad, L, sd, opts=opts)
+    R = oracle.System(pos, rad, L, sd)
+    S.first_update()
+    R.first_update()
+    S.compute_velocities(True)
+    R.set_in_predictor(True)
+    R.compute_velocities()
+    gs, rs = S.state(), R.state()
+    for key in ("na_vel", "na_ang_vel"):
+        err = np.linalg.norm(gs[key] - rs[key]) / np.linalg.norm(rs[key])
+        print(key, "rel err", err)
+        assert err <= REL_VEL
+    assert gs["max_velocity"] == pytest.approx(rs["max_velocity"], rel=1e-8)
+
+
+def test_stokes_dropin_matches_direct(golden, oracle, capi):
+    """Blocks handed over through the StokesSolver column protocol."""
+    pos, rad, L, sd = _c1()
+    R = oracle.System(pos, rad, L, sd)
+    R.first_update()
+    db, cp, rows, od = R.blocks()
+    b = R.rhs()
+    n = len(rad)
+    ss = capi.StokesSolver(n)
+    # the oracle's assembled diagonal blocks enter as the reset blocks, the
+    # pair diagonals as zeros: the column protocol then carries the ODBlocks
+    ss.resetResistanceMatrix(len(rows), db, True)
+    zero = np.zeros(15)
+    for i in range(n - 1):
+        ss.startNewColumn()
+        for k in range(cp[i], cp[i + 1]):
+            ss.addResistanceBlocks(i, rows[k], zero, zero, od[k])
+    ss.matrixFillingDone()
+    x_ref = oracle.Chol(n, db, cp, rows, od).solve(b)
+    A = oracle.assemble_dense(n, db, cp, rows, od)
+    y = ss.multiply(x_ref)
+    assert np.abs(y - A @ x_ref).max() <= 1e-12 * np.abs(A @ x_ref).max()
+    ss.resetRHS()
+    ss.setRHS(b)
+    u, w = ss.solve()
+    x = np.concatenate([u, w], axis=1).ravel()
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    print("drop-in rel err", err, ss.last_stats.iters_inner, ss.last_stats.iters_outer)
+    assert err <= REL_VEL
+    with pytest.raises(capi.LfdError):
+        ss.solve(out_len=6 * n - 1)   # StokesSolver::solve throws when the output is too small
+
+
+def test_multi_step_trajectory(golden, oracle, capi):
+    """A few predictor-corrector steps: same pair sets, positions within 1e-8."""
+    pos, rad, L, sd = _c1()
+    S = capi.System(pos, rad, L, sd)
+    R = oracle.System(pos, rad, L, sd)
+    S.first_update()
+    R.first_update()
+    for k in range(5):
+        S.step(1)
+        R.step()
+        gs, rs = S.state(), R.state()
+        print("step", k, "dt", gs["dt"], rs["dt"], flush=True)
+        assert gs["dt"] == pytest.approx(rs["dt"], rel=1e-7)
+        d = gs["pos"] - rs["pos"]
+        d -= np.round(d / L) * L
+        assert np.abs(d).max() <= 1e-8
+        assert _pairset(S.pairs()) == _pairset(R.pairs())
