@@ -138,6 +138,11 @@ int lfd_system_get_blocks(lfd_system *s, int cap, double *dblocks, double *odblo
 int lfd_system_get_rhs(lfd_system *s, double *rhs);
 int lfd_system_get_solve_stats(lfd_system *s, lfd_solve_stats *st, long long *total_inner);
 int lfd_system_set_opts(lfd_system *s, const lfd_solve_opts *o);
+/* Solver-launch timing with HIP events on the library's stream: returns the
+ * summed duration (ms) and count of the launches recorded since the previous
+ * call, then enables (enable != 0) or disables recording.  No LF_DEM
+ * counterpart (LF_DEM's own timing is wall clock, Simulation.cpp:478-489). */
+int lfd_system_profile(lfd_system *s, int enable, double *solve_ms_total, int *solve_launches);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,7 @@ SIGNATURES = {
     "lfd_system_get_rhs": ([_vp, _dp], ctypes.c_int),
     "lfd_system_get_solve_stats": ([_vp, _vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "lfd_system_set_opts": ([_vp, _vp], ctypes.c_int),
+    "lfd_system_profile": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
 }
 
 
@@ -312,6 +313,13 @@ class System:
         r = np.zeros(6 * self.np)
         _check(lib().lfd_system_get_rhs(self.h, _d(r)))
         return r
+
+    def profile(self, enable):
+        """(summed ms, launches) of solver launches since the last call; then (dis)ables recording."""
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int(0)
+        _check(lib().lfd_system_profile(self.h, int(enable), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def solve_stats(self):
         st = SolveStats()

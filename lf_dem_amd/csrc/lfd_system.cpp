@@ -133,6 +133,8 @@ struct Solver {
     int n = 0, pcap = 0;
     SolveOpts opts{};
     int kind = LFD_SOLVER_MIXED;
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof;   // per solve launch
     double *diag64 = nullptr, *od64 = nullptr, *minv64 = nullptr;
     float *diag32 = nullptr, *od32 = nullptr, *minv32 = nullptr;
     int *p0 = nullptr, *p1 = nullptr, *row_ptr = nullptr, *tptr = nullptr, *tidx = nullptr;
@@ -162,6 +164,8 @@ struct Solver {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
                         w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials};
         for (void* p : ptrs) if (p) (void)hipFree(p);
+        for (auto& e : prof) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        prof.clear();
     }
     BsrView view(const int* npairs) const
     {
@@ -201,7 +205,36 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         HIPCHK(hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, s));
         return LFD_OK;
     }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (S.profiling) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, s));
+    }
     HIPCHK(solve_mixed(A, b, x, S.w, sc, S.opts, s));
+    if (S.profiling) {
+        HIPCHK(hipEventRecord(e1, s));
+        S.prof.emplace_back(e0, e1);
+    }
+    return LFD_OK;
+}
+
+// Sum of the durations of the profiled solver launches (HIP events on the
+// library's stream), then the record is cleared.
+static int collect_profile(Solver& S, hipStream_t s, double* ms_total, int* launches)
+{
+    HIPCHK(hipStreamSynchronize(s));
+    double tot = 0;
+    for (auto& e : S.prof) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (ms_total) *ms_total = tot;
+    if (launches) *launches = (int)S.prof.size();
+    S.prof.clear();
     return LFD_OK;
 }
 
@@ -464,6 +497,14 @@ extern "C" int lfd_system_create(int np, const double* pos, const double* radius
 }
 
 extern "C" void lfd_system_destroy(lfd_system* y) { delete y; }
+
+extern "C" int lfd_system_profile(lfd_system* y, int enable, double* solve_ms_total, int* solve_launches)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    int rc = collect_profile(y->S, y->s, solve_ms_total, solve_launches);
+    y->S.profiling = enable != 0;
+    return rc;
+}
 
 extern "C" int lfd_system_set_opts(lfd_system* y, const lfd_solve_opts* o)
 {

@@ -1156,6 +1156,7 @@ int orc_build_rhs(orc_system *s, double *rhs)
 	int np = s->np, i, k;
 	double *F = calloc(6*(size_t)np, sizeof(double));
 	double *T = F+3*(size_t)np;
+	compute_uinf(s);
 	memset(rhs, 0, 6*(size_t)np*sizeof(double));
 	/* contact (Contact.cpp:411-437) */
 	for (k = 0; k < s->npairs; k++) {
@@ -1680,7 +1681,6 @@ int orc_compute_velocities(orc_system *s)
 	/* System::computeVelocities, rate control, no components (System.cpp:2273-2319) */
 	int i, st;
 	double *rhs = malloc(12*(size_t)s->np*sizeof(double)), *x = rhs+6*(size_t)s->np;
-	compute_uinf(s);
 	orc_build_rhs(s, rhs);
 	st = orc_solve_current(s, rhs, x);
 	for (i = 0; i < s->np; i++) {
