@@ -45,8 +45,9 @@ typedef struct lfd_params {
 } lfd_params;
 
 typedef enum lfd_solver_kind {
-	LFD_SOLVER_MIXED = 0,   /* FP64 refinement around persistent FP32 block-Jacobi PCG */
-	LFD_SOLVER_PLAIN = 1    /* FP64 block-Jacobi PCG, one kernel per operation */
+	LFD_SOLVER_MIXED = 0,   /* FP64 refinement around persistent FP32 block-Jacobi PCG, blocks in LDS */
+	LFD_SOLVER_PLAIN = 1,   /* FP64 block-Jacobi PCG, one kernel per operation */
+	LFD_SOLVER_MIXED_GLOBAL = 2  /* as MIXED, blocks read from HBM each iteration */
 } lfd_solver_kind;
 
 typedef struct lfd_solve_opts {

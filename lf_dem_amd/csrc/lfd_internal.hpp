@@ -61,6 +61,7 @@ struct SolverWork {
     // FP64 vectors for the plain form
     double *pr, *pz, *pp, *pq;
     double* partials;      // [max_groups * 8]
+    unsigned* barrier;     // [16384] tree-barrier words, zeroed once at allocation
     int max_groups;
 };
 
@@ -123,6 +124,8 @@ void launch_copy_double(double* dst, const double* src, size_t n, hipStream_t s)
 int persistent_groups(const SolveOpts& o);
 hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s);
+hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                     const SolveOpts& o, hipStream_t s);
 hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
 void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);

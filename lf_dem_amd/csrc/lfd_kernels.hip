@@ -1156,6 +1156,340 @@ __global__ __launch_bounds__(PS_BLOCK) void k_solve_mixed(BsrView A, const doubl
     }
 }
 
+// ------------------------------------------------------------------ LDS-resident persistent solver
+// Same algorithm as k_solve_mixed, restructured for latency:
+//  * 8 lanes per block row; lane c < 6 owns component c of the row's r, z,
+//    p, q, d in registers and row c of the diagonal block and of its inverse;
+//  * each workgroup stages the packed FP32 off-diagonal blocks of its rows in
+//    LDS once per solve, so an SpMV touches HBM only for the neighbours' z;
+//  * a two-level arrival tree (8 workgroups per leaf counter, one root) with
+//    a generation word replaces the single-counter grid barrier.
+static constexpr int LS_BLOCK = 256, LS_LPR = 8, LS_ROWS = LS_BLOCK/LS_LPR;
+static constexpr int LS_ENTRY_CAP = 640;          // staged blocks per workgroup (48 KB)
+
+// barrier words: [0] root count, [32] generation, [64 + 32*leaf] leaf counts (128-byte spacing)
+__device__ inline unsigned ld_acquire(unsigned* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ inline void tree_barrier(unsigned* bar, int G, unsigned& gen)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int leaf = blockIdx.x >> 3, nleaf = (G+7) >> 3;
+        const int lsize = min(8, G-(leaf << 3));
+        unsigned* lc = bar+64+32*leaf;
+        unsigned old = atomicAdd(lc, 1u);
+        if (old == (unsigned)(lsize-1)) {
+            atomicExch(lc, 0u);
+            unsigned r = atomicAdd(bar, 1u);
+            if (r == (unsigned)(nleaf-1)) {
+                atomicExch(bar, 0u);
+                __threadfence();
+                atomicAdd(bar+32, 1u);
+            }
+        }
+        while (ld_acquire(bar+32) == gen) __builtin_amdgcn_s_sleep(1);
+        gen++;
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+struct LdsRow { int e0, ne; };      // staged entries of a row (ne < 0: not staged)
+
+__global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double* __restrict__ b,
+    double* __restrict__ x, SolverWork w, DevScalars* sc, SolveOpts o, unsigned* bar)
+{
+    __shared__ int ent_j[LS_ENTRY_CAP];             // partner; ~partner for the transposed block
+    __shared__ float ent_b[LS_ENTRY_CAP*18];
+    __shared__ LdsRow rows[LS_ROWS];
+    __shared__ double red[3*LS_BLOCK/WAVE+4];
+    const int G = gridDim.x, n = A.n;
+    const int lr = threadIdx.x/LS_LPR, c = threadIdx.x%LS_LPR;
+    const int i = blockIdx.x*LS_ROWS+lr;
+    const bool live = i < n;
+    unsigned gen = ld_acquire(bar+32);
+    // ---- stage this workgroup's rows
+    if (threadIdx.x == 0) {
+        int used = 0;
+        for (int t = 0; t < LS_ROWS; t++) {
+            int ii = blockIdx.x*LS_ROWS+t;
+            int deg = ii < n ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
+            if (ii < n && used+deg <= LS_ENTRY_CAP) { rows[t].e0 = used; rows[t].ne = deg; used += deg; }
+            else { rows[t].e0 = 0; rows[t].ne = ii < n ? -1 : 0; }
+        }
+    }
+    __syncthreads();
+    if (live && rows[lr].ne > 0) {
+        const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+        for (int e = c; e < rows[lr].ne; e += LS_LPR) {
+            int k, j;
+            if (e < nl) { k = A.tidx[l0+e]; j = A.p0[k]; }
+            else { k = u0+(e-nl); j = ~A.p1[k]; }
+            int s = rows[lr].e0+e;
+            ent_j[s] = j;
+            for (int m = 0; m < 18; m++) ent_b[18*s+m] = A.od32[18*(size_t)k+m];
+        }
+    }
+    // row c of D and of D^-1 in registers (lanes 0..5)
+    float Dr[6] = {0, 0, 0, 0, 0, 0}, Mr[6] = {0, 0, 0, 0, 0, 0};
+    if (live && c < 6) {
+        for (int m = 0; m < 6; m++) {
+            float e6[6] = {0, 0, 0, 0, 0, 0};
+            e6[m] = 1.f;
+            float y[6] = {0, 0, 0, 0, 0, 0};
+            dmul<float, float>(A.diag32+15*(size_t)i, e6, y);
+            Dr[m] = y[c];
+            Mr[m] = A.minv32[36*(size_t)i+6*c+m];
+        }
+    }
+    __syncthreads();
+    const int ne = live ? rows[lr].ne : 0, e0 = live ? rows[lr].e0 : 0;
+
+    // y = A32 v for this row (v of neighbours read from global vbuf, own v in registers)
+    auto spmv32 = [&](const float* vbuf, float vown) -> float {
+        float y[6] = {0, 0, 0, 0, 0, 0};
+        if (live) {
+            if (ne >= 0) {
+                for (int e = c; e < ne; e += LS_LPR) {
+                    int jj = ent_j[e0+e];
+                    const float* bl = ent_b+18*(e0+e);
+                    float xj[6];
+                    if (jj >= 0) {
+                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        omul<float, float>(bl, xj, y);
+                    } else {
+                        jj = ~jj;
+                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        omul_t<float, float>(bl, xj, y);
+                    }
+                }
+            } else {
+                const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+                const int nt = nl+(A.row_ptr[i+1]-u0);
+                for (int e = c; e < nt; e += LS_LPR) {
+                    float xj[6];
+                    if (e < nl) {
+                        int k = A.tidx[l0+e], jj = A.p0[k];
+                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        omul<float, float>(A.od32+18*(size_t)k, xj, y);
+                    } else {
+                        int k = u0+(e-nl), jj = A.p1[k];
+                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        omul_t<float, float>(A.od32+18*(size_t)k, xj, y);
+                    }
+                }
+            }
+        }
+        for (int ofs = 1; ofs < LS_LPR; ofs <<= 1)
+            for (int m = 0; m < 6; m++) y[m] += __shfl_xor(y[m], ofs, LS_LPR);
+        // diagonal part: own v gathered from the 6 owning lanes
+        float yc = 0;
+        const int base = (threadIdx.x & (WAVE-1)) & ~(LS_LPR-1);
+        for (int m = 0; m < 6; m++) {
+            float vm = __shfl(vown, base+m, WAVE);
+            yc += Dr[m]*vm;
+        }
+        float mine = 0;
+        for (int m = 0; m < 6; m++) if (m == c) mine = y[m];
+        return mine+yc;
+    };
+
+    double* part = w.partials;
+    int buf = 0;
+    // block sum of three values, publish, tree barrier, ordered grid reduction
+    auto gsum3 = [&](double& a, double& bsum, double& csum) {
+        for (int ofs = WAVE/2; ofs > 0; ofs >>= 1) {
+            a += __shfl_xor(a, ofs, WAVE); bsum += __shfl_xor(bsum, ofs, WAVE); csum += __shfl_xor(csum, ofs, WAVE);
+        }
+        const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
+        if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bsum; red[3*wid+2] = csum; }
+        __syncthreads();
+        double* pb = part+(size_t)buf*3*G;
+        if (threadIdx.x == 0) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int k = 0; k < LS_BLOCK/WAVE; k++) { s0 += red[3*k]; s1 += red[3*k+1]; s2 += red[3*k+2]; }
+            pb[3*blockIdx.x] = s0; pb[3*blockIdx.x+1] = s1; pb[3*blockIdx.x+2] = s2;
+        }
+        tree_barrier(bar, G, gen);
+        double s0 = 0, s1 = 0, s2 = 0;
+        if (threadIdx.x < WAVE) {
+            for (int g = threadIdx.x; g < G; g += WAVE) {
+                s0 += __hip_atomic_load(pb+3*g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s1 += __hip_atomic_load(pb+3*g+1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s2 += __hip_atomic_load(pb+3*g+2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            for (int ofs = WAVE/2; ofs > 0; ofs >>= 1) {
+                s0 += __shfl_xor(s0, ofs, WAVE); s1 += __shfl_xor(s1, ofs, WAVE); s2 += __shfl_xor(s2, ofs, WAVE);
+            }
+            if (threadIdx.x == 0) { red[3*LS_BLOCK/WAVE] = s0; red[3*LS_BLOCK/WAVE+1] = s1; red[3*LS_BLOCK/WAVE+2] = s2; }
+        }
+        __syncthreads();
+        a = red[3*LS_BLOCK/WAVE]; bsum = red[3*LS_BLOCK/WAVE+1]; csum = red[3*LS_BLOCK/WAVE+2];
+        __syncthreads();
+        buf ^= 1;
+    };
+    auto gsum2 = [&](double& a, double& bsum) { double t = 0; gsum3(a, bsum, t); };
+
+    const bool own = live && c < 6;
+    const size_t gi = 6*(size_t)i+c;
+    double bb = 0, rr64 = 0;
+    int outer, inner_total = 0;
+    bool converged = false, use_classic = false;
+    for (outer = 0; outer <= o.max_outer; outer++) {
+        // FP64 residual from global FP64 blocks
+        double yv[6] = {0, 0, 0, 0, 0, 0};
+        if (live) spmv_row<double, double, LS_LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, c, yv);
+        double r64 = 0, bi = 0;
+        if (own) {
+            for (int m = 0; m < 6; m++) if (m == c) r64 = yv[m];
+            bi = b[gi];
+            r64 = bi-r64;
+        }
+        double s_rr = r64*r64, s_bb = bi*bi;
+        gsum2(s_rr, s_bb);
+        if (outer == 0) bb = s_bb;
+        if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
+        rr64 = s_rr;
+        converged = bb == 0 || sqrt(rr64) <= o.tol*sqrt(bb);
+        if (converged || outer == o.max_outer) break;
+        const double scale = 1.0/sqrt(rr64);
+        const int base = (threadIdx.x & (WAVE-1)) & ~(LS_LPR-1);
+        auto precond = [&](float rv) -> float {
+            float z = 0;
+            for (int m = 0; m < 6; m++) { float rm = __shfl(rv, base+m, WAVE); z += Mr[m]*rm; }
+            return own ? z : 0.f;
+        };
+        float d = 0.f;
+        int it = 0;
+        if (!use_classic) {
+        // inner FP32 pipelined PCG (Ghysels & Vanroose 2014), d0 = 0: the
+        // reduction of (r,u), (w,u), (r,r) and the neighbour exchange of
+        // m = M^-1 w share one barrier per iteration; m is double-buffered so
+        // a workgroup may publish m_{i+1} while others still read m_i.
+        float* mbuf[2] = {w.z, w.q};
+        float r = own ? (float)(r64*scale) : 0.f;
+        float u = precond(r);
+        if (own) mbuf[1][gi] = u;
+        double s_r2 = (double)r*r, dummy = 0, dummy2 = 0;
+        gsum3(s_r2, dummy, dummy2);
+        const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+        float wv = spmv32(mbuf[1], u);
+        if (!own) wv = 0.f;
+        float zv = 0, qv = 0, sv = 0, pv = 0;
+        d = 0;
+        double gamma_old = 1, alpha_old = 1;
+        int par = 0;
+        for (it = 0; it < o.max_inner; it++) {
+            double g = (double)r*u, dl = (double)wv*u, rho = (double)r*r;
+            float mv = precond(wv);
+            if (own) mbuf[par][gi] = mv;
+            gsum3(g, dl, rho);
+            if (rho <= stop2) break;
+            double beta, alpha;
+            if (it > 0) {
+                beta = g/gamma_old;
+                double den = dl-beta*g/alpha_old;
+                if (!(den > 0)) break;
+                alpha = g/den;
+            } else {
+                if (!(dl > 0)) break;
+                beta = 0;
+                alpha = g/dl;
+            }
+            float nv = spmv32(mbuf[par], mv);
+            if (!own) nv = 0.f;
+            const float bf = (float)beta, af = (float)alpha;
+            zv = nv+bf*zv;
+            qv = mv+bf*qv;
+            sv = wv+bf*sv;
+            pv = u+bf*pv;
+            d += af*pv;
+            r -= af*sv;
+            u -= af*qv;
+            wv -= af*zv;
+            gamma_old = g;
+            alpha_old = alpha;
+            par ^= 1;
+        }
+        } else {
+            // classic PCG (two barriers per iteration): fallback when the
+            // pipelined recurrences stop reducing the true residual
+            float r = own ? (float)(r64*scale) : 0.f;
+            float z = precond(r), p = z;
+            d = 0;
+            if (own) w.z[gi] = z;
+            double s_rz = (double)r*z, s_r2 = (double)r*r;
+            gsum2(s_rz, s_r2);
+            double rz = s_rz;
+            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            float q = spmv32(w.z, z);
+            if (!own) q = 0.f;
+            double s_pq = (double)p*q, dmy = 0;
+            gsum2(s_pq, dmy);
+            double pq = s_pq;
+            for (it = 0; it < o.max_inner; it++) {
+                if (!(pq > 0)) break;
+                const float alpha = (float)(rz/pq);
+                d += alpha*p;
+                r -= alpha*q;
+                z = precond(r);
+                if (own) w.z[gi] = z;
+                double s_rz2 = (double)r*z, s_rr2 = (double)r*r;
+                gsum2(s_rz2, s_rr2);
+                if (s_rr2 <= stop2) { it++; break; }
+                const float beta = (float)(s_rz2/rz);
+                rz = s_rz2;
+                float wz = spmv32(w.z, z);
+                if (!own) wz = 0.f;
+                p = z+beta*p;
+                q = wz+beta*q;
+                double s_pq2 = (double)p*q;
+                dmy = 0;
+                gsum2(s_pq2, dmy);
+                pq = s_pq2;
+            }
+        }
+        inner_total += it;
+        if (own) x[gi] += (double)d*sqrt(rr64);
+        tree_barrier(bar, G, gen);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->res_norm = sqrt(rr64);
+        sc->rhs_norm = sqrt(bb);
+        sc->iters_inner = inner_total;
+        sc->iters_outer = outer;
+        sc->total_inner += inner_total;
+        if (!converged) atomicOr(&sc->error, E_NOT_CONVERGED);
+    }
+}
+
+int lds_solver_max_groups()
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_lds, LS_BLOCK, 0) != hipSuccess || per < 1) per = 1;
+        cached = ncu*per;
+    }
+    return cached;
+}
+
+hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                     const SolveOpts& o, hipStream_t s)
+{
+    int G = (A.n+LS_ROWS-1)/LS_ROWS;
+    if (G > lds_solver_max_groups() || G > w.max_groups) return hipErrorCooperativeLaunchTooLarge;
+    BsrView Av = A;
+    SolverWork wv = w;
+    SolveOpts ov = o;
+    unsigned* bar = w.barrier;
+    void* args[] = {&Av, (void*)&b, (void*)&x, &wv, &sc, &ov, &bar};
+    return hipLaunchCooperativeKernel((const void*)k_solve_lds, dim3(G), dim3(LS_BLOCK), args, 0, s);
+}
+
 int persistent_groups(const SolveOpts& o)
 {
     static int cached[9] = {0};

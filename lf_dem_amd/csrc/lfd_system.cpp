@@ -62,8 +62,8 @@ extern "C" void lfd_default_params(lfd_params* p)
 extern "C" void lfd_default_solve_opts(lfd_solve_opts* o)
 {
     o->tol = 1e-12;
-    o->inner_tol = 1e-4;
-    o->max_outer = 12;
+    o->inner_tol = 1e-2;
+    o->max_outer = 40;
     o->max_inner = 2000;
     o->lanes_per_row = 4;
     o->groups = 0;
@@ -156,13 +156,16 @@ struct Solver {
         A(dalloc(&w.pr, m)); A(dalloc(&w.pz, m)); A(dalloc(&w.pp, m)); A(dalloc(&w.pq, m));
         w.max_groups = 2048;
         A(dalloc(&w.partials, 8*(size_t)w.max_groups));
+        A(dalloc(&w.barrier, 16384));
+        A(hipMemset(w.barrier, 0, 16384*sizeof(unsigned)));
 #undef A
         return e;
     }
     void release()
     {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
-                        w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials};
+                        w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials,
+                        w.barrier};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (auto& e : prof) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         prof.clear();
@@ -211,7 +214,13 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, s));
     }
-    HIPCHK(solve_mixed(A, b, x, S.w, sc, S.opts, s));
+    hipError_t e = hipErrorCooperativeLaunchTooLarge;
+    if (S.kind == LFD_SOLVER_MIXED) e = solve_lds(A, b, x, S.w, sc, S.opts, s);
+    if (e == hipErrorCooperativeLaunchTooLarge) {
+        (void)hipGetLastError();
+        e = solve_mixed(A, b, x, S.w, sc, S.opts, s);
+    }
+    HIPCHK(e);
     if (S.profiling) {
         HIPCHK(hipEventRecord(e1, s));
         S.prof.emplace_back(e0, e1);

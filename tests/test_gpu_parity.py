@@ -94,7 +94,7 @@ def test_blocks_and_rhs(case, golden, oracle, capi):
     assert np.abs(r_gpu - r_orc).max() <= REL_BLOCK * np.abs(r_orc).max()
 
 
-@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
 @pytest.mark.parametrize("case", ALL)
 def test_velocities_vs_direct(case, kind, golden, oracle, capi):
     pos, rad, L, sd = _make(case, golden)
