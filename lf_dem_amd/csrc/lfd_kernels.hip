@@ -1157,41 +1157,102 @@ __global__ __launch_bounds__(PS_BLOCK) void k_solve_mixed(BsrView A, const doubl
 }
 
 // ------------------------------------------------------------------ LDS-resident persistent solver
-// Same algorithm as k_solve_mixed, restructured for latency:
-//  * 8 lanes per block row; lane c < 6 owns component c of the row's r, z,
-//    p, q, d in registers and row c of the diagonal block and of its inverse;
+// FP64 iterative refinement around FP32 block-Jacobi PCG, restructured for
+// latency on MI355X:
+//  * 8 lanes per block row; lane c < 6 owns component c of every Krylov
+//    vector of the row in registers, and row c of the diagonal block and of
+//    its inverse;
 //  * each workgroup stages the packed FP32 off-diagonal blocks of its rows in
-//    LDS once per solve, so an SpMV touches HBM only for the neighbours' z;
-//  * a two-level arrival tree (8 workgroups per leaf counter, one root) with
-//    a generation word replaces the single-counter grid barrier.
-static constexpr int LS_BLOCK = 256, LS_LPR = 8, LS_ROWS = LS_BLOCK/LS_LPR;
-static constexpr int LS_ENTRY_CAP = 640;          // staged blocks per workgroup (48 KB)
+//    LDS once per solve, so an SpMV touches HBM only for neighbours' vectors;
+//  * the inner solver is pipelined PCG (Ghysels & Vanroose 2014): the
+//    reduction of (r,u), (w,u), (r,r) and the exchange of m = M^-1 w share one
+//    grid barrier per iteration; m is double-buffered so a workgroup may
+//    publish m_{i+1} while others still read m_i; if a refinement pass fails
+//    to halve the true residual the solve continues with classic PCG;
+//  * cross-lane traffic uses DPP lane moves, block sums a DPP tree, and the
+//    grid barrier is a two-level arrival tree with a generation word.
+static constexpr int LS_BLOCK = 1024, LS_LPR = 8, LS_ROWS = LS_BLOCK/LS_LPR, LS_WAVES = LS_BLOCK/WAVE;
+static constexpr int LS_ENTRY_CAP = 1536;         // staged blocks per workgroup (117 KB)
+
+// ---- DPP lane moves (a DPP row is 16 lanes)
+template <int CTRL> __device__ inline int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL> __device__ inline float dpp_f(float v) { return __int_as_float(dpp_i<CTRL>(__float_as_int(v))); }
+template <int CTRL> __device__ inline double dpp_d(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(dpp_i<CTRL>(hi), dpp_i<CTRL>(lo));
+}
+static constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+// sum over each aligned group of 8 lanes, every lane of the group gets it
+__device__ inline float sum8f(float v)
+{
+    v += dpp_f<DPP_XOR1>(v);
+    v += dpp_f<DPP_XOR2>(v);
+    v += dpp_f<DPP_HALF_MIRROR>(v);
+    return v;
+}
+__device__ inline double sum16d(double v)
+{
+    v += dpp_d<DPP_XOR1>(v);
+    v += dpp_d<DPP_XOR2>(v);
+    v += dpp_d<DPP_HALF_MIRROR>(v);
+    v += dpp_d<DPP_MIRROR>(v);
+    return v;
+}
+// wave-uniform sum of a double over the 64 lanes, fixed order
+__device__ inline double wave_sum_d(double v)
+{
+    v = sum16d(v);
+    double r = 0;
+    for (int k = 0; k < 4; k++) {
+        int lo = __builtin_amdgcn_readlane(__double2loint(v), 16*k);
+        int hi = __builtin_amdgcn_readlane(__double2hiint(v), 16*k);
+        r += __hiloint2double(hi, lo);
+    }
+    return r;
+}
+// value of lane M of this lane's 8-lane group (sub = lane % 8)
+template <int M> __device__ inline float bcast8(float v, int sub)
+{
+    constexpr int q = M & 3;
+    constexpr int perm = q | (q << 2) | (q << 4) | (q << 6);
+    float x = dpp_f<perm>(v);
+    float y = dpp_f<DPP_HALF_MIRROR>(x);
+    return ((sub < 4) == (M < 4)) ? x : y;
+}
+__device__ inline void bcast8_6(float v, int sub, float* o)
+{
+    o[0] = bcast8<0>(v, sub); o[1] = bcast8<1>(v, sub); o[2] = bcast8<2>(v, sub);
+    o[3] = bcast8<3>(v, sub); o[4] = bcast8<4>(v, sub); o[5] = bcast8<5>(v, sub);
+}
 
 // barrier words: [0] root count, [32] generation, [64 + 32*leaf] leaf counts (128-byte spacing)
 __device__ inline unsigned ld_acquire(unsigned* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
+// Arrival and wait of one workgroup, run by a single thread after a
+// __syncthreads: release on arrival, acquire on the generation load.
+__device__ inline void tree_barrier_core(unsigned* bar, int G, unsigned& gen)
+{
+    const int leaf = blockIdx.x >> 3, nleaf = (G+7) >> 3;
+    const int lsize = min(8, G-(leaf << 3));
+    unsigned* lc = bar+64+32*leaf;
+    unsigned old = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == (unsigned)(lsize-1)) {
+        __hip_atomic_store(lc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned r = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (r == (unsigned)(nleaf-1)) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(bar+32, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    while (ld_acquire(bar+32) == gen) __builtin_amdgcn_s_sleep(1);
+    gen++;
+}
+
 __device__ inline void tree_barrier(unsigned* bar, int G, unsigned& gen)
 {
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int leaf = blockIdx.x >> 3, nleaf = (G+7) >> 3;
-        const int lsize = min(8, G-(leaf << 3));
-        unsigned* lc = bar+64+32*leaf;
-        unsigned old = atomicAdd(lc, 1u);
-        if (old == (unsigned)(lsize-1)) {
-            atomicExch(lc, 0u);
-            unsigned r = atomicAdd(bar, 1u);
-            if (r == (unsigned)(nleaf-1)) {
-                atomicExch(bar, 0u);
-                __threadfence();
-                atomicAdd(bar+32, 1u);
-            }
-        }
-        while (ld_acquire(bar+32) == gen) __builtin_amdgcn_s_sleep(1);
-        gen++;
-        __threadfence();
-    }
+    if (threadIdx.x == 0) tree_barrier_core(bar, G, gen);
     __syncthreads();
 }
 
@@ -1203,19 +1264,26 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     __shared__ int ent_j[LS_ENTRY_CAP];             // partner; ~partner for the transposed block
     __shared__ float ent_b[LS_ENTRY_CAP*18];
     __shared__ LdsRow rows[LS_ROWS];
-    __shared__ double red[3*LS_BLOCK/WAVE+4];
+    __shared__ double red[3*LS_WAVES+4];
     const int G = gridDim.x, n = A.n;
     const int lr = threadIdx.x/LS_LPR, c = threadIdx.x%LS_LPR;
+    const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
     const int i = blockIdx.x*LS_ROWS+lr;
     const bool live = i < n;
+    const bool own = live && c < 6;
+    const size_t gi = 6*(size_t)i+c;
     unsigned gen = ld_acquire(bar+32);
-    // ---- stage this workgroup's rows
+    // ---- stage this workgroup's rows: degrees in parallel, serial prefix in LDS
+    if (threadIdx.x < LS_ROWS) {
+        int ii = blockIdx.x*LS_ROWS+threadIdx.x;
+        rows[threadIdx.x].ne = ii < n ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         int used = 0;
         for (int t = 0; t < LS_ROWS; t++) {
-            int ii = blockIdx.x*LS_ROWS+t;
-            int deg = ii < n ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
-            if (ii < n && used+deg <= LS_ENTRY_CAP) { rows[t].e0 = used; rows[t].ne = deg; used += deg; }
+            int ii = blockIdx.x*LS_ROWS+t, deg = rows[t].ne;
+            if (ii < n && used+deg <= LS_ENTRY_CAP) { rows[t].e0 = used; used += deg; }
             else { rows[t].e0 = 0; rows[t].ne = ii < n ? -1 : 0; }
         }
     }
@@ -1233,12 +1301,12 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     }
     // row c of D and of D^-1 in registers (lanes 0..5)
     float Dr[6] = {0, 0, 0, 0, 0, 0}, Mr[6] = {0, 0, 0, 0, 0, 0};
-    if (live && c < 6) {
+    if (own) {
         for (int m = 0; m < 6; m++) {
             float e6[6] = {0, 0, 0, 0, 0, 0};
             e6[m] = 1.f;
             float y[6] = {0, 0, 0, 0, 0, 0};
-            dmul<float, float>(A.diag32+15*(size_t)i, e6, y);
+            dmul<float, float>(A.diag32+15*(size_t)i, e6, y);   // column m of the symmetric D
             Dr[m] = y[c];
             Mr[m] = A.minv32[36*(size_t)i+6*c+m];
         }
@@ -1246,7 +1314,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     __syncthreads();
     const int ne = live ? rows[lr].ne : 0, e0 = live ? rows[lr].e0 : 0;
 
-    // y = A32 v for this row (v of neighbours read from global vbuf, own v in registers)
+    // component c of A32 v for this row: neighbours' v from vbuf, own v from the group's lanes
     auto spmv32 = [&](const float* vbuf, float vown) -> float {
         float y[6] = {0, 0, 0, 0, 0, 0};
         if (live) {
@@ -1281,63 +1349,74 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                 }
             }
         }
-        for (int ofs = 1; ofs < LS_LPR; ofs <<= 1)
-            for (int m = 0; m < 6; m++) y[m] += __shfl_xor(y[m], ofs, LS_LPR);
-        // diagonal part: own v gathered from the 6 owning lanes
-        float yc = 0;
-        const int base = (threadIdx.x & (WAVE-1)) & ~(LS_LPR-1);
-        for (int m = 0; m < 6; m++) {
-            float vm = __shfl(vown, base+m, WAVE);
-            yc += Dr[m]*vm;
-        }
-        float mine = 0;
-        for (int m = 0; m < 6; m++) if (m == c) mine = y[m];
-        return mine+yc;
+        for (int m = 0; m < 6; m++) y[m] = sum8f(y[m]);
+        float vv[6];
+        bcast8_6(vown, c, vv);
+        float yc = 0, mine = 0;
+        for (int m = 0; m < 6; m++) { yc += Dr[m]*vv[m]; if (m == c) mine = y[m]; }
+        return own ? mine+yc : 0.f;
+    };
+    auto precond = [&](float rv) -> float {
+        float rr[6];
+        bcast8_6(rv, c, rr);
+        float z = 0;
+        for (int m = 0; m < 6; m++) z += Mr[m]*rr[m];
+        return own ? z : 0.f;
     };
 
     double* part = w.partials;
     int buf = 0;
-    // block sum of three values, publish, tree barrier, ordered grid reduction
-    auto gsum3 = [&](double& a, double& bsum, double& csum) {
-        for (int ofs = WAVE/2; ofs > 0; ofs >>= 1) {
-            a += __shfl_xor(a, ofs, WAVE); bsum += __shfl_xor(bsum, ofs, WAVE); csum += __shfl_xor(csum, ofs, WAVE);
-        }
-        const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
-        if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bsum; red[3*wid+2] = csum; }
+    // block sums of three values (DPP tree, fixed order) written to the
+    // current partial buffer; then one grid barrier
+    auto publish3 = [&](double a, double bs, double cs) -> double* {
+        a = wave_sum_d(a); bs = wave_sum_d(bs); cs = wave_sum_d(cs);
+        if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bs; red[3*wid+2] = cs; }
         __syncthreads();
         double* pb = part+(size_t)buf*3*G;
-        if (threadIdx.x == 0) {
-            double s0 = 0, s1 = 0, s2 = 0;
-            for (int k = 0; k < LS_BLOCK/WAVE; k++) { s0 += red[3*k]; s1 += red[3*k+1]; s2 += red[3*k+2]; }
-            pb[3*blockIdx.x] = s0; pb[3*blockIdx.x+1] = s1; pb[3*blockIdx.x+2] = s2;
-        }
-        tree_barrier(bar, G, gen);
-        double s0 = 0, s1 = 0, s2 = 0;
-        if (threadIdx.x < WAVE) {
-            for (int g = threadIdx.x; g < G; g += WAVE) {
-                s0 += __hip_atomic_load(pb+3*g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s1 += __hip_atomic_load(pb+3*g+1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                s2 += __hip_atomic_load(pb+3*g+2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (wid == 0) {
+            double t0 = lane < LS_WAVES ? red[3*lane] : 0.0;
+            double t1 = lane < LS_WAVES ? red[3*lane+1] : 0.0;
+            double t2 = lane < LS_WAVES ? red[3*lane+2] : 0.0;
+            t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
+            if (lane == 0) {
+                pb[3*blockIdx.x] = t0; pb[3*blockIdx.x+1] = t1; pb[3*blockIdx.x+2] = t2;
+                tree_barrier_core(bar, G, gen);
             }
-            for (int ofs = WAVE/2; ofs > 0; ofs >>= 1) {
-                s0 += __shfl_xor(s0, ofs, WAVE); s1 += __shfl_xor(s1, ofs, WAVE); s2 += __shfl_xor(s2, ofs, WAVE);
-            }
-            if (threadIdx.x == 0) { red[3*LS_BLOCK/WAVE] = s0; red[3*LS_BLOCK/WAVE+1] = s1; red[3*LS_BLOCK/WAVE+2] = s2; }
         }
         __syncthreads();
-        a = red[3*LS_BLOCK/WAVE]; bsum = red[3*LS_BLOCK/WAVE+1]; csum = red[3*LS_BLOCK/WAVE+2];
+        return pb;
+    };
+    // grid sums of the published partials; wave 0 loads, caller syncs
+    auto gather3_start = [&](const double* pb, double& t0, double& t1, double& t2) {
+        t0 = t1 = t2 = 0;
+        if (wid == 0)
+            for (int q = lane; q < G; q += WAVE) {
+                t0 += __hip_atomic_load(pb+3*q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                t1 += __hip_atomic_load(pb+3*q+1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                t2 += __hip_atomic_load(pb+3*q+2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+    };
+    auto gather3_finish = [&](double& t0, double& t1, double& t2) {
+        if (wid == 0) {
+            t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
+            if (lane == 0) { red[3*LS_WAVES] = t0; red[3*LS_WAVES+1] = t1; red[3*LS_WAVES+2] = t2; }
+        }
+        __syncthreads();
+        t0 = red[3*LS_WAVES]; t1 = red[3*LS_WAVES+1]; t2 = red[3*LS_WAVES+2];
         __syncthreads();
         buf ^= 1;
     };
-    auto gsum2 = [&](double& a, double& bsum) { double t = 0; gsum3(a, bsum, t); };
+    auto gsum3 = [&](double& a, double& bs, double& cs) {
+        const double* pb = publish3(a, bs, cs);
+        gather3_start(pb, a, bs, cs);
+        gather3_finish(a, bs, cs);
+    };
 
-    const bool own = live && c < 6;
-    const size_t gi = 6*(size_t)i+c;
     double bb = 0, rr64 = 0;
     int outer, inner_total = 0;
     bool converged = false, use_classic = false;
     for (outer = 0; outer <= o.max_outer; outer++) {
-        // FP64 residual from global FP64 blocks
+        // FP64 residual from the FP64 blocks in HBM
         double yv[6] = {0, 0, 0, 0, 0, 0};
         if (live) spmv_row<double, double, LS_LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, c, yv);
         double r64 = 0, bi = 0;
@@ -1346,87 +1425,73 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             bi = b[gi];
             r64 = bi-r64;
         }
-        double s_rr = r64*r64, s_bb = bi*bi;
-        gsum2(s_rr, s_bb);
+        double s_rr = r64*r64, s_bb = bi*bi, s_0 = 0;
+        gsum3(s_rr, s_bb, s_0);
         if (outer == 0) bb = s_bb;
         if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
         rr64 = s_rr;
         converged = bb == 0 || sqrt(rr64) <= o.tol*sqrt(bb);
         if (converged || outer == o.max_outer) break;
         const double scale = 1.0/sqrt(rr64);
-        const int base = (threadIdx.x & (WAVE-1)) & ~(LS_LPR-1);
-        auto precond = [&](float rv) -> float {
-            float z = 0;
-            for (int m = 0; m < 6; m++) { float rm = __shfl(rv, base+m, WAVE); z += Mr[m]*rm; }
-            return own ? z : 0.f;
-        };
         float d = 0.f;
         int it = 0;
         if (!use_classic) {
-        // inner FP32 pipelined PCG (Ghysels & Vanroose 2014), d0 = 0: the
-        // reduction of (r,u), (w,u), (r,r) and the neighbour exchange of
-        // m = M^-1 w share one barrier per iteration; m is double-buffered so
-        // a workgroup may publish m_{i+1} while others still read m_i.
-        float* mbuf[2] = {w.z, w.q};
-        float r = own ? (float)(r64*scale) : 0.f;
-        float u = precond(r);
-        if (own) mbuf[1][gi] = u;
-        double s_r2 = (double)r*r, dummy = 0, dummy2 = 0;
-        gsum3(s_r2, dummy, dummy2);
-        const double stop2 = o.inner_tol*o.inner_tol*s_r2;
-        float wv = spmv32(mbuf[1], u);
-        if (!own) wv = 0.f;
-        float zv = 0, qv = 0, sv = 0, pv = 0;
-        d = 0;
-        double gamma_old = 1, alpha_old = 1;
-        int par = 0;
-        for (it = 0; it < o.max_inner; it++) {
-            double g = (double)r*u, dl = (double)wv*u, rho = (double)r*r;
-            float mv = precond(wv);
-            if (own) mbuf[par][gi] = mv;
-            gsum3(g, dl, rho);
-            if (rho <= stop2) break;
-            double beta, alpha;
-            if (it > 0) {
-                beta = g/gamma_old;
-                double den = dl-beta*g/alpha_old;
-                if (!(den > 0)) break;
-                alpha = g/den;
-            } else {
-                if (!(dl > 0)) break;
-                beta = 0;
-                alpha = g/dl;
+            float* mbuf[2] = {w.z, w.q};
+            float r = own ? (float)(r64*scale) : 0.f;
+            float u = precond(r);
+            if (own) mbuf[1][gi] = u;
+            double s_r2 = (double)r*r, z0 = 0, z1 = 0;
+            gsum3(s_r2, z0, z1);
+            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            float wv = spmv32(mbuf[1], u);
+            float zv = 0, qv = 0, sv = 0, pv = 0;
+            double gamma_old = 1, alpha_old = 1;
+            int par = 0;
+            for (it = 0; it < o.max_inner; it++) {
+                float mv = precond(wv);
+                if (own) mbuf[par][gi] = mv;
+                const double* pb = publish3((double)r*u, (double)wv*u, (double)r*r);
+                double g, dl, rho;
+                gather3_start(pb, g, dl, rho);     // overlaps with the neighbour gathers below
+                float nv = spmv32(mbuf[par], mv);
+                gather3_finish(g, dl, rho);
+                if (rho <= stop2) break;
+                double beta, alpha;
+                if (it > 0) {
+                    beta = g/gamma_old;
+                    double den = dl-beta*g/alpha_old;
+                    if (!(den > 0)) break;
+                    alpha = g/den;
+                } else {
+                    if (!(dl > 0)) break;
+                    beta = 0;
+                    alpha = g/dl;
+                }
+                const float bf = (float)beta, af = (float)alpha;
+                zv = nv+bf*zv;
+                qv = mv+bf*qv;
+                sv = wv+bf*sv;
+                pv = u+bf*pv;
+                d += af*pv;
+                r -= af*sv;
+                u -= af*qv;
+                wv -= af*zv;
+                gamma_old = g;
+                alpha_old = alpha;
+                par ^= 1;
             }
-            float nv = spmv32(mbuf[par], mv);
-            if (!own) nv = 0.f;
-            const float bf = (float)beta, af = (float)alpha;
-            zv = nv+bf*zv;
-            qv = mv+bf*qv;
-            sv = wv+bf*sv;
-            pv = u+bf*pv;
-            d += af*pv;
-            r -= af*sv;
-            u -= af*qv;
-            wv -= af*zv;
-            gamma_old = g;
-            alpha_old = alpha;
-            par ^= 1;
-        }
         } else {
-            // classic PCG (two barriers per iteration): fallback when the
-            // pipelined recurrences stop reducing the true residual
+            // classic PCG, two barriers per iteration
             float r = own ? (float)(r64*scale) : 0.f;
             float z = precond(r), p = z;
-            d = 0;
             if (own) w.z[gi] = z;
-            double s_rz = (double)r*z, s_r2 = (double)r*r;
-            gsum2(s_rz, s_r2);
+            double s_rz = (double)r*z, s_r2 = (double)r*r, z0 = 0;
+            gsum3(s_rz, s_r2, z0);
             double rz = s_rz;
             const double stop2 = o.inner_tol*o.inner_tol*s_r2;
             float q = spmv32(w.z, z);
-            if (!own) q = 0.f;
-            double s_pq = (double)p*q, dmy = 0;
-            gsum2(s_pq, dmy);
+            double s_pq = (double)p*q, z1 = 0, z2 = 0;
+            gsum3(s_pq, z1, z2);
             double pq = s_pq;
             for (it = 0; it < o.max_inner; it++) {
                 if (!(pq > 0)) break;
@@ -1435,18 +1500,16 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                 r -= alpha*q;
                 z = precond(r);
                 if (own) w.z[gi] = z;
-                double s_rz2 = (double)r*z, s_rr2 = (double)r*r;
-                gsum2(s_rz2, s_rr2);
+                double s_rz2 = (double)r*z, s_rr2 = (double)r*r, z3 = 0;
+                gsum3(s_rz2, s_rr2, z3);
                 if (s_rr2 <= stop2) { it++; break; }
                 const float beta = (float)(s_rz2/rz);
                 rz = s_rz2;
                 float wz = spmv32(w.z, z);
-                if (!own) wz = 0.f;
                 p = z+beta*p;
                 q = wz+beta*q;
-                double s_pq2 = (double)p*q;
-                dmy = 0;
-                gsum2(s_pq2, dmy);
+                double s_pq2 = (double)p*q, z4 = 0, z5 = 0;
+                gsum3(s_pq2, z4, z5);
                 pq = s_pq2;
             }
         }
