@@ -82,7 +82,7 @@ def run(args):
     del P
     # roofline of the dominant kernel (the persistent solver launch)
     avg_ms = solve_ms / max(launches, 1)
-    outer_per_solve = 3.0
+    outer_per_solve = float(max(st1["iters_outer"], 1))     # refinement passes of the last solve
     alg_bytes = iters_per_solve * solver_bytes_per_iter(args.n, npairs) + \
         outer_per_solve * solver_bytes_per_outer(args.n, npairs)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -100,7 +100,7 @@ def run(args):
                        "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
         "solver": {"inner_iters_per_solve": round(iters_per_solve, 1), "solve_launches": launches,
                    "avg_solve_ms": round(avg_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_solve_mixed", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_solve_lds", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None},
         "confgen_s": round(gen_s, 2),

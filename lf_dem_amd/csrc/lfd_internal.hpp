@@ -92,7 +92,8 @@ struct CellGrid {
 void launch_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc, hipStream_t s);
 void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int* mem, int* cursor,
                 int* scan_tmp, hipStream_t s);
-void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, DevScalars* err, hipStream_t s);
+void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, int* probe, DevScalars* err,
+                             hipStream_t s);
 void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_start, const int* home_mem,
                        const int* snap_start, const int* snap_mem, const DevScalars* sc, Consts C,
                        int* found, int* found_cnt, int K, DevScalars* err, hipStream_t s);
@@ -105,11 +106,11 @@ void launch_compact_pairs(const PairRec* in, PairRec* out, const int* keep, int*
                           int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, DevScalars* sc,
                           int pcap, int* scan_tmp, hipStream_t s);
 void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
-                       int* tcnt, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
+                       int* tcnt, int* tmp, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
 void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
                      const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
-                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, int pcap,
-                     DevScalars* err, hipStream_t s);
+                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, double* pdb,
+                     double* prhs, int pcap, DevScalars* err, hipStream_t s);
 void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float* diag32, const double* od64,
                            float* od32, double* minv64, float* minv32, int pcap, DevScalars* err, hipStream_t s);
 void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,

@@ -191,26 +191,40 @@ void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int*
 
 // BoxSet::updateNeighbors (BoxSet.cpp:408-442): moving neighbours of top and
 // bottom cells from the 16 probing positions, with the current LE shift.
-__global__ void k_moving_neighbors(CellGrid g, const DevScalars* sc, Consts C, DevScalars* err)
+// One thread per (cell, probe) computes the probe's cell label; a second pass
+// appends the new labels to each cell's list (top probes first).
+__global__ void k_probe_labels(CellGrid g, const DevScalars* sc, Consts C, int* probe, DevScalars* err)
+{
+    int t = blockIdx.x*blockDim.x+threadIdx.x;
+    int b = t >> 5, k = t & 31;
+    if (b >= g.nbox) return;
+    int ty = g.ctype[b];
+    bool top = k < 16;
+    bool want = g.boxed && ((top && (ty == 2 || ty == 3)) || (!top && (ty == 1 || ty == 3)));
+    int l = -1;
+    if (want) {
+        unsigned ix = b/g.yz, iy = (b%g.yz)/g.nz, iz = b%g.nz;
+        double c[3] = {g.bx*(ix+0.5), g.by*(iy+0.5), g.bz*(iz+0.5)};
+        double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+        const double* pr = top ? g.probe_top[k] : g.probe_bot[k-16];
+        double q[3] = {c[0]+pr[0], c[1]+pr[1], c[2]+pr[2]};
+        periodize(C.L, sd, q);
+        l = which_box(q, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
+        if (l < 0) raise_error(err, E_BOX_OOB);
+        if (l == b) l = -1;
+    }
+    probe[t] = l;
+}
+__global__ void k_moving_merge(CellGrid g, const int* probe)
 {
     int b = blockIdx.x*blockDim.x+threadIdx.x;
     if (b >= g.nbox) return;
-    int t = g.ctype[b];
     int cnt = g.nstatic[b];
-    if (t == 0 || !g.boxed) { g.nnb[b] = cnt; return; }
-    unsigned ix = b/g.yz, iy = (b%g.yz)/g.nz, iz = b%g.nz;
-    double c[3] = {g.bx*(ix+0.5), g.by*(iy+0.5), g.bz*(iz+0.5)};
-    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
     int* nb = g.nb+64*b;
-    for (int side = 0; side < 2; side++) {
-        if (side == 0 && !(t == 2 || t == 3)) continue;
-        if (side == 1 && !(t == 1 || t == 3)) continue;
-        for (int k = 0; k < 16; k++) {
-            const double* pr = side == 0 ? g.probe_top[k] : g.probe_bot[k];
-            double q[3] = {c[0]+pr[0], c[1]+pr[1], c[2]+pr[2]};
-            periodize(C.L, sd, q);
-            int l = which_box(q, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
-            if (l < 0) { raise_error(err, E_BOX_OOB); continue; }
+    if (g.ctype[b] != 0 && g.boxed) {
+        for (int k = 0; k < 32; k++) {
+            int l = probe[32*b+k];
+            if (l < 0) continue;
             bool dup = false;
             for (int m = 0; m < cnt; m++) if (nb[m] == l) { dup = true; break; }
             if (!dup && cnt < 64) nb[cnt++] = l;
@@ -218,11 +232,13 @@ __global__ void k_moving_neighbors(CellGrid g, const DevScalars* sc, Consts C, D
     }
     g.nnb[b] = cnt;
 }
-void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, DevScalars* err, hipStream_t s)
+void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, int* probe, DevScalars* err,
+                             hipStream_t s)
 {
     Consts C{};
     C.L[0] = L[0]; C.L[1] = L[1]; C.L[2] = L[2];
-    k_moving_neighbors<<<(g.nbox+255)/256, 256, 0, s>>>(g, sc, C, err);
+    k_probe_labels<<<(32*g.nbox+255)/256, 256, 0, s>>>(g, sc, C, probe, err);
+    k_moving_merge<<<(g.nbox+255)/256, 256, 0, s>>>(g, probe);
 }
 
 // ------------------------------------------------------------------ neighbour search
@@ -343,7 +359,7 @@ __global__ void k_merge_count(int n, const PairRec* old_pairs, const int* old_ro
     }
     cnt[i] = m;
 }
-__global__ void k_merge_write(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
+__global__ __launch_bounds__(256) void k_merge_write(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
                               const int* found_cnt, int K, const int* new_row_ptr, PairRec* new_pairs,
                               const double4* pos, Consts C, const DevScalars* sc, int pcap, DevScalars* err)
 {
@@ -399,7 +415,7 @@ void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_row_ptr,
 // ------------------------------------------------------------------ pair update
 // System::updateInteractions (System.cpp:1375-1403) without the swap-with-last
 // removal: every pair is updated once and removals are compacted afterwards.
-__global__ void k_update_pairs(PairRec* pairs, const double4* __restrict__ pos, const double* __restrict__ vel6,
+__global__ __launch_bounds__(256) void k_update_pairs(PairRec* pairs, const double4* __restrict__ pos, const double* __restrict__ vel6,
                                Consts C, DevScalars* sc, int* keep, int pcap, int in_predictor, double dt_override)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
@@ -441,7 +457,7 @@ void launch_compact_pairs(const PairRec* in, PairRec* out, const int* keep, int*
     exclusive_scan_int(keep, keep_scan, &sc->npairs, pcap, scan_tmp, s);
     k_compact<<<(pcap+255)/256, 256, 0, s>>>(in, out, keep, keep_scan, sc);
     k_set_npairs<<<1, 1, 0, s>>>(sc, keep_scan);
-    launch_pair_index(out, n, row_ptr, p0, p1, tptr, tidx, tcnt, sc, pcap, scan_tmp, s);
+    launch_pair_index(out, n, row_ptr, p0, p1, tptr, tidx, tcnt, const_cast<int*>(keep), sc, pcap, scan_tmp, s);
 }
 
 // ------------------------------------------------------------------ pair index
@@ -461,85 +477,38 @@ __global__ void k_row_ptr(const int* p0, int n, int* row_ptr, const DevScalars* 
     while (lo < hi) { int m = (lo+hi) >> 1; if (p0[m] < i) lo = m+1; else hi = m; }
     row_ptr[i] = lo;
 }
-__global__ void k_tfill(const int* p1, int* cursor, int* tidx, const DevScalars* sc)
+__global__ void k_tfill(const int* p1, int* cursor, int* tmp, const DevScalars* sc)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= sc->npairs) return;
-    tidx[atomicAdd(&cursor[p1[k]], 1)] = k;
+    tmp[atomicAdd(&cursor[p1[k]], 1)] = k;
 }
-// lower incidence lists in ascending pair index, i.e. ascending p0
-__global__ void k_tsort(const int* tptr, int n, int* tidx)
+// lower incidence lists in ascending pair index (= ascending p0): each entry
+// is placed by its rank within the list
+__global__ void k_tsort(const int* tptr, int n, const int* tmp, int* tidx)
 {
-    int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i >= n) return;
-    int s0 = tptr[i], s1 = tptr[i+1];
-    for (int a = s0+1; a < s1; a++) {
-        int x = tidx[a], c = a-1;
-        while (c >= s0 && tidx[c] > x) { tidx[c+1] = tidx[c]; c--; }
-        tidx[c+1] = x;
+    int t = blockIdx.x*blockDim.x+threadIdx.x;
+    if (t >= n) return;
+    int s0 = tptr[t], s1 = tptr[t+1];
+    for (int a = s0; a < s1; a++) {
+        int x = tmp[a], rank = 0;
+        for (int b = s0; b < s1; b++) rank += tmp[b] < x;
+        tidx[s0+rank] = x;
     }
 }
 void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
-                       int* tcnt, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
+                       int* tcnt, int* tmp, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
 {
     launch_fill_int(tcnt, 0, n, s);
     k_pair_arrays<<<(pcap+255)/256, 256, 0, s>>>(pairs, p0, p1, tcnt, sc);
     k_row_ptr<<<(n+1+255)/256, 256, 0, s>>>(p0, n, row_ptr, sc);
     exclusive_scan_int(tcnt, tptr, nullptr, n, scan_tmp, s);
     k_copy_int<<<(n+255)/256, 256, 0, s>>>(tcnt, tptr, n);
-    k_tfill<<<(pcap+255)/256, 256, 0, s>>>(p1, tcnt, tidx, sc);
-    k_tsort<<<(n+255)/256, 256, 0, s>>>(tptr, n, tidx);
+    k_tfill<<<(pcap+255)/256, 256, 0, s>>>(p1, tcnt, tmp, sc);
+    k_tsort<<<(n+255)/256, 256, 0, s>>>(tptr, n, tmp, tidx);
 }
 
 // ------------------------------------------------------------------ assembly
-__device__ inline void dense_from_dblock(const double* b, double M[6][6])
-{
-    // lower triangle per StokesSolver::insertDBlockValues (StokesSolver.cpp:154-184)
-    double L[6][6] = {};
-    L[0][0] = b[0]; L[1][0] = b[1]; L[2][0] = b[2]; L[4][0] = b[3]; L[5][0] = b[4];
-    L[1][1] = b[5]; L[2][1] = b[6]; L[3][1] = -b[3]; L[5][1] = b[7];
-    L[2][2] = b[8]; L[3][2] = -b[4]; L[4][2] = -b[7];
-    L[3][3] = b[9]; L[4][3] = b[10]; L[5][3] = b[11];
-    L[4][4] = b[12]; L[5][4] = b[13];
-    L[5][5] = b[14];
-    for (int r = 0; r < 6; r++)
-        for (int c = 0; c < 6; c++) M[r][c] = r >= c ? L[r][c] : L[c][r];
-}
-
-// inverse of an SPD 6x6 block through its Cholesky factor; returns false if not SPD
-__device__ inline bool spd_inverse6(double M[6][6], double Inv[6][6])
-{
-    double Lc[6][6] = {};
-    for (int j = 0; j < 6; j++) {
-        double d = M[j][j];
-        for (int k = 0; k < j; k++) d -= Lc[j][k]*Lc[j][k];
-        if (!(d > 0)) return false;
-        d = sqrt(d);
-        Lc[j][j] = d;
-        for (int i = j+1; i < 6; i++) {
-            double v = M[i][j];
-            for (int k = 0; k < j; k++) v -= Lc[i][k]*Lc[j][k];
-            Lc[i][j] = v/d;
-        }
-    }
-    // inverse column by column: solve L L^T x = e_c
-    for (int c = 0; c < 6; c++) {
-        double y[6];
-        for (int r = 0; r < 6; r++) {
-            double v = (r == c) ? 1.0 : 0.0;
-            for (int k = 0; k < r; k++) v -= Lc[r][k]*y[k];
-            y[r] = v/Lc[r][r];
-        }
-        for (int r = 5; r >= 0; r--) {
-            double v = y[r];
-            for (int k = r+1; k < 6; k++) v -= Lc[k][r]*y[k];
-            y[r] = v/Lc[r][r];
-        }
-        for (int r = 0; r < 6; r++) Inv[r][c] = y[r];
-    }
-    return true;
-}
-
 __device__ inline void uinf(const Consts& C, const double* x, double* u)
 {
     // System::computeUInf (System.cpp:2357-2367)
@@ -634,89 +603,154 @@ __device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, doub
     }
 }
 
-// Off-diagonal blocks, one thread per pair.
-__global__ void k_assemble_od(const PairRec* __restrict__ pairs, const double4* __restrict__ pos, Consts C,
-                              const DevScalars* sc, double* __restrict__ od64, float* __restrict__ od32, int pcap)
+// Lower triangle (row r >= col c) of the symmetric 6x6 block of a packed
+// DBlock, as laid out by StokesSolver::insertDBlockValues (StokesSolver.cpp:154-184),
+// packed row by row: index r*(r+1)/2 + c.
+__device__ inline void dblock_lower_packed(const double* b, double* Lp)
+{
+    Lp[0] = b[0];
+    Lp[1] = b[1]; Lp[2] = b[5];
+    Lp[3] = b[2]; Lp[4] = b[6]; Lp[5] = b[8];
+    Lp[6] = 0; Lp[7] = -b[3]; Lp[8] = -b[4]; Lp[9] = b[9];
+    Lp[10] = b[3]; Lp[11] = 0; Lp[12] = -b[7]; Lp[13] = b[10]; Lp[14] = b[12];
+    Lp[15] = b[4]; Lp[16] = b[7]; Lp[17] = 0; Lp[18] = b[11]; Lp[19] = b[13]; Lp[20] = b[14];
+}
+#define LP(r, c) Lp[(r)*((r)+1)/2+(c)]
+// Block-Jacobi inverse of a packed DBlock through an in-place packed
+// Cholesky factor; writes the dense row-major inverse in FP64 and FP32.
+__device__ inline bool dblock_inverse(const double* b, double* inv64, float* inv32)
+{
+    double Lp[21];
+    dblock_lower_packed(b, Lp);
+    #pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double d = LP(j, j);
+        #pragma unroll
+        for (int k = 0; k < j; k++) d -= LP(j, k)*LP(j, k);
+        if (!(d > 0)) return false;
+        d = sqrt(d);
+        LP(j, j) = d;
+        #pragma unroll
+        for (int i = j+1; i < 6; i++) {
+            double v = LP(i, j);
+            #pragma unroll
+            for (int k = 0; k < j; k++) v -= LP(i, k)*LP(j, k);
+            LP(i, j) = v/d;
+        }
+    }
+    #pragma unroll
+    for (int c = 0; c < 6; c++) {
+        double y[6];
+        #pragma unroll
+        for (int r = 0; r < 6; r++) {
+            double v = (r == c) ? 1.0 : 0.0;
+            #pragma unroll
+            for (int k = 0; k < r; k++) v -= LP(r, k)*y[k];
+            y[r] = v/LP(r, r);
+        }
+        #pragma unroll
+        for (int r = 5; r >= 0; r--) {
+            double v = y[r];
+            #pragma unroll
+            for (int k = r+1; k < 6; k++) v -= LP(k, r)*y[k];
+            y[r] = v/LP(r, r);
+        }
+        #pragma unroll
+        for (int r = 0; r < 6; r++) { inv64[6*r+c] = y[r]; inv32[6*r+c] = (float)y[r]; }
+    }
+    return true;
+}
+#undef LP
+
+// Per pair: off-diagonal block (FP64 and FP32 copies), the pair's two
+// diagonal contributions and its right-hand-side terms for both particles.
+// pdb[30*k]: DBlock of p0 then of p1; prhs[42*k]: RhsAcc of p0 then of p1.
+static constexpr int RHS_TERMS = 21;
+__global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restrict__ pairs, const double4* __restrict__ pos, Consts C,
+                                 const DevScalars* sc, double* __restrict__ od64, float* __restrict__ od32,
+                                 double* __restrict__ pdb, double* __restrict__ prhs, int pcap)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= sc->npairs || k >= pcap) return;
-    PairRec q = pairs[k];
+    const PairRec q = pairs[k];
+    const double4 x04 = pos[q.p0], x14 = pos[q.p1];
     double db0[15], db1[15], od[18];
-    pair_blocks(q, pos[q.p0].w, pos[q.p1].w, C, db0, db1, od);
+    pair_blocks(q, x04.w, x14.w, C, db0, db1, od);
     for (int m = 0; m < 18; m++) { od64[18*(size_t)k+m] = od[m]; od32[18*(size_t)k+m] = (float)od[m]; }
+    for (int m = 0; m < 15; m++) { pdb[30*(size_t)k+m] = db0[m]; pdb[30*(size_t)k+15+m] = db1[m]; }
+    const double x0[3] = {x04.x, x04.y, x04.z}, x1[3] = {x14.x, x14.y, x14.z};
+    for (int side = 0; side < 2; side++) {
+        RhsAcc R;
+        for (int m = 0; m < 3; m++) {
+            R.Fc[m] = 0; R.Tc[m] = 0; R.Fd[m] = 0; R.Td[m] = 0; R.Fh[m] = 0; R.Th[m] = 0; R.Fr[m] = 0;
+        }
+        pair_rhs_side(q, side, x04.w, x14.w, x0, x1, C, R);
+        double* o = prhs+42*(size_t)k+RHS_TERMS*side;
+        for (int m = 0; m < 3; m++) {
+            o[m] = R.Fc[m]; o[3+m] = R.Tc[m]; o[6+m] = R.Fd[m]; o[9+m] = R.Td[m];
+            o[12+m] = R.Fh[m]; o[15+m] = R.Th[m]; o[18+m] = R.Fr[m];
+        }
+    }
 }
 
-// Diagonal blocks (drag seed, System.cpp:558-575, then pair blocks in
-// ascending partner order, System.cpp:1459-1471), block-Jacobi inverse and
-// the right-hand side (StokesSolver.cpp:513-549), one thread per particle.
-__global__ void k_assemble_diag_rhs(const PairRec* __restrict__ pairs, const double4* __restrict__ pos, int n,
-                                    Consts C, const int* __restrict__ row_ptr, const int* __restrict__ tptr,
-                                    const int* __restrict__ tidx, double* __restrict__ diag64,
-                                    float* __restrict__ diag32, double* __restrict__ minv64, float* __restrict__ minv32,
-                                    double* __restrict__ rhs, DevScalars* err)
+// Per particle: diagonal block = drag seed (System.cpp:558-575) plus the
+// pair contributions in ascending partner order (System.cpp:1459-1471),
+// block-Jacobi inverse, and the RHS summed per force component in the same
+// order, then across components in LF_DEM's std::map order
+// (StokesSolver.cpp:513-549, System.cpp:1873-1885).
+__global__ __launch_bounds__(128) void k_gather_diag_rhs(const double* __restrict__ pdb, const double* __restrict__ prhs,
+                                  const double4* __restrict__ pos, int n, Consts C, const int* __restrict__ row_ptr,
+                                  const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                  double* __restrict__ diag64, float* __restrict__ diag32,
+                                  double* __restrict__ minv64, float* __restrict__ minv32,
+                                  double* __restrict__ rhs, DevScalars* err)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
-    double4 xi4 = pos[i];
-    double a = xi4.w;
+    const double a = pos[i].w;
     double D[15];
     for (int m = 0; m < 15; m++) D[m] = 0;
-    double FU = C.sd_coeff*a;
-    double TW = C.sd_coeff*(a*a*a)*4.0/3;
+    const double FU = C.sd_coeff*a;
+    const double TW = C.sd_coeff*(a*a*a)*4.0/3;
     D[0] = FU; D[5] = FU; D[8] = FU; D[9] = TW; D[12] = TW; D[14] = TW;
-    RhsAcc R;
-    for (int k = 0; k < 3; k++) {
-        R.Fc[k] = 0; R.Tc[k] = 0; R.Fd[k] = 0; R.Td[k] = 0; R.Fh[k] = 0; R.Th[k] = 0; R.Fr[k] = 0;
+    double R[RHS_TERMS];
+    for (int m = 0; m < RHS_TERMS; m++) R[m] = 0;
+    for (int t = tptr[i]; t < tptr[i+1]; t++) {           // pairs (j, i), j < i
+        const size_t k = tidx[t];
+        #pragma unroll
+        for (int m = 0; m < 15; m++) D[m] += pdb[30*k+15+m];
+        #pragma unroll
+        for (int m = 0; m < RHS_TERMS; m++) R[m] += prhs[42*k+RHS_TERMS+m];
     }
-    double xi[3] = {xi4.x, xi4.y, xi4.z};
-    // pairs (j, i), j < i: this particle is p1
-    for (int t = tptr[i]; t < tptr[i+1]; t++) {
-        const PairRec q = pairs[tidx[t]];
-        double4 x04 = pos[q.p0];
-        double x0[3] = {x04.x, x04.y, x04.z};
-        double db0[15], db1[15], od[18];
-        if (pair_blocks(q, x04.w, a, C, db0, db1, od)) for (int m = 0; m < 15; m++) D[m] += db1[m];
-        pair_rhs_side(q, 1, x04.w, a, x0, xi, C, R);
-    }
-    // pairs (i, j), j > i: this particle is p0
-    for (int k = row_ptr[i]; k < row_ptr[i+1]; k++) {
-        const PairRec q = pairs[k];
-        double4 x14 = pos[q.p1];
-        double x1[3] = {x14.x, x14.y, x14.z};
-        double db0[15], db1[15], od[18];
-        if (pair_blocks(q, a, x14.w, C, db0, db1, od)) for (int m = 0; m < 15; m++) D[m] += db0[m];
-        pair_rhs_side(q, 0, a, x14.w, xi, x1, C, R);
+    for (int k = row_ptr[i]; k < row_ptr[i+1]; k++) {     // pairs (i, j), j > i
+        #pragma unroll
+        for (int m = 0; m < 15; m++) D[m] += pdb[30*(size_t)k+m];
+        #pragma unroll
+        for (int m = 0; m < RHS_TERMS; m++) R[m] += prhs[42*(size_t)k+m];
     }
     for (int m = 0; m < 15; m++) { diag64[15*(size_t)i+m] = D[m]; diag32[15*(size_t)i+m] = (float)D[m]; }
-    double M[6][6], Inv[6][6];
-    dense_from_dblock(D, M);
-    if (!spd_inverse6(M, Inv)) raise_error(err, E_NOT_SPD);
-    for (int r = 0; r < 6; r++)
-        for (int c = 0; c < 6; c++) {
-            minv64[36*(size_t)i+6*r+c] = Inv[r][c];
-            minv32[36*(size_t)i+6*r+c] = (float)Inv[r][c];
-        }
-    double F[3], T[3];
-    for (int k = 0; k < 3; k++) {
-        F[k] = 0; T[k] = 0;
-        F[k] += R.Fc[k];
-        if (C.friction) T[k] += R.Tc[k];
-        F[k] += R.Fd[k];
-        T[k] += R.Td[k];
-        F[k] += R.Fh[k];
-        if (C.tangential) T[k] += R.Th[k];
-        if (C.repulsive) F[k] += R.Fr[k];
+    if (!dblock_inverse(D, minv64+36*(size_t)i, minv32+36*(size_t)i)) raise_error(err, E_NOT_SPD);
+    for (int m = 0; m < 3; m++) {
+        double F = 0, T = 0;
+        F += R[m];                           // contact
+        if (C.friction) T += R[3+m];
+        F += R[6+m];                         // dashpot
+        T += R[9+m];
+        F += R[12+m];                        // hydro
+        if (C.tangential) T += R[15+m];
+        if (C.repulsive) F += R[18+m];       // repulsion
+        rhs[6*(size_t)i+m] = F;
+        rhs[6*(size_t)i+3+m] = T;
     }
-    for (int k = 0; k < 3; k++) { rhs[6*(size_t)i+k] = F[k]; rhs[6*(size_t)i+3+k] = T[k]; }
 }
 void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
                      const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
-                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, int pcap,
-                     DevScalars* err, hipStream_t s)
+                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, double* pdb,
+                     double* prhs, int pcap, DevScalars* err, hipStream_t s)
 {
-    k_assemble_od<<<(pcap+255)/256, 256, 0, s>>>(pairs, pos, C, sc, od64, od32, pcap);
-    k_assemble_diag_rhs<<<(n+127)/128, 128, 0, s>>>(pairs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
-                                                    minv64, minv32, rhs, err);
+    k_assemble_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, C, sc, od64, od32, pdb, prhs, pcap);
+    k_gather_diag_rhs<<<(n+127)/128, 128, 0, s>>>(pdb, prhs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
+                                                  minv64, minv32, rhs, err);
 }
 
 // Matrix handed over through the StokesSolver staging protocol: FP32 copy and
@@ -727,20 +761,14 @@ __global__ void k_prepare_od(const double* od64, float* od32, const int* npairs,
     size_t m = (size_t)min(*npairs, pcap)*18;
     if (k < m) od32[k] = (float)od64[k];
 }
-__global__ void k_prepare_diag(int n, const double* diag64, float* diag32, double* minv64, float* minv32,
+__global__ __launch_bounds__(128) void k_prepare_diag(int n, const double* diag64, float* diag32, double* minv64, float* minv32,
                                DevScalars* err)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
-    for (int m = 0; m < 15; m++) diag32[15*(size_t)i+m] = (float)diag64[15*(size_t)i+m];
-    double M[6][6], Inv[6][6];
-    dense_from_dblock(diag64+15*(size_t)i, M);
-    if (!spd_inverse6(M, Inv)) raise_error(err, E_NOT_SPD);
-    for (int r = 0; r < 6; r++)
-        for (int c = 0; c < 6; c++) {
-            minv64[36*(size_t)i+6*r+c] = Inv[r][c];
-            minv32[36*(size_t)i+6*r+c] = (float)Inv[r][c];
-        }
+    double D[15];
+    for (int m = 0; m < 15; m++) { D[m] = diag64[15*(size_t)i+m]; diag32[15*(size_t)i+m] = (float)D[m]; }
+    if (!dblock_inverse(D, minv64+36*(size_t)i, minv32+36*(size_t)i)) raise_error(err, E_NOT_SPD);
 }
 void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float* diag32, const double* od64,
                            float* od32, double* minv64, float* minv32, int pcap, DevScalars* err, hipStream_t s)

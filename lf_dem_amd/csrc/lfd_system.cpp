@@ -261,6 +261,8 @@ struct lfd_system {
     PairRec* pairs[2] = {nullptr, nullptr};
     int cur = 0;
     int *keep = nullptr, *keep_scan = nullptr, *tcnt = nullptr, *merge_cnt = nullptr, *scan_tmp = nullptr;
+    int* probe = nullptr;
+    double *pdb = nullptr, *prhs = nullptr;
     DevScalars* sc = nullptr;
     Solver S;
     ~lfd_system();
@@ -271,7 +273,7 @@ lfd_system::~lfd_system()
     if (s) (void)hipStreamSynchronize(s);
     void* ptrs[] = {pos, vel6, na6, pred6, rhs, box_of, snap_start, snap_mem, home_start, home_mem, bin_count,
                     bin_cursor, found, found_cnt, pairs[0], pairs[1], keep, keep_scan, tcnt, merge_cnt,
-                    scan_tmp, sc, g.nb, g.nnb, g.nstatic, g.ctype};
+                    scan_tmp, sc, g.nb, g.nnb, g.nstatic, g.ctype, probe, pdb, prhs};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     S.release();
     if (s) (void)hipStreamDestroy(s);
@@ -345,7 +347,7 @@ static int sys_bin_snapshot(lfd_system* y)
 {
     // BoxSet::update (BoxSet.cpp:498-506): moving neighbours, then the
     // neighbourhood snapshot (cell membership at this instant)
-    launch_moving_neighbors(y->g, y->sc, y->C.L, y->sc, y->s);
+    launch_moving_neighbors(y->g, y->sc, y->C.L, y->probe, y->sc, y->s);
     launch_bin(y->box_of, y->n, y->g.nbox, y->bin_count, y->snap_start, y->snap_mem, y->bin_cursor, y->scan_tmp, y->s);
     return LFD_OK;
 }
@@ -361,8 +363,8 @@ static int sys_check_new(lfd_system* y)
     launch_merge_pairs(y->n, oldp, y->S.row_ptr, y->found, y->found_cnt, y->K, y->merge_cnt, y->keep_scan, newp,
                        y->pos, y->C, y->sc, y->pcap, y->scan_tmp, y->s);
     y->cur ^= 1;
-    launch_pair_index(newp, y->n, y->S.row_ptr, y->S.p0, y->S.p1, y->S.tptr, y->S.tidx, y->tcnt, y->sc, y->pcap,
-                      y->scan_tmp, y->s);
+    launch_pair_index(newp, y->n, y->S.row_ptr, y->S.p0, y->S.p1, y->S.tptr, y->S.tidx, y->tcnt, y->keep, y->sc,
+                      y->pcap, y->scan_tmp, y->s);
     return LFD_OK;
 }
 
@@ -381,7 +383,8 @@ static int sys_update(lfd_system* y, int in_predictor, double dt_override)
 static int sys_assemble(lfd_system* y)
 {
     launch_assemble(y->pairs[y->cur], y->pos, y->n, y->C, y->sc, y->S.row_ptr, y->S.tptr, y->S.tidx, y->S.diag64,
-                    y->S.diag32, y->S.od64, y->S.od32, y->S.minv64, y->S.minv32, y->rhs, y->pcap, y->sc, y->s);
+                    y->S.diag32, y->S.od64, y->S.od32, y->S.minv64, y->S.minv32, y->rhs, y->pdb, y->prhs, y->pcap,
+                    y->sc, y->s);
     return LFD_OK;
 }
 
@@ -465,7 +468,8 @@ extern "C" int lfd_system_create(int np, const double* pos, const double* radius
     A(dalloc(&y->keep, pcap)); A(dalloc(&y->keep_scan, (size_t)std::max(pcap, n)+1)); A(dalloc(&y->tcnt, n));
     A(dalloc(&y->merge_cnt, n)); A(dalloc(&y->scan_tmp, 4096)); A(dalloc(&y->sc, 1));
     A(dalloc(&y->g.nb, 64*(size_t)nbox)); A(dalloc(&y->g.nnb, nbox)); A(dalloc(&y->g.nstatic, nbox));
-    A(dalloc(&y->g.ctype, nbox));
+    A(dalloc(&y->g.ctype, nbox)); A(dalloc(&y->probe, 32*(size_t)nbox));
+    A(dalloc(&y->pdb, 30*(size_t)pcap)); A(dalloc(&y->prhs, 42*(size_t)pcap));
     A(y->S.alloc(n, pcap));
 #undef A
     if (e != hipSuccess) {
