@@ -77,7 +77,8 @@ struct SolveOpts {
 // ---------------- launchers (lfd_kernels.hip) ----------------
 void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
 void launch_fill_double(double* p, double v, size_t n, hipStream_t s);
-void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s);
+void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s,
+                        int* out2 = nullptr);
 
 // cell list
 struct CellGrid {
@@ -97,8 +98,8 @@ void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, 
 void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_start, const int* home_mem,
                        const int* snap_start, const int* snap_mem, const DevScalars* sc, Consts C,
                        int* found, int* found_cnt, int K, DevScalars* err, hipStream_t s);
-void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
-                        const int* found_cnt, int K, int* cnt, int* new_row_ptr, PairRec* new_pairs,
+void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_p1, const int* old_row_ptr, const int* found,
+                        const int* found_cnt, int K, int* cnt, int* new_row_ptr, int* src, PairRec* new_pairs,
                         const double4* pos, Consts C, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
 void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, Consts C, DevScalars* sc,
                          int* keep, int pcap, int in_predictor, double dt_override, hipStream_t s);

@@ -126,13 +126,45 @@ __global__ void k_scan_down(const int* in, int* out, const int* count_dev, int c
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) out[count] = sums[gridDim.x];
 }
-void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s)
+// Whole scan in one workgroup for arrays up to SCAN1_MAX entries: tiles of
+// SCAN1_T*SCAN_I entries with a running carry.  out2 (optional) receives a
+// second copy of out[0..count) (a cursor array); it may alias in.
+static constexpr int SCAN1_T = 1024, SCAN1_MAX = 1 << 17;
+__global__ __launch_bounds__(SCAN1_T) void k_scan_single(const int* in, int* out, int* out2, const int* count_dev,
+                                                         int cap)
 {
+    __shared__ int lds[SCAN1_T/WAVE+1];
+    const int count = count_dev ? min(*count_dev, cap) : cap;
+    int carry = 0;
+    for (int base = 0; base < count; base += SCAN1_T*SCAN_I) {
+        const int b0 = base+threadIdx.x*SCAN_I;
+        int v[SCAN_I], acc = 0;
+        for (int k = 0; k < SCAN_I; k++) { int i = b0+k; v[k] = i < count ? in[i] : 0; acc += v[k]; }
+        int total;
+        int e = block_exclusive_scan(acc, lds, total)+carry;
+        for (int k = 0; k < SCAN_I; k++) {
+            int i = b0+k;
+            if (i < count) { out[i] = e; if (out2) out2[i] = e; }
+            e += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) out[count] = carry;
+}
+__global__ void k_copy_int(int* d, const int* s, int n);
+void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s,
+                        int* out2)
+{
+    if (cap <= SCAN1_MAX) {
+        k_scan_single<<<1, SCAN1_T, 0, s>>>(in, out, out2, count_dev, cap);
+        return;
+    }
     int nb = (cap+SCAN_TILE-1)/SCAN_TILE;
     if (nb < 1) nb = 1;
     k_scan_reduce<<<nb, SCAN_T, 0, s>>>(in, count_dev, cap, tmp);
     k_scan_top<<<1, SCAN_T, 0, s>>>(tmp, nb);
     k_scan_down<<<nb, SCAN_T, 0, s>>>(in, out, count_dev, cap, tmp);
+    if (out2) k_copy_int<<<(cap+255)/256, 256, 0, s>>>(out2, out, cap);
 }
 
 // ------------------------------------------------------------------ cells
@@ -183,8 +215,7 @@ void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int*
 {
     launch_fill_int(count, 0, nbox, s);
     k_bin_count<<<(n+255)/256, 256, 0, s>>>(box_of, n, count);
-    exclusive_scan_int(count, start, nullptr, nbox, scan_tmp, s);
-    k_copy_int<<<(nbox+255)/256, 256, 0, s>>>(cursor, start, nbox);
+    exclusive_scan_int(count, start, nullptr, nbox, scan_tmp, s, cursor);
     k_bin_scatter<<<(n+255)/256, 256, 0, s>>>(box_of, n, cursor, mem);
     k_bin_sort<<<(nbox+255)/256, 256, 0, s>>>(start, nbox, mem);
 }
@@ -215,22 +246,30 @@ __global__ void k_probe_labels(CellGrid g, const DevScalars* sc, Consts C, int* 
     }
     probe[t] = l;
 }
-__global__ void k_moving_merge(CellGrid g, const int* probe)
+// One wave per cell: the static neighbours sit one per lane, the 32 probe
+// labels are appended in probe order when no lane already holds them.
+__global__ __launch_bounds__(256) void k_moving_merge(CellGrid g, const int* probe)
 {
-    int b = blockIdx.x*blockDim.x+threadIdx.x;
+    const int b = blockIdx.x*(256/WAVE)+(int)(threadIdx.x/WAVE);
+    const int lane = threadIdx.x & (WAVE-1);
     if (b >= g.nbox) return;
     int cnt = g.nstatic[b];
     int* nb = g.nb+64*b;
     if (g.ctype[b] != 0 && g.boxed) {
+        int mine = lane < cnt ? nb[lane] : -1;
+        const int pl = lane < 32 ? probe[32*b+lane] : -1;
         for (int k = 0; k < 32; k++) {
-            int l = probe[32*b+k];
+            const int l = __shfl(pl, k);
             if (l < 0) continue;
-            bool dup = false;
-            for (int m = 0; m < cnt; m++) if (nb[m] == l) { dup = true; break; }
-            if (!dup && cnt < 64) nb[cnt++] = l;
+            const bool dup = __any(mine == l);
+            if (!dup && cnt < 64) {
+                if (lane == cnt) mine = l;
+                cnt++;
+            }
         }
+        if (lane < cnt) nb[lane] = mine;
     }
-    g.nnb[b] = cnt;
+    if (lane == 0) g.nnb[b] = cnt;
 }
 void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, int* probe, DevScalars* err,
                              hipStream_t s)
@@ -238,7 +277,7 @@ void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, 
     Consts C{};
     C.L[0] = L[0]; C.L[1] = L[1]; C.L[2] = L[2];
     k_probe_labels<<<(32*g.nbox+255)/256, 256, 0, s>>>(g, sc, C, probe, err);
-    k_moving_merge<<<(g.nbox+255)/256, 256, 0, s>>>(g, probe);
+    k_moving_merge<<<(g.nbox+3)/4, 256, 0, s>>>(g, probe);
 }
 
 // ------------------------------------------------------------------ neighbour search
@@ -333,7 +372,7 @@ void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_st
 
 // ------------------------------------------------------------------ merge new pairs
 // Row i: union of the existing pairs (i, j) and the pairs found this pass.
-__global__ void k_merge_count(int n, const PairRec* old_pairs, const int* old_row_ptr, int* found,
+__global__ void k_merge_count(int n, const int* __restrict__ old_p1, const int* old_row_ptr, int* found,
                               const int* found_cnt, int K, int* cnt)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
@@ -350,7 +389,7 @@ __global__ void k_merge_count(int n, const PairRec* old_pairs, const int* old_ro
         if (a >= nf) { o++; }
         else if (o >= oe) { a++; }
         else {
-            int jo = old_pairs[o].p1, jf = f[a];
+            int jo = old_p1[o], jf = f[a];
             if (jo == jf) { o++; a++; }
             else if (jo < jf) o++;
             else a++;
@@ -359,9 +398,11 @@ __global__ void k_merge_count(int n, const PairRec* old_pairs, const int* old_ro
     }
     cnt[i] = m;
 }
-__global__ __launch_bounds__(256) void k_merge_write(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
-                              const int* found_cnt, int K, const int* new_row_ptr, PairRec* new_pairs,
-                              const double4* pos, Consts C, const DevScalars* sc, int pcap, DevScalars* err)
+// Source of every slot of the merged rows: an old pair index (>= 0) or
+// ~(slot in the found list) for a new pair.
+__global__ void k_merge_index(int n, const int* __restrict__ old_p1, const int* __restrict__ old_row_ptr,
+                              const int* __restrict__ found, const int* __restrict__ found_cnt, int K,
+                              const int* __restrict__ new_row_ptr, int* __restrict__ src, int pcap, DevScalars* err)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
@@ -369,31 +410,37 @@ __global__ __launch_bounds__(256) void k_merge_write(int n, const PairRec* old_p
     const int* f = found+(size_t)i*K;
     int o = old_row_ptr[i], oe = old_row_ptr[i+1], a = 0;
     int w = new_row_ptr[i];
-    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
-    double4 xi = pos[i];
     while (o < oe || a < nf) {
-        int take_old;
-        if (a >= nf) take_old = 1;
-        else if (o >= oe) take_old = 0;
+        int v;
+        if (a >= nf) v = o++;
+        else if (o >= oe) { v = ~(i*K+a); a++; }
         else {
-            int jo = old_pairs[o].p1, jf = f[a];
-            if (jo == jf) { take_old = 1; a++; }
-            else take_old = jo < jf;
+            int jo = old_p1[o], jf = f[a];
+            if (jo == jf) { v = o++; a++; }
+            else if (jo < jf) v = o++;
+            else { v = ~(i*K+a); a++; }
         }
         if (w >= pcap) { raise_error(err, E_PAIR_OVERFLOW); return; }
-        if (take_old) {
-            new_pairs[w] = old_pairs[o];
-            o++;
-        } else {
-            int j = f[a++];
-            double4 xj = pos[j];
-            double pi[3] = {xi.x, xi.y, xi.z}, pj[3] = {xj.x, xj.y, xj.z};
-            PairRec q;
-            pair_create(q, i, j, pi, pj, xi.w, xj.w, C, sd);
-            new_pairs[w] = q;
-        }
-        w++;
+        src[w++] = v;
     }
+}
+// One thread per merged slot: copy the old record or create the new pair.
+__global__ __launch_bounds__(256) void k_merge_copy(const PairRec* __restrict__ old_pairs, const int* __restrict__ src,
+    const int* __restrict__ found, int K, const int* __restrict__ total, PairRec* __restrict__ new_pairs,
+    const double4* __restrict__ pos, Consts C, const DevScalars* sc, int pcap)
+{
+    int w = blockIdx.x*blockDim.x+threadIdx.x;
+    if (w >= min(*total, pcap)) return;
+    int v = src[w];
+    if (v >= 0) { new_pairs[w] = old_pairs[v]; return; }
+    v = ~v;
+    int i = v/K, j = found[v];
+    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    double4 xi = pos[i], xj = pos[j];
+    double pi[3] = {xi.x, xi.y, xi.z}, pj[3] = {xj.x, xj.y, xj.z};
+    PairRec q;
+    pair_create(q, i, j, pi, pj, xi.w, xj.w, C, sd);
+    new_pairs[w] = q;
 }
 __global__ void k_set_npairs_from(DevScalars* sc, const int* total, int pcap)
 {
@@ -401,14 +448,14 @@ __global__ void k_set_npairs_from(DevScalars* sc, const int* total, int pcap)
     if (t > pcap) { atomicOr(&sc->error, E_PAIR_OVERFLOW); t = pcap; }
     sc->npairs = t;
 }
-void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_row_ptr, const int* found,
-                        const int* found_cnt, int K, int* cnt, int* new_row_ptr, PairRec* new_pairs,
+void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_p1, const int* old_row_ptr, const int* found,
+                        const int* found_cnt, int K, int* cnt, int* new_row_ptr, int* src, PairRec* new_pairs,
                         const double4* pos, Consts C, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
 {
-    k_merge_count<<<(n+255)/256, 256, 0, s>>>(n, old_pairs, old_row_ptr, const_cast<int*>(found), found_cnt, K, cnt);
+    k_merge_count<<<(n+255)/256, 256, 0, s>>>(n, old_p1, old_row_ptr, const_cast<int*>(found), found_cnt, K, cnt);
     exclusive_scan_int(cnt, new_row_ptr, nullptr, n, scan_tmp, s);
-    k_merge_write<<<(n+255)/256, 256, 0, s>>>(n, old_pairs, old_row_ptr, found, found_cnt, K, new_row_ptr,
-                                              new_pairs, pos, C, sc, pcap, sc);
+    k_merge_index<<<(n+255)/256, 256, 0, s>>>(n, old_p1, old_row_ptr, found, found_cnt, K, new_row_ptr, src, pcap, sc);
+    k_merge_copy<<<(pcap+255)/256, 256, 0, s>>>(old_pairs, src, found, K, new_row_ptr+n, new_pairs, pos, C, sc, pcap);
     k_set_npairs_from<<<1, 1, 0, s>>>(sc, new_row_ptr+n, pcap);
 }
 
@@ -483,18 +530,18 @@ __global__ void k_tfill(const int* p1, int* cursor, int* tmp, const DevScalars* 
     if (k >= sc->npairs) return;
     tmp[atomicAdd(&cursor[p1[k]], 1)] = k;
 }
-// lower incidence lists in ascending pair index (= ascending p0): each entry
-// is placed by its rank within the list
-__global__ void k_tsort(const int* tptr, int n, const int* tmp, int* tidx)
+// lower incidence lists in ascending pair index (= ascending p0): one thread
+// per entry places it by its rank within its list
+__global__ void k_tsort(const int* __restrict__ tptr, const int* __restrict__ p1, const int* __restrict__ tmp,
+                        int* __restrict__ tidx, const DevScalars* sc)
 {
-    int t = blockIdx.x*blockDim.x+threadIdx.x;
-    if (t >= n) return;
-    int s0 = tptr[t], s1 = tptr[t+1];
-    for (int a = s0; a < s1; a++) {
-        int x = tmp[a], rank = 0;
-        for (int b = s0; b < s1; b++) rank += tmp[b] < x;
-        tidx[s0+rank] = x;
-    }
+    const int a = blockIdx.x*blockDim.x+threadIdx.x;
+    if (a >= sc->npairs) return;
+    const int x = tmp[a], j = p1[x];
+    const int s0 = tptr[j], s1 = tptr[j+1];
+    int rank = 0;
+    for (int q = s0; q < s1; q++) rank += tmp[q] < x;
+    tidx[s0+rank] = x;
 }
 void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
                        int* tcnt, int* tmp, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
@@ -502,10 +549,9 @@ void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* 
     launch_fill_int(tcnt, 0, n, s);
     k_pair_arrays<<<(pcap+255)/256, 256, 0, s>>>(pairs, p0, p1, tcnt, sc);
     k_row_ptr<<<(n+1+255)/256, 256, 0, s>>>(p0, n, row_ptr, sc);
-    exclusive_scan_int(tcnt, tptr, nullptr, n, scan_tmp, s);
-    k_copy_int<<<(n+255)/256, 256, 0, s>>>(tcnt, tptr, n);
+    exclusive_scan_int(tcnt, tptr, nullptr, n, scan_tmp, s, tcnt);
     k_tfill<<<(pcap+255)/256, 256, 0, s>>>(p1, tcnt, tmp, sc);
-    k_tsort<<<(n+255)/256, 256, 0, s>>>(tptr, n, tmp, tidx);
+    k_tsort<<<(pcap+255)/256, 256, 0, s>>>(tptr, p1, tmp, tidx, sc);
 }
 
 // ------------------------------------------------------------------ assembly
@@ -698,38 +744,70 @@ __global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restric
 // block-Jacobi inverse, and the RHS summed per force component in the same
 // order, then across components in LF_DEM's std::map order
 // (StokesSolver.cpp:513-549, System.cpp:1873-1885).
-__global__ __launch_bounds__(128) void k_gather_diag_rhs(const double* __restrict__ pdb, const double* __restrict__ prhs,
-                                  const double4* __restrict__ pos, int n, Consts C, const int* __restrict__ row_ptr,
-                                  const int* __restrict__ tptr, const int* __restrict__ tidx,
-                                  double* __restrict__ diag64, float* __restrict__ diag32,
-                                  double* __restrict__ minv64, float* __restrict__ minv32,
-                                  double* __restrict__ rhs, DevScalars* err)
+// Per-particle sums of the pair terms.  Eight lanes per particle, lane c
+// summing components c, c+8, ... of the 15 diagonal-block and 21 RHS values,
+// each in ascending partner order (drag seed, then pairs (j, i) with j < i,
+// then pairs (i, j)); then one lane per particle inverts the block and
+// combines the RHS from the workgroup's LDS copy.
+static constexpr int GD_LANES = 8, GD_BLOCK = 256, GD_PER = GD_BLOCK/GD_LANES, GD_VALS = 15+RHS_TERMS;
+
+__global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __restrict__ pdb,
+    const double* __restrict__ prhs, const double4* __restrict__ pos, int n, Consts C,
+    const int* __restrict__ row_ptr, const int* __restrict__ tptr, const int* __restrict__ tidx,
+    double* __restrict__ diag64, float* __restrict__ diag32, double* __restrict__ minv64,
+    float* __restrict__ minv32, double* __restrict__ rhs, DevScalars* err)
 {
-    int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i >= n) return;
-    const double a = pos[i].w;
-    double D[15];
-    for (int m = 0; m < 15; m++) D[m] = 0;
-    const double FU = C.sd_coeff*a;
-    const double TW = C.sd_coeff*(a*a*a)*4.0/3;
-    D[0] = FU; D[5] = FU; D[8] = FU; D[9] = TW; D[12] = TW; D[14] = TW;
-    double R[RHS_TERMS];
-    for (int m = 0; m < RHS_TERMS; m++) R[m] = 0;
-    for (int t = tptr[i]; t < tptr[i+1]; t++) {           // pairs (j, i), j < i
-        const size_t k = tidx[t];
+    __shared__ double sv[GD_PER][GD_VALS+1];
+    const int lp = threadIdx.x/GD_LANES, c = threadIdx.x%GD_LANES;
+    const int i = blockIdx.x*GD_PER+lp;
+    if (i < n) {
+        const double a = pos[i].w;
+        const double FU = C.sd_coeff*a;
+        const double TW = C.sd_coeff*(a*a*a)*4.0/3;
+        double acc[5];
         #pragma unroll
-        for (int m = 0; m < 15; m++) D[m] += pdb[30*k+15+m];
+        for (int u = 0; u < 5; u++) {
+            const int q = c+GD_LANES*u;
+            double v = 0;
+            if (q == 0 || q == 5 || q == 8) v = FU;
+            if (q == 9 || q == 12 || q == 14) v = TW;
+            acc[u] = v;
+        }
+        const int t0 = tptr[i], t1 = tptr[i+1];
+        for (int t = t0; t < t1; t++) {                      // pairs (j, i), j < i
+            const size_t k = tidx[t];
+            #pragma unroll
+            for (int u = 0; u < 5; u++) {
+                const int q = c+GD_LANES*u;
+                if (q < 15) acc[u] += pdb[30*k+15+q];
+                else if (q < GD_VALS) acc[u] += prhs[42*k+RHS_TERMS+(q-15)];
+            }
+        }
+        const int k0 = row_ptr[i], k1 = row_ptr[i+1];
+        for (int k = k0; k < k1; k++) {                      // pairs (i, j), j > i
+            #pragma unroll
+            for (int u = 0; u < 5; u++) {
+                const int q = c+GD_LANES*u;
+                if (q < 15) acc[u] += pdb[30*(size_t)k+q];
+                else if (q < GD_VALS) acc[u] += prhs[42*(size_t)k+(q-15)];
+            }
+        }
         #pragma unroll
-        for (int m = 0; m < RHS_TERMS; m++) R[m] += prhs[42*k+RHS_TERMS+m];
+        for (int u = 0; u < 5; u++) {
+            const int q = c+GD_LANES*u;
+            if (q < GD_VALS) sv[lp][q] = acc[u];
+            if (q < 15) { diag64[15*(size_t)i+q] = acc[u]; diag32[15*(size_t)i+q] = (float)acc[u]; }
+        }
     }
-    for (int k = row_ptr[i]; k < row_ptr[i+1]; k++) {     // pairs (i, j), j > i
-        #pragma unroll
-        for (int m = 0; m < 15; m++) D[m] += pdb[30*(size_t)k+m];
-        #pragma unroll
-        for (int m = 0; m < RHS_TERMS; m++) R[m] += prhs[42*(size_t)k+m];
-    }
-    for (int m = 0; m < 15; m++) { diag64[15*(size_t)i+m] = D[m]; diag32[15*(size_t)i+m] = (float)D[m]; }
-    if (!dblock_inverse(D, minv64+36*(size_t)i, minv32+36*(size_t)i)) raise_error(err, E_NOT_SPD);
+    __syncthreads();
+    if (threadIdx.x >= GD_PER) return;
+    const int ip = blockIdx.x*GD_PER+threadIdx.x;
+    if (ip >= n) return;
+    const double* D = sv[threadIdx.x];
+    const double* R = sv[threadIdx.x]+15;
+    double Dl[15];
+    for (int m = 0; m < 15; m++) Dl[m] = D[m];
+    if (!dblock_inverse(Dl, minv64+36*(size_t)ip, minv32+36*(size_t)ip)) raise_error(err, E_NOT_SPD);
     for (int m = 0; m < 3; m++) {
         double F = 0, T = 0;
         F += R[m];                           // contact
@@ -739,8 +817,8 @@ __global__ __launch_bounds__(128) void k_gather_diag_rhs(const double* __restric
         F += R[12+m];                        // hydro
         if (C.tangential) T += R[15+m];
         if (C.repulsive) F += R[18+m];       // repulsion
-        rhs[6*(size_t)i+m] = F;
-        rhs[6*(size_t)i+3+m] = T;
+        rhs[6*(size_t)ip+m] = F;
+        rhs[6*(size_t)ip+3+m] = T;
     }
 }
 void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
@@ -749,7 +827,7 @@ void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, 
                      double* prhs, int pcap, DevScalars* err, hipStream_t s)
 {
     k_assemble_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, C, sc, od64, od32, pdb, prhs, pcap);
-    k_gather_diag_rhs<<<(n+127)/128, 128, 0, s>>>(pdb, prhs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
+    k_gather_diag_rhs<<<(n+GD_PER-1)/GD_PER, GD_BLOCK, 0, s>>>(pdb, prhs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
                                                   minv64, minv32, rhs, err);
 }
 

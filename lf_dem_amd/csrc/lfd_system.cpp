@@ -360,8 +360,8 @@ static int sys_check_new(lfd_system* y)
                       y->found, y->found_cnt, y->K, y->sc, y->s);
     PairRec* oldp = y->pairs[y->cur];
     PairRec* newp = y->pairs[y->cur^1];
-    launch_merge_pairs(y->n, oldp, y->S.row_ptr, y->found, y->found_cnt, y->K, y->merge_cnt, y->keep_scan, newp,
-                       y->pos, y->C, y->sc, y->pcap, y->scan_tmp, y->s);
+    launch_merge_pairs(y->n, oldp, y->S.p1, y->S.row_ptr, y->found, y->found_cnt, y->K, y->merge_cnt, y->keep_scan,
+                       y->keep, newp, y->pos, y->C, y->sc, y->pcap, y->scan_tmp, y->s);
     y->cur ^= 1;
     launch_pair_index(newp, y->n, y->S.row_ptr, y->S.p0, y->S.p1, y->S.tptr, y->S.tidx, y->tcnt, y->keep, y->sc,
                       y->pcap, y->scan_tmp, y->s);
