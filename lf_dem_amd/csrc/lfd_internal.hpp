@@ -16,6 +16,7 @@ enum : int {
     E_PAIR_OVERFLOW = 4,  // pair capacity exceeded
     E_NOT_SPD = 8,        // a 6x6 diagonal block is not positive definite
     E_NOT_CONVERGED = 16, // Krylov iteration limit reached before the tolerance
+    E_SYNC_TIMEOUT = 32,  // a solver workgroup gave up waiting for the others (bounded spin)
 };
 
 // Per-instance scalars kept on the device so a step needs no host round trip.

@@ -1336,10 +1336,12 @@ __device__ inline void bcast8_6(float v, int sub, float* o)
 __device__ inline unsigned ld_acquire(unsigned* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Arrival and wait of one workgroup, run by a single thread after a
-// __syncthreads: release on arrival, acquire on the generation load.
-__device__ inline void tree_barrier_core(unsigned* bar, int G, unsigned& gen)
+// __syncthreads: release on arrival, acquire on the generation load.  The
+// wait gives up after SPIN_LIMIT polls and reports false.
+static constexpr long SPIN_LIMIT = 1L << 22;
+__device__ inline bool tree_barrier_core(unsigned* bar, int G, int wg, unsigned& gen)
 {
-    const int leaf = blockIdx.x >> 3, nleaf = (G+7) >> 3;
+    const int leaf = wg >> 3, nleaf = (G+7) >> 3;
     const int lsize = min(8, G-(leaf << 3));
     unsigned* lc = bar+64+32*leaf;
     unsigned old = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -1351,14 +1353,53 @@ __device__ inline void tree_barrier_core(unsigned* bar, int G, unsigned& gen)
             __hip_atomic_fetch_add(bar+32, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    while (ld_acquire(bar+32) == gen) __builtin_amdgcn_s_sleep(1);
+    for (long spin = 0; ld_acquire(bar+32) == gen; spin++) {
+        if (spin >= SPIN_LIMIT) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
     gen++;
+    return true;
 }
 
-__device__ inline void tree_barrier(unsigned* bar, int G, unsigned& gen)
+// Relaxed device-scope (sc1) accesses, performed at the device-coherent level.
+template <class T> __device__ inline void st_dev(T* p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ inline T ld_dev(const T* p)
+{
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until every memory operation this wave has issued is complete.
+__device__ inline void drain_vmem()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// A partial sum with the low 8 mantissa bits replaced by a round tag
+// (relative perturbation below 3e-14).  Aligned 64-bit stores and loads are
+// single-copy atomic, so a record read with the expected tag is that round's.
+__device__ inline double tag_partial(double v, unsigned tag)
+{
+    return __longlong_as_double((__double_as_longlong(v) & ~0xFFll) | (long long)(tag & 0xFF));
+}
+__device__ inline double untag_partial(unsigned long long v)
+{
+    return __longlong_as_double((long long)(v & ~0xFFull));
+}
+// the 6 components of row j of an exchanged vector (8-byte aligned)
+__device__ inline void load6(const float* p, float* o)
+{
+    const float2* q = reinterpret_cast<const float2*>(p);
+    for (int h = 0; h < 3; h++) { float2 v = q[h]; o[2*h] = v.x; o[2*h+1] = v.y; }
+}
+// Full grid barrier (release / acquire) of the whole workgroup; skipped once
+// the workgroup has failed, and *fail raised when the wait gives up.
+__device__ inline void tree_barrier(unsigned* bar, int G, int wg, unsigned& gen, int* fail)
 {
     __syncthreads();
-    if (threadIdx.x == 0) tree_barrier_core(bar, G, gen);
+    if (threadIdx.x == 0 && !*fail && !tree_barrier_core(bar, G, wg, gen)) *fail = 1;
     __syncthreads();
 }
 
@@ -1371,24 +1412,28 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     __shared__ float ent_b[LS_ENTRY_CAP*18];
     __shared__ LdsRow rows[LS_ROWS];
     __shared__ double red[3*LS_WAVES+4];
-    const int G = gridDim.x, n = A.n;
+    __shared__ int s_fail;
+    const int wg = blockIdx.x, G = gridDim.x, n = A.n;
+    unsigned seq = 0;
+    double g0 = 0, g1 = 0, g2 = 0;        // wave 0: this lane's share of the polled partial sums
     const int lr = threadIdx.x/LS_LPR, c = threadIdx.x%LS_LPR;
     const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
-    const int i = blockIdx.x*LS_ROWS+lr;
+    const int i = wg*LS_ROWS+lr;
     const bool live = i < n;
     const bool own = live && c < 6;
     const size_t gi = 6*(size_t)i+c;
     unsigned gen = ld_acquire(bar+32);
+    if (threadIdx.x == 0) s_fail = 0;
     // ---- stage this workgroup's rows: degrees in parallel, serial prefix in LDS
     if (threadIdx.x < LS_ROWS) {
-        int ii = blockIdx.x*LS_ROWS+threadIdx.x;
+        int ii = wg*LS_ROWS+threadIdx.x;
         rows[threadIdx.x].ne = ii < n ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int used = 0;
         for (int t = 0; t < LS_ROWS; t++) {
-            int ii = blockIdx.x*LS_ROWS+t, deg = rows[t].ne;
+            int ii = wg*LS_ROWS+t, deg = rows[t].ne;
             if (ii < n && used+deg <= LS_ENTRY_CAP) { rows[t].e0 = used; used += deg; }
             else { rows[t].e0 = 0; rows[t].ne = ii < n ? -1 : 0; }
         }
@@ -1430,11 +1475,11 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                     const float* bl = ent_b+18*(e0+e);
                     float xj[6];
                     if (jj >= 0) {
-                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        load6(vbuf+6*(size_t)jj, xj);
                         omul<float, float>(bl, xj, y);
                     } else {
                         jj = ~jj;
-                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        load6(vbuf+6*(size_t)jj, xj);
                         omul_t<float, float>(bl, xj, y);
                     }
                 }
@@ -1445,11 +1490,11 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                     float xj[6];
                     if (e < nl) {
                         int k = A.tidx[l0+e], jj = A.p0[k];
-                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        load6(vbuf+6*(size_t)jj, xj);
                         omul<float, float>(A.od32+18*(size_t)k, xj, y);
                     } else {
                         int k = u0+(e-nl), jj = A.p1[k];
-                        for (int m = 0; m < 6; m++) xj[m] = vbuf[6*(size_t)jj+m];
+                        load6(vbuf+6*(size_t)jj, xj);
                         omul_t<float, float>(A.od32+18*(size_t)k, xj, y);
                     }
                 }
@@ -1472,36 +1517,48 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
 
     double* part = w.partials;
     int buf = 0;
-    // block sums of three values (DPP tree, fixed order) written to the
-    // current partial buffer; then one grid barrier
-    auto publish3 = [&](double a, double bs, double cs) -> double* {
+    // Grid sums of three values.  Block sums (DPP tree, fixed order) are
+    // published as tagged partial records in the current buffer after every
+    // wave's vector stores have completed and a release fence; wave 0 then
+    // polls all records with device-scope loads until each carries this
+    // round's tag, keeps its lanes' shares in g0..g2, and issues the acquire
+    // fence that makes the other workgroups' vector stores visible.
+    auto publish3 = [&](double a, double bs, double cs) {
         a = wave_sum_d(a); bs = wave_sum_d(bs); cs = wave_sum_d(cs);
         if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bs; red[3*wid+2] = cs; }
+        drain_vmem();
         __syncthreads();
         double* pb = part+(size_t)buf*3*G;
+        seq++;
         if (wid == 0) {
             double t0 = lane < LS_WAVES ? red[3*lane] : 0.0;
             double t1 = lane < LS_WAVES ? red[3*lane+1] : 0.0;
             double t2 = lane < LS_WAVES ? red[3*lane+2] : 0.0;
-            t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
-            if (lane == 0) {
-                pb[3*blockIdx.x] = t0; pb[3*blockIdx.x+1] = t1; pb[3*blockIdx.x+2] = t2;
-                tree_barrier_core(bar, G, gen);
+            const unsigned tg = seq & 0xFF;
+            t0 = tag_partial(wave_sum_d(t0), tg);
+            t1 = tag_partial(wave_sum_d(t1), tg);
+            t2 = tag_partial(wave_sum_d(t2), tg);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
+            const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
+            bool ok = false;
+            for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
+                double a0 = 0, a1 = 0, a2 = 0;
+                bool done = true;
+                for (int q = lane; q < G; q += WAVE) {
+                    const unsigned long long v0 = ld_dev(pq+3*q), v1 = ld_dev(pq+3*q+1), v2 = ld_dev(pq+3*q+2);
+                    done &= (v0 & 0xFF) == tg && (v1 & 0xFF) == tg && (v2 & 0xFF) == tg;
+                    a0 += untag_partial(v0); a1 += untag_partial(v1); a2 += untag_partial(v2);
+                }
+                if (__all(done)) { g0 = a0; g1 = a1; g2 = a2; ok = true; break; }
+                __builtin_amdgcn_s_sleep(1);
             }
+            if (!ok && lane == 0) s_fail = 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
-        return pb;
     };
-    // grid sums of the published partials; wave 0 loads, caller syncs
-    auto gather3_start = [&](const double* pb, double& t0, double& t1, double& t2) {
-        t0 = t1 = t2 = 0;
-        if (wid == 0)
-            for (int q = lane; q < G; q += WAVE) {
-                t0 += __hip_atomic_load(pb+3*q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                t1 += __hip_atomic_load(pb+3*q+1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                t2 += __hip_atomic_load(pb+3*q+2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-    };
+    auto gather3_start = [&](double& t0, double& t1, double& t2) { t0 = g0; t1 = g1; t2 = g2; };
     auto gather3_finish = [&](double& t0, double& t1, double& t2) {
         if (wid == 0) {
             t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
@@ -1513,8 +1570,8 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         buf ^= 1;
     };
     auto gsum3 = [&](double& a, double& bs, double& cs) {
-        const double* pb = publish3(a, bs, cs);
-        gather3_start(pb, a, bs, cs);
+        publish3(a, bs, cs);
+        gather3_start(a, bs, cs);
         gather3_finish(a, bs, cs);
     };
 
@@ -1533,6 +1590,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         }
         double s_rr = r64*r64, s_bb = bi*bi, s_0 = 0;
         gsum3(s_rr, s_bb, s_0);
+        if (s_fail) break;
         if (outer == 0) bb = s_bb;
         if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
         rr64 = s_rr;
@@ -1556,12 +1614,12 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             for (it = 0; it < o.max_inner; it++) {
                 float mv = precond(wv);
                 if (own) mbuf[par][gi] = mv;
-                const double* pb = publish3((double)r*u, (double)wv*u, (double)r*r);
+                publish3((double)r*u, (double)wv*u, (double)r*r);
                 double g, dl, rho;
-                gather3_start(pb, g, dl, rho);     // overlaps with the neighbour gathers below
+                gather3_start(g, dl, rho);
                 float nv = spmv32(mbuf[par], mv);
                 gather3_finish(g, dl, rho);
-                if (rho <= stop2) break;
+                if (s_fail || rho <= stop2) break;
                 double beta, alpha;
                 if (it > 0) {
                     beta = g/gamma_old;
@@ -1600,7 +1658,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             gsum3(s_pq, z1, z2);
             double pq = s_pq;
             for (it = 0; it < o.max_inner; it++) {
-                if (!(pq > 0)) break;
+                if (s_fail || !(pq > 0)) break;
                 const float alpha = (float)(rz/pq);
                 d += alpha*p;
                 r -= alpha*q;
@@ -1621,9 +1679,10 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         }
         inner_total += it;
         if (own) x[gi] += (double)d*sqrt(rr64);
-        tree_barrier(bar, G, gen);
+        tree_barrier(bar, G, wg, gen, &s_fail);
+        if (s_fail) break;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (wg == 0 && threadIdx.x == 0) {
         sc->res_norm = sqrt(rr64);
         sc->rhs_norm = sqrt(bb);
         sc->iters_inner = inner_total;
@@ -1631,6 +1690,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         sc->total_inner += inner_total;
         if (!converged) atomicOr(&sc->error, E_NOT_CONVERGED);
     }
+    if (threadIdx.x == 0 && s_fail) atomicOr(&sc->error, E_SYNC_TIMEOUT);
 }
 
 int lds_solver_max_groups()
@@ -1646,6 +1706,7 @@ int lds_solver_max_groups()
     return cached;
 }
 
+// One workgroup of 1024 threads per 128 block rows, all co-resident.
 hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                      const SolveOpts& o, hipStream_t s)
 {
@@ -1655,6 +1716,7 @@ hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w
     SolverWork wv = w;
     SolveOpts ov = o;
     unsigned* bar = w.barrier;
+    launch_fill_double(w.partials, 0.0, 6*(size_t)G, s);     // no stale round tags
     void* args[] = {&Av, (void*)&b, (void*)&x, &wv, &sc, &ov, &bar};
     return hipLaunchCooperativeKernel((const void*)k_solve_lds, dim3(G), dim3(LS_BLOCK), args, 0, s);
 }

@@ -188,6 +188,7 @@ static int check_scalars(DevScalars* sc_dev, hipStream_t s, DevScalars* out = nu
     if (h.error & E_BOX_OOB) return fail(LFD_ERR_BOX, "BoxSet: trying to box position out of boundaries");
     if (h.error & (E_FOUND_OVERFLOW | E_PAIR_OVERFLOW)) return fail(LFD_ERR_CAPACITY, "pair capacity exceeded");
     if (h.error & E_NOT_SPD) return fail(LFD_ERR_NOT_SPD, "resistance diagonal block not positive definite");
+    if (h.error & E_SYNC_TIMEOUT) return fail(LFD_ERR_HIP, "solver workgroups lost synchronisation (spin limit)");
     if (h.error & E_NOT_CONVERGED) return fail(LFD_ERR_NOT_CONVERGED, "solver did not reach the tolerance");
     return LFD_OK;
 }
