@@ -559,6 +559,12 @@ typedef struct {
 	double geo_factor, rep_norm, rep_force[3];
 } orc_pair;
 
+/* stress components in LF_DEM's std::map order (Stress.cpp:16-73): the
+ * M:E term, GU of the four velocity components, the contact xF (rate
+ * control: one slot; stress control: rate independent, rate proportional)
+ * and the repulsive xF */
+enum { SC_ME = 0, SC_GU = 1, SC_XF_CONTACT = 5, SC_XF_CONTACT_PROP = 6, SC_XF_REP = 7, ORC_NSC = 8 };
+
 struct orc_system {
 	orc_params p;
 	int np;
@@ -589,6 +595,14 @@ struct orc_system {
 	double *dblocks, *odblocks;
 	int *od_cp, *od_rows;
 	int od_cap;
+	/* velocity components and stress (System.h na_velo_components,
+	 * stress_components, total_stress_groups) */
+	int control;                   /* 0 rate, 1 stress */
+	double target_stress, init_strain_limit, init_rate_limit;
+	double *comp_vel, *comp_ang;   /* ORC_NCOMP x 3 np */
+	int comp_present[ORC_NCOMP];
+	double *pstress;               /* ORC_NSC x 6 np, stress components per particle */
+	double groups[ORC_NGROUP][6], total_stress[6];
 	char err[256];
 };
 
@@ -1148,85 +1162,93 @@ static void compute_uinf(orc_system *s)
 static void add3(double *a, const double *b) { a[0] += b[0]; a[1] += b[1]; a[2] += b[2]; }
 static void sub3(double *a, const double *b) { a[0] -= b[0]; a[1] -= b[1]; a[2] -= b[2]; }
 
-int orc_build_rhs(orc_system *s, double *rhs)
+/* Force on p0 from the dashpot at the given (total or non-affine) velocities
+ * (ContactDashpot.cpp:342-385; getForceOnP0_nonaffine :387-424 drops the LE
+ * velocity jump). */
+static void dashpot_force(const orc_system *s, const orc_pair *q, const double *v0, const double *v1,
+                          const double *w0, const double *w1, int nonaffine, double f[3])
 {
-	/* force components in std::map order contact, dashpot, hydro, repulsion
-	 * (System.cpp:169-215, 1727-1828, 1873-1885) summed into the RHS
-	 * (StokesSolver.cpp:513-549) */
-	int np = s->np, i, k;
-	double *F = calloc(6*(size_t)np, sizeof(double));
-	double *T = F+3*(size_t)np;
-	compute_uinf(s);
-	memset(rhs, 0, 6*(size_t)np*sizeof(double));
-	/* contact (Contact.cpp:411-437) */
-	for (k = 0; k < s->npairs; k++) {
-		orc_pair *q = &s->pairs[k];
-		if (!q->contact_active) continue;
-		add3(F+3*q->p0, q->f_total);
-		sub3(F+3*q->p1, q->f_total);
-		if (q->state >= 2) {
-			double t[3], t0[3], t1[3];
-			cross3(q->nvec, q->f_tan, t);
-			t0[0] = q->a0*t[0]; t0[1] = q->a0*t[1]; t0[2] = q->a0*t[2];
-			t1[0] = q->a1*t[0]; t1[1] = q->a1*t[1]; t1[2] = q->a1*t[2];
-			add3(T+3*q->p0, t0);
-			add3(T+3*q->p1, t1);
-		}
+	const double *n = q->nvec, *c = q->dc;
+	double vi[3], vj[3], t[3], d;
+	memcpy(vi, v0, sizeof(vi));
+	memcpy(vj, v1, sizeof(vj));
+	if (!nonaffine) {
+		double zo = (double)q->z_offset;
+		vj[0] += zo*s->vel_difference[0];
+		vj[1] += zo*s->vel_difference[1];
+		vj[2] += zo*s->vel_difference[2];
 	}
-	for (i = 0; i < np; i++) {
-		add3(rhs+6*i, F+3*i);
-		if (s->friction) add3(rhs+6*i+3, T+3*i);
+	t[0] = c[0]*vi[0]+c[1]*vj[0];
+	t[1] = c[0]*vi[1]+c[1]*vj[1];
+	t[2] = c[0]*vi[2]+c[1]*vj[2];
+	d = -dot3(t, n);
+	f[0] = d*n[0]; f[1] = d*n[1]; f[2] = d*n[2];
+	if (s->tangential_coeff > 0) {
+		double di = dot3(n, vi), dj = dot3(n, vj), a[3], b[3], ci[3], cj[3];
+		a[0] = vi[0]-n[0]*di; a[1] = vi[1]-n[1]*di; a[2] = vi[2]-n[2]*di;
+		b[0] = vj[0]-n[0]*dj; b[1] = vj[1]-n[1]*dj; b[2] = vj[2]-n[2]*dj;
+		f[0] += -c[4]*a[0]-c[5]*b[0];
+		f[1] += -c[4]*a[1]-c[5]*b[1];
+		f[2] += -c[4]*a[2]-c[5]*b[2];
+		cross3(n, w0, ci);
+		cross3(n, w1, cj);
+		f[0] += -c[8]*ci[0]-c[10]*cj[0];
+		f[1] += -c[8]*ci[1]-c[10]*cj[1];
+		f[2] += -c[8]*ci[2]-c[10]*cj[2];
 	}
-	/* dashpot: R_FU^dash U_inf (ContactDashpot.cpp:342-385, 426-441) */
-	memset(F, 0, 6*(size_t)np*sizeof(double));
-	if (s->p.shear_rate != 0) {
+}
+
+/* Per-particle force F and torque T (3 np each, zeroed here) of one force
+ * component (System.cpp:169-215): ORC_CONTACT, ORC_DASHPOT, ORC_HYDRO,
+ * ORC_REPULSION.  Returns 1 when the component carries a torque, 0 when not,
+ * -1 when the component is not declared for this system. */
+static int component_forces(orc_system *s, int comp, double *F, double *T)
+{
+	int np = s->np, k;
+	memset(F, 0, 3*(size_t)np*sizeof(double));
+	memset(T, 0, 3*(size_t)np*sizeof(double));
+	if (comp == ORC_CONTACT) {
+		/* System::setContactForceToParticle (System.cpp:1800-1814; Contact.cpp:411-437) */
 		for (k = 0; k < s->npairs; k++) {
 			orc_pair *q = &s->pairs[k];
-			const double *n = q->nvec, *c = q->dc;
-			double vi[3], vj[3], t[3], f[3], d, zo = (double)q->z_offset;
 			if (!q->contact_active) continue;
-			memcpy(vi, s->u_inf+3*q->p0, sizeof(vi));
-			memcpy(vj, s->u_inf+3*q->p1, sizeof(vj));
-			vj[0] += zo*s->vel_difference[0];
-			vj[1] += zo*s->vel_difference[1];
-			vj[2] += zo*s->vel_difference[2];
-			t[0] = c[0]*vi[0]+c[1]*vj[0];
-			t[1] = c[0]*vi[1]+c[1]*vj[1];
-			t[2] = c[0]*vi[2]+c[1]*vj[2];
-			d = -dot3(t, n);
-			f[0] = d*n[0]; f[1] = d*n[1]; f[2] = d*n[2];
-			if (s->tangential_coeff > 0) {
-				double di = dot3(n, vi), dj = dot3(n, vj), a[3], b[3], ci[3], cj[3];
-				a[0] = vi[0]-n[0]*di; a[1] = vi[1]-n[1]*di; a[2] = vi[2]-n[2]*di;
-				b[0] = vj[0]-n[0]*dj; b[1] = vj[1]-n[1]*dj; b[2] = vj[2]-n[2]*dj;
-				f[0] += -c[4]*a[0]-c[5]*b[0];
-				f[1] += -c[4]*a[1]-c[5]*b[1];
-				f[2] += -c[4]*a[2]-c[5]*b[2];
-				cross3(n, s->omega, ci);
-				cross3(n, s->omega, cj);
-				f[0] += -c[8]*ci[0]-c[10]*cj[0];
-				f[1] += -c[8]*ci[1]-c[10]*cj[1];
-				f[2] += -c[8]*ci[2]-c[10]*cj[2];
+			add3(F+3*q->p0, q->f_total);
+			sub3(F+3*q->p1, q->f_total);
+			if (q->state >= 2) {
+				double t[3], t0[3], t1[3];
+				cross3(q->nvec, q->f_tan, t);
+				t0[0] = q->a0*t[0]; t0[1] = q->a0*t[1]; t0[2] = q->a0*t[2];
+				t1[0] = q->a1*t[0]; t1[1] = q->a1*t[1]; t1[2] = q->a1*t[2];
+				add3(T+3*q->p0, t0);
+				add3(T+3*q->p1, t1);
 			}
+		}
+		return s->friction ? 1 : 0;
+	}
+	if (comp == ORC_DASHPOT) {
+		/* R_FU^dash U_inf (System.cpp:1727-1750; ContactDashpot.cpp:426-441) */
+		if (s->p.shear_rate == 0) return -1;
+		for (k = 0; k < s->npairs; k++) {
+			orc_pair *q = &s->pairs[k];
+			double f[3];
+			if (!q->contact_active) continue;
+			dashpot_force(s, q, s->u_inf+3*q->p0, s->u_inf+3*q->p1, s->omega, s->omega, 0, f);
 			add3(F+3*q->p0, f);
 			sub3(F+3*q->p1, f);
 			if (s->tangential_coeff > 0) {
 				double tq[3], t0[3], t1[3], ratio = q->a1/q->a0;
-				cross3(n, f, tq);
+				cross3(q->nvec, f, tq);
 				t0[0] = q->a0*tq[0]; t0[1] = q->a0*tq[1]; t0[2] = q->a0*tq[2];
 				t1[0] = ratio*t0[0]; t1[1] = ratio*t0[1]; t1[2] = ratio*t0[2];
 				add3(T+3*q->p0, t0);
 				add3(T+3*q->p1, t1);
 			}
 		}
+		return 1;
 	}
-	for (i = 0; i < np; i++) {
-		add3(rhs+6*i, F+3*i);
-		add3(rhs+6*i+3, T+3*i);
-	}
-	/* hydro: R_FE:E_inf (Lubrication.cpp:378-403 / 332-355, System.cpp:1752-1798) */
-	memset(F, 0, 6*(size_t)np*sizeof(double));
-	if (s->p.shear_rate != 0) {
+	if (comp == ORC_HYDRO) {
+		/* R_FE:E_inf (Lubrication.cpp:378-403 / 332-355, System.cpp:1752-1798) */
+		if (s->p.shear_rate == 0) return -1;
 		for (k = 0; k < s->npairs; k++) {
 			orc_pair *q = &s->pairs[k];
 			const double *n = q->nvec;
@@ -1259,23 +1281,57 @@ int orc_build_rhs(orc_system *s, double *rhs)
 				add3(F+3*q->p0, GEi); add3(F+3*q->p1, GEj);
 			}
 		}
+		return s->p.lubrication_model == 2 ? 1 : 0;
 	}
-	for (i = 0; i < np; i++) {
-		add3(rhs+6*i, F+3*i);
-		if (s->p.lubrication_model == 2) add3(rhs+6*i+3, T+3*i);
-	}
-	/* repulsion (RepulsiveForce.cpp:101-105) */
-	if (s->repulsive) {
-		memset(F, 0, 3*(size_t)np*sizeof(double));
+	if (comp == ORC_REPULSION) {
+		/* System::setRepulsiveForceToParticle (System.cpp:1816-1828; RepulsiveForce.cpp:101-105) */
+		if (!s->repulsive) return -1;
 		for (k = 0; k < s->npairs; k++) {
 			orc_pair *q = &s->pairs[k];
 			add3(F+3*q->p0, q->rep_force);
 			sub3(F+3*q->p1, q->rep_force);
 		}
-		for (i = 0; i < np; i++) add3(rhs+6*i, F+3*i);
+		return 0;
+	}
+	return -1;
+}
+
+int orc_build_rhs(orc_system *s, double *rhs)
+{
+	/* System::computeVelocityWithoutComponents: the force components in
+	 * std::map order contact, dashpot, hydro, repulsion added into the RHS
+	 * (System.cpp:1932-1945, 1873-1885; StokesSolver.cpp:513-549) */
+	int np = s->np, i, comp;
+	double *F = malloc(6*(size_t)np*sizeof(double)), *T = F+3*(size_t)np;
+	compute_uinf(s);
+	memset(rhs, 0, 6*(size_t)np*sizeof(double));
+	for (comp = 0; comp < ORC_NCOMP; comp++) {
+		int tq = component_forces(s, comp, F, T);
+		if (tq < 0) continue;
+		for (i = 0; i < np; i++) {
+			add3(rhs+6*i, F+3*i);
+			if (tq) add3(rhs+6*i+3, T+3*i);
+		}
 	}
 	free(F);
 	return 0;
+}
+
+int orc_build_rhs_component(orc_system *s, int comp, double *rhs)
+{
+	/* System::setSolverRHS (System.cpp:1858-1871): force and torque set,
+	 * torque zero for a component without torque */
+	int np = s->np, i, tq;
+	double *F = malloc(6*(size_t)np*sizeof(double)), *T = F+3*(size_t)np;
+	tq = component_forces(s, comp, F, T);
+	memset(rhs, 0, 6*(size_t)np*sizeof(double));
+	if (tq >= 0)
+		for (i = 0; i < np; i++) {
+			memcpy(rhs+6*i, F+3*i, 3*sizeof(double));
+			if (tq) memcpy(rhs+6*i+3, T+3*i, 3*sizeof(double));
+		}
+	free(F);
+	return tq < 0 ? -1 : 0;
 }
 
 /* ======================================================================= */
@@ -1604,6 +1660,11 @@ orc_system *orc_create(int np, const double *pos, const double *radius, const do
 	s->repulsive = p->repulsive_length > 0;
 	s->tangential_coeff = dashpot_tangential_coeff(p);
 	s->dt = 1e-4;
+	s->init_strain_limit = 0;       /* System.cpp:90-91 */
+	s->init_rate_limit = 999;
+	s->comp_vel = calloc(ORC_NCOMP*3*(size_t)np, sizeof(double));
+	s->comp_ang = calloc(ORC_NCOMP*3*(size_t)np, sizeof(double));
+	s->pstress = calloc(ORC_NSC*6*(size_t)np, sizeof(double));
 	set_shear_rate(s, p->shear_rate);
 	return s;
 }
@@ -1616,6 +1677,7 @@ void orc_destroy(orc_system *s)
 	free(s->ctype); free(s->static_nb); free(s->n_static); free(s->moving_nb); free(s->n_moving);
 	free(s->snap_start); free(s->pairs); free(s->dblocks); free(s->odblocks); free(s->od_cp);
 	free(s->od_rows);
+	free(s->comp_vel); free(s->comp_ang); free(s->pstress);
 	free(s);
 }
 
@@ -1763,15 +1825,406 @@ static int displacement(orc_system *s, int i, const double dr[3])
 	return box_particle(s, i);
 }
 
-int orc_step(orc_system *s, double strain_end)
+/* ---- velocity components, stress, stress control ---------------------- */
+static void outer_sym3(const double a[3], const double b[3], double o[6])
+{
+	/* Sym2Tensor.h:256-264, (xx, xy, xz, yz, yy, zz) */
+	o[0] = a[0]*b[0];
+	o[1] = 0.5*(a[0]*b[1]+a[1]*b[0]);
+	o[2] = 0.5*(a[0]*b[2]+a[2]*b[0]);
+	o[3] = 0.5*(a[1]*b[2]+a[2]*b[1]);
+	o[4] = a[1]*b[1];
+	o[5] = a[2]*b[2];
+}
+static void outer3(const double a[3], double o[6])
+{
+	/* Sym2Tensor.h:266-274 */
+	o[0] = a[0]*a[0]; o[1] = a[0]*a[1]; o[2] = a[0]*a[2];
+	o[3] = a[1]*a[2]; o[4] = a[1]*a[1]; o[5] = a[2]*a[2];
+}
+static double doubledot6(const double a[6], const double b[6])
+{
+	/* Sym2Tensor.h:246-254 */
+	return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5];
+}
+static double volume(const orc_system *s) { return s->L[0]*s->L[1]*s->L[2]; }
+static const double ORC_PI = 3.14159265358979323846;   /* M_PI */
+
+int orc_set_control(orc_system *s, int control, double target_stress)
+{
+	s->control = control;
+	s->target_stress = target_stress;
+	return 0;
+}
+
+int orc_set_shear_rate_limit(orc_system *s, double init_strain_limit, double init_rate_limit)
+{
+	s->init_strain_limit = init_strain_limit;
+	s->init_rate_limit = init_rate_limit;
+	return 0;
+}
+
+double orc_shear_rate(const orc_system *s) { return s->p.shear_rate; }
+
+/* XM, YM of a lubricated pair at the current gap (calcXYFunctionsStress /
+ * calcXFunctionsStress, Lubrication.cpp:305-330) */
+static void lub_stress_functions(const orc_system *s, const orc_pair *q, double XM[4], double YM[4])
+{
+	int j;
+	for (j = 0; j < 4; j++) {
+		XM[j] = q->lc[4*cXM+j]*q->lub_coeff;
+		YM[j] = s->p.lubrication_model == 2 ? q->lc[4*cYM+j]*q->log_lub_coeff : 0;
+	}
+}
+
+/* Lubrication::addGUStresslet (Lubrication.cpp:587-615) */
+static void add_gu(const orc_system *s, const orc_pair *q, const double YM[4], const double *vi, const double *vj,
+                   const double *oi, const double *oj, double *Si, double *Sj)
+{
+	const double *n = q->nvec;
+	double nvi = dot3(n, vi), nvj = dot3(n, vj), nn[6], ci, cj;
+	int k;
+	outer3(n, nn);
+	ci = -(q->XG[0]*nvi+q->XG[1]*nvj);
+	cj = -(q->XG[2]*nvi+q->XG[3]*nvj);
+	for (k = 0; k < 6; k++) { Si[k] += ci*nn[k]; Sj[k] += cj*nn[k]; }
+	if (s->p.lubrication_model != 2) return;
+	{
+		double ti[6], tj[6], t2i[6], t2j[6], wi[3], wj[3];
+		outer_sym3(n, vi, ti);
+		outer_sym3(n, vj, tj);
+		for (k = 0; k < 6; k++) { ti[k] = ti[k]-nvi*nn[k]; tj[k] = tj[k]-nvj*nn[k]; }
+		for (k = 0; k < 6; k++) {
+			Si[k] += -2*(q->YG[0]*ti[k]+q->YG[1]*tj[k]);
+			Sj[k] += -2*(q->YG[2]*ti[k]+q->YG[3]*tj[k]);
+		}
+		cross3(oi, n, wi);
+		cross3(oj, n, wj);
+		outer_sym3(n, wi, t2i);
+		outer_sym3(n, wj, t2j);
+		for (k = 0; k < 6; k++) {
+			Si[k] += -2*(YM[0]*t2i[k]+YM[1]*t2j[k]);
+			Sj[k] += -2*(YM[2]*t2i[k]+YM[3]*t2j[k]);
+		}
+	}
+}
+
+/* Lubrication::addMEStresslet (Lubrication.cpp:617-650) */
+static void add_me(const orc_system *s, const orc_pair *q, const double XM[4], const double YM[4],
+                   double *Si, double *Sj)
+{
+	const double *n = q->nvec;
+	double En[3], nnE, nn[6], coeff, ci, cj;
+	int k;
+	symdot(s->E, n, En);
+	nnE = dot3(n, En);
+	outer3(n, nn);
+	coeff = 1.5*nnE;
+	ci = coeff*(XM[0]+XM[1]);
+	cj = coeff*(XM[2]+XM[3]);
+	for (k = 0; k < 6; k++) { Si[k] += ci*nn[k]; Sj[k] += cj*nn[k]; }
+	if (s->p.lubrication_model == 2) {
+		double nEn[6], t[6], yi = YM[0]+YM[1], yj = YM[2]+YM[3];
+		outer_sym3(n, En, nEn);
+		for (k = 0; k < 6; k++) t[k] = 2*(nEn[k]-nnE*nn[k]);
+		for (k = 0; k < 6; k++) { Si[k] += yi*t[k]; Sj[k] += yj*t[k]; }
+	}
+}
+
+static void add_scaled(double *S, double a, const double t[6])
+{
+	int k;
+	for (k = 0; k < 6; k++) S[k] += a*t[k];
+}
+
+/* System::calcStressPerParticle (Stress.cpp:194-287) with
+ * calcContactXFPerParticleStressControlled (Stress.cpp:145-192) */
+static void calc_stress_per_particle(orc_system *s)
+{
+	size_t np = (size_t)s->np;
+	double *ps = s->pstress;
+	int k, c, i;
+	memset(ps, 0, ORC_NSC*6*np*sizeof(double));
+	/* GU per velocity component, M:E (lubricated pairs) */
+	for (k = 0; k < s->npairs; k++) {
+		const orc_pair *q = &s->pairs[k];
+		double XM[4], YM[4];
+		if (!q->lub_active) continue;
+		lub_stress_functions(s, q, XM, YM);
+		add_me(s, q, XM, YM, ps+6*(size_t)q->p0, ps+6*(size_t)q->p1);
+		for (c = 0; c < ORC_NCOMP; c++) {
+			const double *v = s->comp_vel+3*np*c, *w = s->comp_ang+3*np*c;
+			double *S = ps+6*np*(SC_GU+c);
+			if (!s->comp_present[c]) continue;
+			add_gu(s, q, YM, v+3*q->p0, v+3*q->p1, w+3*q->p0, w+3*q->p1, S+6*(size_t)q->p0, S+6*(size_t)q->p1);
+		}
+	}
+	if (s->control == 0) {
+		/* Contact::addUpStress (Contact.cpp:439-458): xF of spring + dashpot at the total velocities */
+		double *S = ps+6*np*SC_XF_CONTACT;
+		for (k = 0; k < s->npairs; k++) {
+			const orc_pair *q = &s->pairs[k];
+			double f[3], F[3], st[6], rij;
+			if (!q->contact_active) continue;
+			dashpot_force(s, q, s->vel+3*q->p0, s->vel+3*q->p1, s->ang+3*q->p0, s->ang+3*q->p1, 0, f);
+			F[0] = q->f_total[0]+f[0]; F[1] = q->f_total[1]+f[1]; F[2] = q->f_total[2]+f[2];
+			outer_sym3(q->rvec, F, st);
+			rij = q->a0+q->a1;
+			add_scaled(S+6*(size_t)q->p0, q->a0/rij, st);
+			add_scaled(S+6*(size_t)q->p1, q->a1/rij, st);
+		}
+	} else {
+		/* gatherVelocitiesByRateDependencies (Stress.cpp:116-143) */
+		double *pv = calloc(12*np, sizeof(double));
+		double *pw = pv+3*np, *iv = pv+6*np, *iw = pv+9*np;
+		double *Sind = ps+6*np*SC_XF_CONTACT, *Spro = ps+6*np*SC_XF_CONTACT_PROP;
+		static const int prop[ORC_NCOMP] = {0, 1, 1, 0};
+		for (c = 0; c < ORC_NCOMP; c++) {
+			const double *v = s->comp_vel+3*np*c, *w = s->comp_ang+3*np*c;
+			if (!s->comp_present[c]) continue;
+			for (i = 0; i < (int)(3*np); i++) {
+				if (prop[c]) { pv[i] += v[i]; pw[i] += w[i]; }
+				else { iv[i] += v[i]; iw[i] += w[i]; }
+			}
+		}
+		for (i = 0; i < (int)np; i++) {
+			add3(pv+3*i, s->u_inf+3*i);
+			add3(pw+3*i, s->omega);
+		}
+		for (k = 0; k < s->npairs; k++) {
+			const orc_pair *q = &s->pairs[k];
+			double st[6], f[3], rij = s->radius[q->p0]+s->radius[q->p1];
+			if (!q->contact_active) continue;
+			/* Contact::addUpStressSpring (Contact.cpp:460-467) */
+			outer_sym3(q->rvec, q->f_total, st);
+			add_scaled(Sind+6*(size_t)q->p0, q->a0/(q->a0+q->a1), st);
+			add_scaled(Sind+6*(size_t)q->p1, q->a1/(q->a0+q->a1), st);
+			dashpot_force(s, q, pv+3*q->p0, pv+3*q->p1, pw+3*q->p0, pw+3*q->p1, 0, f);
+			outer_sym3(q->rvec, f, st);
+			add_scaled(Spro+6*(size_t)q->p0, s->radius[q->p0]/rij, st);
+			add_scaled(Spro+6*(size_t)q->p1, s->radius[q->p1]/rij, st);
+			dashpot_force(s, q, iv+3*q->p0, iv+3*q->p1, iw+3*q->p0, iw+3*q->p1, 1, f);
+			outer_sym3(q->rvec, f, st);
+			add_scaled(Sind+6*(size_t)q->p0, s->radius[q->p0]/rij, st);
+			add_scaled(Sind+6*(size_t)q->p1, s->radius[q->p1]/rij, st);
+		}
+		free(pv);
+	}
+	if (s->repulsive) {
+		/* RepulsiveForce::addUpStressXF (RepulsiveForce.cpp:111-131): half to each particle */
+		double *S = ps+6*np*SC_XF_REP;
+		for (k = 0; k < s->npairs; k++) {
+			const orc_pair *q = &s->pairs[k];
+			double st[6], sc[6];
+			int m;
+			outer_sym3(q->rvec, q->rep_force, st);
+			for (m = 0; m < 6; m++) sc[m] = 0.5*st[m];
+			add3(S+6*(size_t)q->p0, sc); add3(S+6*(size_t)q->p0+3, sc+3);
+			add3(S+6*(size_t)q->p1, sc); add3(S+6*(size_t)q->p1+3, sc+3);
+		}
+	}
+}
+
+/* stress component slot -> group, rate dependence (1 proportional, 0 independent) */
+static const int sc_group[ORC_NSC] = {ORC_G_HYDRO, ORC_G_CONTACT, ORC_G_DASHPOT, ORC_G_HYDRO, ORC_G_REPULSION,
+                                      ORC_G_CONTACT, ORC_G_CONTACT, ORC_G_REPULSION};
+static const int sc_prop[ORC_NSC] = {1, 0, 1, 1, 0, 0, 1, 0};
+
+static int sc_present(const orc_system *s, int k)
+{
+	if (k >= SC_GU && k < SC_GU+ORC_NCOMP) return s->comp_present[k-SC_GU];
+	if (k == SC_XF_CONTACT_PROP) return s->control == 1;
+	if (k == SC_XF_REP) return s->repulsive;
+	return 1;
+}
+
+/* StressComponent::getTotalStress (StressComponent.h:26-33): sum over particles */
+static void sc_total(const orc_system *s, int k, double t[6])
+{
+	const double *S = s->pstress+6*(size_t)s->np*k;
+	int i, m;
+	for (m = 0; m < 6; m++) t[m] = 0;
+	for (i = 0; i < s->np; i++) for (m = 0; m < 6; m++) t[m] += S[6*i+m];
+}
+
+/* System::computeShearRate + gatherStressesByRateDependencies
+ * (System.cpp:2017-2055; Stress.cpp:320-340) */
+static void compute_shear_rate(orc_system *s)
+{
+	double prop[6] = {0, 0, 0, 0, 0, 0}, indep[6] = {0, 0, 0, 0, 0, 0}, t[6], vinv, rp, ri;
+	int k, m;
+	calc_stress_per_particle(s);
+	for (k = 0; k < ORC_NSC; k++) {
+		if (!sc_present(s, k)) continue;
+		sc_total(s, k, t);
+		for (m = 0; m < 6; m++) {
+			if (sc_prop[k]) prop[m] += t[m];
+			else indep[m] += t[m];
+		}
+	}
+	vinv = 1.0/volume(s);
+	for (m = 0; m < 6; m++) { prop[m] *= vinv; indep[m] *= vinv; }
+	for (m = 0; m < 6; m++) prop[m] += (2*s->E[m])/(6*ORC_PI);
+	rp = doubledot6(prop, s->E);
+	ri = doubledot6(indep, s->E);
+	if (rp != 0) set_shear_rate(s, (s->target_stress-ri)/rp);
+	if (s->strain < s->init_strain_limit && s->p.shear_rate > s->init_rate_limit)
+		set_shear_rate(s, s->init_rate_limit);
+}
+
+int orc_compute_velocities_components(orc_system *s)
+{
+	/* System::computeVelocities(divided_velocities) (System.cpp:2273-2319)
+	 * with computeVelocityByComponents (System.cpp:1947-1965) */
+	size_t np = (size_t)s->np;
+	int i, c, st = 0;
+	double *rhs = malloc(12*np*sizeof(double)), *x = rhs+6*np;
+	orc_chol *ch;
+	if (s->control == 1) set_shear_rate(s, 1);
+	compute_uinf(s);
+	ensure_block_scratch(s);
+	orc_build_blocks(s, s->dblocks, s->od_cp, s->od_rows, s->odblocks, s->od_cap);
+	ch = orc_chol_factor(s->np, s->dblocks, s->od_cp, s->od_rows, s->odblocks, &st);
+	for (c = 0; c < ORC_NCOMP; c++) {
+		double *v = s->comp_vel+3*np*c, *w = s->comp_ang+3*np*c;
+		s->comp_present[c] = orc_build_rhs_component(s, c, rhs) == 0;
+		if (!s->comp_present[c]) continue;
+		orc_chol_solve(ch, rhs, x);
+		for (i = 0; i < (int)np; i++) {
+			memcpy(v+3*i, x+6*i, 3*sizeof(double));
+			memcpy(w+3*i, x+6*i+3, 3*sizeof(double));
+		}
+	}
+	orc_chol_free(ch);
+	free(rhs);
+	if (s->control == 1) {
+		/* computeShearRate, then rescaleVelHydroStressControlled (System.cpp:2264-2271) */
+		compute_shear_rate(s);
+		for (c = 0; c < ORC_NCOMP; c++) {
+			double *v = s->comp_vel+3*np*c, *w = s->comp_ang+3*np*c;
+			if (!s->comp_present[c] || !(c == ORC_DASHPOT || c == ORC_HYDRO)) continue;
+			for (i = 0; i < (int)(3*np); i++) { v[i] *= s->p.shear_rate; w[i] *= s->p.shear_rate; }
+		}
+	}
+	/* sumUpVelocityComponents (System.cpp:2236-2250) */
+	memset(s->na_vel, 0, 3*np*sizeof(double));
+	memset(s->na_ang, 0, 3*np*sizeof(double));
+	for (c = 0; c < ORC_NCOMP; c++) {
+		const double *v = s->comp_vel+3*np*c, *w = s->comp_ang+3*np*c;
+		if (!s->comp_present[c]) continue;
+		for (i = 0; i < (int)np; i++) { add3(s->na_vel+3*i, v+3*i); add3(s->na_ang+3*i, w+3*i); }
+	}
+	if (s->in_predictor) {
+		double m = 0;
+		for (i = 0; i < (int)np; i++) {
+			const double *v = s->na_vel+3*i;
+			double q = v[0]*v[0]+v[1]*v[1]+v[2]*v[2];
+			if (q > m) m = q;
+		}
+		s->max_velocity = sqrt(m);
+	}
+	/* adjustVelocityPeriodicBoundary (System.cpp:2369-2392) */
+	if (s->control == 1) compute_uinf(s);
+	for (i = 0; i < (int)np; i++) {
+		int k;
+		for (k = 0; k < 3; k++) {
+			s->vel[3*i+k] = s->na_vel[3*i+k]+s->u_inf[3*i+k];
+			s->ang[3*i+k] = s->na_ang[3*i+k]+s->omega[k];
+		}
+	}
+	return st;
+}
+
+int orc_calc_stress(orc_system *s)
+{
+	/* calcStressPerParticle + System::calcStress (Stress.cpp:342-384) */
+	double t[6], vinv = 1.0/volume(s);
+	int k, m, g;
+	calc_stress_per_particle(s);
+	memset(s->groups, 0, sizeof(s->groups));
+	for (k = 0; k < ORC_NSC; k++) {
+		if (!sc_present(s, k)) continue;
+		sc_total(s, k, t);
+		for (m = 0; m < 6; m++) s->groups[sc_group[k]][m] += t[m];
+	}
+	for (g = 0; g < ORC_NGROUP; g++) for (m = 0; m < 6; m++) s->groups[g][m] *= vinv;
+	for (m = 0; m < 6; m++) s->groups[ORC_G_HYDRO][m] += (2*s->E[m])/(6*ORC_PI);
+	for (m = 0; m < 6; m++) s->total_stress[m] = 0;
+	for (g = 0; g < ORC_NGROUP; g++) for (m = 0; m < 6; m++) s->total_stress[m] += s->groups[g][m];
+	return 0;
+}
+
+int orc_get_stress(const orc_system *s, double *groups, double *total, double *particle)
+{
+	int i, k, m;
+	if (groups) memcpy(groups, s->groups, sizeof(s->groups));
+	if (total) memcpy(total, s->total_stress, sizeof(s->total_stress));
+	if (particle) {
+		/* calcTotalStressPerParticle (Stress.cpp:289-300) */
+		memset(particle, 0, 6*(size_t)s->np*sizeof(double));
+		for (k = 0; k < ORC_NSC; k++) {
+			const double *S = s->pstress+6*(size_t)s->np*k;
+			if (!sc_present(s, k)) continue;
+			for (i = 0; i < s->np; i++) for (m = 0; m < 6; m++) particle[6*i+m] += S[6*i+m];
+		}
+	}
+	return 0;
+}
+
+int orc_gu_stress(orc_system *s, const double *vel, const double *ang, double *S)
+{
+	/* addUpInteractionStressGU (Stress.cpp:77-97) for given velocities */
+	int k;
+	memset(S, 0, 6*(size_t)s->np*sizeof(double));
+	for (k = 0; k < s->npairs; k++) {
+		const orc_pair *q = &s->pairs[k];
+		double XM[4], YM[4];
+		if (!q->lub_active) continue;
+		lub_stress_functions(s, q, XM, YM);
+		add_gu(s, q, YM, vel+3*q->p0, vel+3*q->p1, ang+3*q->p0, ang+3*q->p1, S+6*(size_t)q->p0, S+6*(size_t)q->p1);
+	}
+	return 0;
+}
+
+int orc_get_components(const orc_system *s, int comp, double *vel, double *ang)
+{
+	size_t np = (size_t)s->np;
+	if (comp < 0 || comp >= ORC_NCOMP || !s->comp_present[comp]) return -1;
+	if (vel) memcpy(vel, s->comp_vel+3*np*comp, 3*np*sizeof(double));
+	if (ang) memcpy(ang, s->comp_ang+3*np*comp, 3*np*sizeof(double));
+	return 0;
+}
+
+double orc_viscosity(const orc_system *s)
+{
+	/* Simulation::outputData (Simulation.cpp:662-666) */
+	double Eh[6], ct = cos(s->p.theta_shear), st = sin(s->p.theta_shear);
+	const double *E = s->E;
+	if (s->p.shear_rate == 0) {
+		Eh[0] = 0; Eh[1] = 0; Eh[2] = ct/2; Eh[3] = st/2; Eh[4] = 0; Eh[5] = 0;
+		E = Eh;
+	}
+	return 0.5*doubledot6(s->total_stress, E)/doubledot6(E, E);
+}
+
+static int velocities(orc_system *s, int calc_stress)
+{
+	/* System::computeVelocities (System.cpp:2273-2319): by components when
+	 * the stress is wanted or under stress control */
+	if (calc_stress || s->control == 1) return orc_compute_velocities_components(s);
+	return orc_compute_velocities(s);
+}
+
+int orc_step_ex(orc_system *s, double strain_end, int calc_stress, int adapt_first)
 {
 	/* System::timeEvolution loop body + timeEvolutionPredictorCorrectorMethod
-	 * (System.cpp:1241-1256, 888-983, 1124-1187), rate control, no stress */
+	 * (System.cpp:1241-1256, 888-983, 1124-1187) */
 	int i, st = 0;
-	adapt_time_step(s, strain_end);
+	if (adapt_first) adapt_time_step(s, strain_end);
 	/* predictor */
 	s->in_predictor = 1;
-	st |= orc_compute_velocities(s);
+	st |= velocities(s, calc_stress);
+	if (calc_stress) orc_calc_stress(s);
 	adapt_time_step(s, strain_end);
 	s->time += s->dt;
 	if (time_step_boxing(s)) return -1;
@@ -1784,7 +2237,8 @@ int orc_step(orc_system *s, double strain_end)
 	memcpy(s->ang_pred, s->ang, 3*(size_t)s->np*sizeof(double));
 	/* corrector */
 	s->in_predictor = 0;
-	st |= orc_compute_velocities(s);
+	st |= velocities(s, calc_stress);
+	if (calc_stress) orc_calc_stress(s);
 	for (i = 0; i < 3*s->np; i++) {
 		s->vel[i] = 0.5*(s->vel[i]+s->vel_pred[i]);
 		s->ang[i] = 0.5*(s->ang[i]+s->ang_pred[i]);
@@ -1798,6 +2252,25 @@ int orc_step(orc_system *s, double strain_end)
 	orc_check_new_interaction(s);
 	orc_update_interactions(s);
 	return st;
+}
+
+int orc_step(orc_system *s, double strain_end)
+{
+	return orc_step_ex(s, strain_end, 0, 1);
+}
+
+int orc_time_evolution(orc_system *s, double strain_end)
+{
+	/* System::timeEvolution (System.cpp:1203-1279), time_end disabled:
+	 * steps while keepRunning (System.cpp:1189-1201), then the last step
+	 * with the stress, without a new adaptTimeStep */
+	int n = 0;
+	while (!(s->strain > strain_end-1e-8 && strain_end >= 0)) {
+		if (orc_step_ex(s, strain_end, 0, 1) < 0) return -1;
+		n++;
+	}
+	if (orc_step_ex(s, strain_end, 1, 0) < 0) return -1;
+	return n+1;
 }
 
 int orc_set_in_predictor(orc_system *s, int flag) { s->in_predictor = flag; return 0; }

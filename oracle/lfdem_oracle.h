@@ -87,6 +87,34 @@ int orc_build_blocks(orc_system *s, double *dblocks, int *od_col_ptr, int *od_ro
 int orc_build_rhs(orc_system *s, double *rhs);
 int orc_solve_current(orc_system *s, const double *rhs, double *x);
 int orc_compute_velocities(orc_system *s);
+
+/* ---- force / velocity components, stress, stress control ---- */
+/* force components in LF_DEM's std::map order (System.cpp:169-215) */
+enum { ORC_CONTACT = 0, ORC_DASHPOT = 1, ORC_HYDRO = 2, ORC_REPULSION = 3, ORC_NCOMP = 4 };
+/* stress groups (Stress.cpp:16-73), map order */
+enum { ORC_G_CONTACT = 0, ORC_G_DASHPOT = 1, ORC_G_HYDRO = 2, ORC_G_REPULSION = 3, ORC_NGROUP = 4 };
+int orc_build_rhs_component(orc_system *s, int comp, double *rhs);
+/* control: 0 rate, 1 stress (target_stress in internal force units per area) */
+int orc_set_control(orc_system *s, int control, double target_stress);
+int orc_set_shear_rate_limit(orc_system *s, double init_strain_limit, double init_rate_limit);
+/* computeVelocities(divided_velocities=true) incl. the stress-control rate */
+int orc_compute_velocities_components(orc_system *s);
+/* calcStressPerParticle + calcStress at the current velocities */
+int orc_calc_stress(orc_system *s);
+/* group stresses (4 x 6, already / volume, hydro with 2E/(6 pi)), total (6),
+ * per-particle total stress (np x 6, sum of all components) */
+int orc_get_stress(const orc_system *s, double *groups, double *total, double *particle);
+int orc_get_components(const orc_system *s, int comp, double *vel, double *ang);
+/* GU stresslet per particle (np x 6) for given velocities (np x 3 each) */
+int orc_gu_stress(orc_system *s, const double *vel, const double *ang, double *S);
+double orc_shear_rate(const orc_system *s);
+double orc_viscosity(const orc_system *s);
+/* one predictor-corrector step with calc_stress (stress computed after the
+ * corrector); adapt_first = 0 for the last step of timeEvolution */
+int orc_step_ex(orc_system *s, double strain_end, int calc_stress, int adapt_first);
+/* System::timeEvolution to strain_end: steps while keepRunning, then the
+ * stress step; returns the number of steps or -1 */
+int orc_time_evolution(orc_system *s, double strain_end);
 int orc_set_in_predictor(orc_system *s, int flag);
 int orc_set_dt(orc_system *s, double dt);
 

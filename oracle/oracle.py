@@ -77,6 +77,20 @@ def lib():
                                            _ip, _ip, ctypes.c_int]
         L.orc_export_csc.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _ip, _ip, _dp, _ip, _ip, _dp]
         L.orc_csc_nnz.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_build_rhs_component.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_set_control.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.orc_set_shear_rate_limit.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        for name in ("orc_compute_velocities_components", "orc_calc_stress"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+            getattr(L, name).restype = ctypes.c_int
+        L.orc_get_stress.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_get_components.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        for name in ("orc_shear_rate", "orc_viscosity"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+            getattr(L, name).restype = ctypes.c_double
+        L.orc_gu_stress.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_step_ex.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_time_evolution.argtypes = [ctypes.c_void_p, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -243,6 +257,64 @@ class System:
 
     def set_dt(self, dt):
         lib().orc_set_dt(self.h, dt)
+
+    # ---- components, stress, stress control
+    COMPONENTS = ("contact", "dashpot", "hydro", "repulsion")
+    GROUPS = ("contact", "dashpot", "hydro", "repulsion")
+
+    def rhs_component(self, comp):
+        out = np.zeros(6*self.np)
+        rc = lib().orc_build_rhs_component(self.h, self.COMPONENTS.index(comp), out.ctypes.data_as(_dp))
+        return None if rc < 0 else out.reshape(self.np, 6)
+
+    def set_control(self, control, target_stress=0.0):
+        """control: "rate" or "stress" (target_stress in internal units)."""
+        lib().orc_set_control(self.h, {"rate": 0, "stress": 1}[control], ctypes.c_double(target_stress))
+
+    def set_shear_rate_limit(self, init_strain_limit, init_rate_limit):
+        lib().orc_set_shear_rate_limit(self.h, ctypes.c_double(init_strain_limit), ctypes.c_double(init_rate_limit))
+
+    def compute_velocities_components(self):
+        return lib().orc_compute_velocities_components(self.h)
+
+    def calc_stress(self):
+        return lib().orc_calc_stress(self.h)
+
+    def stress(self):
+        g = np.zeros((4, 6))
+        t = np.zeros(6)
+        ps = np.zeros((self.np, 6))
+        lib().orc_get_stress(self.h, g.ctypes.data_as(_dp), t.ctypes.data_as(_dp), ps.ctypes.data_as(_dp))
+        return dict(groups=dict(zip(self.GROUPS, g)), total=t, particle=ps, viscosity=self.viscosity,
+                    shear_rate=self.shear_rate)
+
+    def components(self, comp):
+        v = np.zeros((self.np, 3))
+        w = np.zeros((self.np, 3))
+        rc = lib().orc_get_components(self.h, self.COMPONENTS.index(comp), v.ctypes.data_as(_dp),
+                                      w.ctypes.data_as(_dp))
+        return None if rc < 0 else (v, w)
+
+    def gu_stress(self, vel, ang):
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        w = np.ascontiguousarray(ang, dtype=np.float64)
+        S = np.zeros((self.np, 6))
+        lib().orc_gu_stress(self.h, v.ctypes.data_as(_dp), w.ctypes.data_as(_dp), S.ctypes.data_as(_dp))
+        return S
+
+    @property
+    def shear_rate(self):
+        return lib().orc_shear_rate(self.h)
+
+    @property
+    def viscosity(self):
+        return lib().orc_viscosity(self.h)
+
+    def step_ex(self, strain_end=-1.0, calc_stress=False, adapt_first=True):
+        return self._check(lib().orc_step_ex(self.h, ctypes.c_double(strain_end), int(calc_stress), int(adapt_first)))
+
+    def time_evolution(self, strain_end):
+        return self._check(lib().orc_time_evolution(self.h, ctypes.c_double(strain_end)))
 
     def set_in_predictor(self, f):
         lib().orc_set_in_predictor(self.h, int(f))

@@ -1,0 +1,152 @@
+"""Oracle checks for the stress path and stress control (CPU only).
+
+Pins of the oracle's stresslet and stress-control code:
+  * the reference's own stress-controlled golden (check/reference/
+    data_D3N200VF0.45Bidi1.4_0.5Cubic_1_repulsive_stress_stress5r.dat, first
+    row; parameters from check/repulsive_stress.txt, target stress 5r) --
+    shear rate, viscosity, lubrication and repulsive viscosities, maximum
+    velocities, to the 6 digits the file prints;
+  * the Lorentz reciprocal relation between the translational GU stresslet
+    (Lubrication.cpp:587-615) and the R_FE:E force (Lubrication.cpp:378-403):
+    sum_i S_i(V):E = -sum_i F_E,i . V;
+  * linearity: velocity components sum to the single-solve velocities, and
+    the component right-hand sides sum to the combined one bit for bit;
+  * stress control lands on the target, and at the rate of a rate-controlled
+    run it reproduces that run's velocities and stresses.
+
+The viscosity in the reference file is 6*pi times the internal value
+0.5*S:E/(E:E) (internal unit of viscosity 1/(6 pi)), and the target "5r" is
+5/(6 pi) in internal stress units.
+"""
+import numpy as np
+import pytest
+
+from lf_dem_amd import confgen
+from oracle import oracle as O
+
+E_HAT = np.array([0, 0, 0.5, 0, 0, 0])
+
+
+def ddot(a, b):
+    return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5]
+
+
+def golden_system(golden):
+    p = O.default_params(kn=5000, kt=2500, sd_coeff=1e-8, tau_n=2e-3, tau_t=0, repulsion=1.0,
+                         repulsive_length=0.02, shear_rate=1.0)
+    S = O.System(golden["dat_pos"], golden["dat_radius"], golden["dat_L"], shear_disp=golden["dat_disp"], params=p)
+    S.set_control("stress", 5.0/(6*np.pi))
+    S.first_update()
+    return S
+
+
+def test_stress_control_matches_reference_golden(golden):
+    S = golden_system(golden)
+    assert S.npairs == 745
+    S.set_in_predictor(True)
+    S.compute_velocities_components()
+    S.calc_stress()
+    st = S.stress()
+    sv = S.state()
+    E = E_HAT*st["shear_rate"]
+    visc = lambda T: 6*np.pi*0.5*ddot(T, E)/ddot(E, E)
+    # first data row: rate 1.08058, viscosity 4.62714, Viscosity(lub) 4.62741,
+    # repulsive XF -0.000481679 + GU 0.00021242, max velocity 2.16531, max angular velocity 1.01205
+    assert st["shear_rate"] == pytest.approx(1.08058, abs=5e-6)
+    assert 6*np.pi*st["viscosity"] == pytest.approx(4.62714, abs=5e-6)
+    assert visc(st["groups"]["hydro"]) == pytest.approx(4.62741, abs=5e-6)
+    assert visc(st["groups"]["repulsion"]) == pytest.approx(-0.000481679+0.00021242, abs=2e-9)
+    assert visc(st["groups"]["contact"]) == 0 and visc(st["groups"]["dashpot"]) == 0
+    assert np.sqrt((sv["na_vel"]**2).sum(1)).max() == pytest.approx(2.16531, abs=5e-6)
+    assert np.sqrt((sv["na_ang_vel"]**2).sum(1)).max() == pytest.approx(1.01205, abs=5e-6)
+
+
+@pytest.fixture(scope="module")
+def c1_sheared():
+    """N=300 phi=0.5 with contacts after a few steps (rate control)."""
+    pos, rad, L = confgen.generate(300, 0.52, 4)
+    S = O.System(pos, rad, L)
+    S.first_update()
+    for _ in range(3):
+        S.step()
+    return S, pos, rad, L
+
+
+def test_gu_reciprocal_to_hydro_force(c1_sheared):
+    S = c1_sheared[0]
+    rhs = S.rhs_component("hydro")
+    rng = np.random.default_rng(0)
+    V = rng.standard_normal((S.np, 3))
+    G = S.gu_stress(V, np.zeros((S.np, 3))).sum(0)
+    lhs, rhs_v = ddot(G, E_HAT), -(rhs[:, :3]*V).sum()
+    assert lhs == pytest.approx(rhs_v, rel=1e-12)
+    assert abs(lhs) > 1
+
+
+def test_rhs_components_sum_to_combined(c1_sheared):
+    S = c1_sheared[0]
+    comb = S.rhs().reshape(S.np, 6)
+    acc = np.zeros_like(comb)
+    for c in S.COMPONENTS:
+        r = S.rhs_component(c)
+        acc = acc+r
+    assert np.array_equal(acc, comb)
+
+
+def test_velocity_components_sum_to_single_solve(c1_sheared):
+    S = c1_sheared[0]
+    S.set_in_predictor(False)
+    S.compute_velocities()
+    ref = S.state()
+    S.compute_velocities_components()
+    got = S.state()
+    for k in ("na_vel", "na_ang_vel"):
+        assert np.linalg.norm(got[k]-ref[k]) <= 1e-11*np.linalg.norm(ref[k])
+    parts = [S.components(c) for c in S.COMPONENTS]
+    v = sum(p[0] for p in parts)
+    assert np.allclose(v, got["na_vel"], rtol=0, atol=1e-12*np.abs(got["na_vel"]).max())
+
+
+def test_stress_totals_are_particle_sums(c1_sheared):
+    S = c1_sheared[0]
+    S.compute_velocities_components()
+    S.calc_stress()
+    st = S.stress()
+    V = np.prod(S.L)
+    tot = st["particle"].sum(0)/V+2*E_HAT/(6*np.pi)
+    assert np.allclose(tot, st["total"], rtol=1e-12, atol=1e-14)
+    assert sum(st["groups"].values()) == pytest.approx(st["total"], rel=1e-12)
+    assert np.any(st["groups"]["contact"] != 0)
+
+
+def test_stress_control_hits_target_and_matches_rate_control():
+    pos, rad, L = confgen.generate(300, 0.52, 4)
+    R = O.System(pos, rad, L)
+    R.first_update()
+    for _ in range(3):
+        R.step()
+    # rate control at rate 1, by components, with stress
+    R.set_in_predictor(False)
+    R.compute_velocities_components()
+    R.calc_stress()
+    sr = R.stress()
+    ref = R.state()
+    target = ddot(sr["total"], E_HAT)
+    # the same state under stress control with that target: rate 1 again
+    R.set_control("stress", target)
+    R.compute_velocities_components()
+    R.calc_stress()
+    ss = R.stress()
+    assert ss["shear_rate"] == pytest.approx(1.0, rel=1e-10)
+    assert ddot(ss["total"], E_HAT) == pytest.approx(target, rel=1e-12)
+    for g in R.GROUPS:
+        assert np.allclose(ss["groups"][g], sr["groups"][g], rtol=1e-8, atol=1e-10*np.abs(sr["total"]).max())
+    got = R.state()
+    assert np.linalg.norm(got["na_vel"]-ref["na_vel"]) <= 1e-9*np.linalg.norm(ref["na_vel"])
+    # a doubled target stress: the new rate reproduces it
+    R.set_control("stress", 2*target)
+    R.compute_velocities_components()
+    R.calc_stress()
+    s2 = R.stress()
+    assert ddot(s2["total"], E_HAT) == pytest.approx(2*target, rel=1e-12)
+    assert s2["shear_rate"] > 1.5
