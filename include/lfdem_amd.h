@@ -145,6 +145,42 @@ int lfd_system_set_opts(lfd_system *s, const lfd_solve_opts *o);
  * counterpart (LF_DEM's own timing is wall clock, Simulation.cpp:478-489). */
 int lfd_system_profile(lfd_system *s, int enable, double *solve_ms_total, int *solve_launches);
 
+/* ---- stress and stress control ---- */
+/* Imposed-flow control: the reference's rate (-r) and stress (-s) modes
+ * (Simulation.cpp:241-264, System.cpp:2273-2299).  Under stress control every
+ * velocity computation solves the rate independent and rate proportional
+ * parts and sets the shear rate from target_stress (System::computeShearRate,
+ * System.cpp:2017-2055); target_stress is in the internal stress unit. */
+#define LFD_RATE_CONTROL 0
+#define LFD_STRESS_CONTROL 1
+int lfd_system_set_control(lfd_system *s, int control, double target_stress);
+/* init_strain_shear_rate_limit / init_shear_rate_limit (System.cpp:90-91, 2050-2054) */
+int lfd_system_set_shear_rate_limit(lfd_system *s, double init_strain_limit, double init_rate_limit);
+/* One predictor-corrector step with the stress computed after the corrector
+ * and no new adaptTimeStep before the predictor: the last step of
+ * System::timeEvolution (System.cpp:1270-1273). */
+int lfd_system_step_stress(lfd_system *s, double strain_end);
+/* System::timeEvolution to strain_end >= 0 (System.cpp:1203-1279): steps while
+ * the strain is below strain_end - 1e-8 (keepRunning, System.cpp:1189-1201),
+ * then the stress step.  *nsteps receives the number of steps taken. */
+int lfd_system_time_evolution(lfd_system *s, double strain_end, int *nsteps);
+/* Velocities by components (computeVelocityByComponents, System.cpp:1947-1965)
+ * and the stress at the current configuration (calcStressPerParticle +
+ * calcStress, Stress.cpp:194-287, 342-384). */
+int lfd_system_compute_stress(lfd_system *s, int in_predictor);
+/* Last computed stress: groups24 = contact, dashpot, hydro, repulsion (6 each,
+ * (xx, xy, xz, yz, yy, zz), already divided by the volume, hydro including the
+ * solvent 2E/(6 pi)); total6; the shear rate; the viscosity
+ * 0.5 S:E/(E:E) (Simulation.cpp:662-666); particle6 (np x 6, optional) the
+ * per-particle total stress (calcTotalStressPerParticle, Stress.cpp:289-300).
+ * Any output pointer may be NULL. */
+int lfd_system_get_stress(lfd_system *s, double *groups24, double *total6, double *shear_rate,
+                          double *viscosity, double *particle6);
+/* Velocity part (0 contact, 1 dashpot, 2 hydro, 3 repulsion) of the last
+ * stress computation, np x 6 (U, W) non-affine; rate proportional parts
+ * already scaled by the shear rate under stress control. */
+int lfd_system_get_parts(lfd_system *s, int part, double *vel6);
+
 #ifdef __cplusplus
 }
 #endif

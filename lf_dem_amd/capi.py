@@ -64,6 +64,13 @@ SIGNATURES = {
     "lfd_system_get_solve_stats": ([_vp, _vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "lfd_system_set_opts": ([_vp, _vp], ctypes.c_int),
     "lfd_system_profile": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
+    "lfd_system_set_control": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "lfd_system_set_shear_rate_limit": ([_vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "lfd_system_step_stress": ([_vp, ctypes.c_double], ctypes.c_int),
+    "lfd_system_time_evolution": ([_vp, ctypes.c_double, _ip], ctypes.c_int),
+    "lfd_system_compute_stress": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_system_get_stress": ([_vp, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "lfd_system_get_parts": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
 }
 
 
@@ -320,6 +327,46 @@ class System:
         n = ctypes.c_int(0)
         _check(lib().lfd_system_profile(self.h, int(enable), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    # ---- stress and stress control
+    GROUPS = ("contact", "dashpot", "hydro", "repulsion")
+    PARTS = ("contact", "dashpot", "hydro", "repulsion")
+
+    def set_control(self, control, target_stress=0.0):
+        """control: "rate" or "stress" (target_stress in internal units)."""
+        _check(lib().lfd_system_set_control(self.h, {"rate": 0, "stress": 1}[control], float(target_stress)))
+
+    def set_shear_rate_limit(self, init_strain_limit, init_rate_limit):
+        _check(lib().lfd_system_set_shear_rate_limit(self.h, float(init_strain_limit), float(init_rate_limit)))
+
+    def step_stress(self, strain_end=-1.0):
+        _check(lib().lfd_system_step_stress(self.h, float(strain_end)))
+
+    def time_evolution(self, strain_end):
+        n = ctypes.c_int(0)
+        _check(lib().lfd_system_time_evolution(self.h, float(strain_end), ctypes.byref(n)))
+        return n.value
+
+    def compute_stress(self, in_predictor=True):
+        _check(lib().lfd_system_compute_stress(self.h, int(in_predictor)))
+
+    def stress(self, particle=False):
+        g = np.zeros((4, 6))
+        t = np.zeros(6)
+        rate = ctypes.c_double(0)
+        visc = ctypes.c_double(0)
+        ps = np.zeros((self.np, 6)) if particle else None
+        _check(lib().lfd_system_get_stress(self.h, _d(g), _d(t), ctypes.byref(rate), ctypes.byref(visc),
+                                           _d(ps) if particle else None))
+        out = dict(groups=dict(zip(self.GROUPS, g)), total=t, shear_rate=rate.value, viscosity=visc.value)
+        if particle:
+            out["particle"] = ps
+        return out
+
+    def part(self, name):
+        v = np.zeros((self.np, 6))
+        _check(lib().lfd_system_get_parts(self.h, self.PARTS.index(name), _d(v)))
+        return v
 
     def solve_stats(self):
         st = SolveStats()

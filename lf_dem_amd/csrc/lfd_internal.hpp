@@ -32,6 +32,12 @@ struct DevScalars {
     int iters_inner, iters_outer;       // last solve
     int total_inner;                    // accumulated over the instance's life
     int pad;
+    Flow flow;                          // imposed flow at the current shear rate
+    // stress control (System::computeShearRate, System.cpp:2017-2055)
+    double target_stress, init_strain_limit, init_rate_limit;
+    // last stress computation: groups contact, dashpot, hydro, repulsion
+    // (already / volume, hydro with the solvent 2E/(6 pi)) and their total
+    double stress_groups[4][6], stress_total[6];
 };
 
 // 6x6-block sparse symmetric matrix.  Off-diagonal blocks are stored once per
@@ -112,7 +118,17 @@ void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* 
 void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
                      const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
                      double* od64, float* od32, double* minv64, float* minv32, double* rhs, double* pdb,
-                     double* prhs, int pcap, DevScalars* err, hipStream_t s);
+                     double* prhs, int pcap, double* rhs_parts, int nparts, DevScalars* err, hipStream_t s);
+// stress (mode 0: stress-control shear rate, mode 1: output groups) and velocity parts
+static constexpr int STRESS_VALS = 48;      // 8 slots x 6 per particle / pair side
+static constexpr int STRESS_CHUNK = 128;
+void launch_stress(const PairRec* pairs, const double4* pos, Consts C, DevScalars* sc, const double* parts,
+                   int nparts, const double* vel6, int n, int control, int mode, const int* row_ptr,
+                   const int* tptr, const int* tidx, double* pst, double* ps, double* partial, int pcap,
+                   hipStream_t s);
+void launch_sum_parts(double* parts, int nparts, double* na6, int n, const DevScalars* sc, int rescale,
+                      hipStream_t s);
+void launch_set_rate(DevScalars* sc, Consts C, double rate, hipStream_t s);
 void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float* diag32, const double* od64,
                            float* od32, double* minv64, float* minv32, int pcap, DevScalars* err, hipStream_t s);
 void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,

@@ -473,7 +473,8 @@ __global__ __launch_bounds__(256) void k_update_pairs(PairRec* pairs, const doub
     double a[3] = {x0.x, x0.y, x0.z}, b[3] = {x1.x, x1.y, x1.z};
     double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
     double dt = dt_override >= 0 ? dt_override : sc->dt;
-    int drop = pair_update(q, a, b, x0.w, x1.w, vel6+6*(size_t)q.p0, vel6+6*(size_t)q.p1, C, sd, dt,
+    const double vd[3] = {sc->flow.vel_diff[0], sc->flow.vel_diff[1], sc->flow.vel_diff[2]};
+    int drop = pair_update(q, a, b, x0.w, x1.w, vel6+6*(size_t)q.p0, vel6+6*(size_t)q.p1, C, vd, sd, dt,
                            in_predictor);
     if (C.friction && (q.flags & F_CONTACT)) atomic_max_nonneg(&sc->max_slide_bits, q.rsv_sq);
     pairs[k] = q;
@@ -555,12 +556,12 @@ void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* 
 }
 
 // ------------------------------------------------------------------ assembly
-__device__ inline void uinf(const Consts& C, const double* x, double* u)
+__device__ inline void uinf(const Flow& F, const double* x, double* u)
 {
     // System::computeUInf (System.cpp:2357-2367)
     double e[3], w[3];
-    symdot(C.E, x, e);
-    cross3(C.omega, x, w);
+    symdot(F.E, x, e);
+    cross3(F.omega, x, w);
     u[0] = e[0]+w[0]; u[1] = e[1]+w[1]; u[2] = e[2]+w[2];
 }
 
@@ -571,7 +572,7 @@ __device__ inline void uinf(const Consts& C, const double* x, double* u)
 struct RhsAcc { double Fc[3], Tc[3], Fd[3], Td[3], Fh[3], Th[3], Fr[3]; };
 
 __device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, double a1, const double* x0,
-                                     const double* x1, const Consts& C, RhsAcc& R)
+                                     const double* x1, const Consts& C, const Flow& F, RhsAcc& R)
 {
     const double* n = q.n;
     if (q.flags & F_CONTACT) {
@@ -583,14 +584,14 @@ __device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, doub
             double a = side == 0 ? a0 : a1;
             for (int k = 0; k < 3; k++) R.Tc[k] += a*t[k];
         }
-        if (C.shear_rate != 0) {
+        if (F.rate != 0) {
             DashCoef c;
             dashpot_coeffs(a0, a1, C.normal_coeff, C.tangential_coeff, c);
             double vi[3], vj[3], t[3], f[3];
-            uinf(C, x0, vi);
-            uinf(C, x1, vj);
+            uinf(F, x0, vi);
+            uinf(F, x1, vj);
             double zo = (double)q.zoff;
-            for (int k = 0; k < 3; k++) vj[k] += zo*C.vel_diff[k];
+            for (int k = 0; k < 3; k++) vj[k] += zo*F.vel_diff[k];
             for (int k = 0; k < 3; k++) t[k] = c.XA[0]*vi[k]+c.XA[1]*vj[k];
             double d = -dot3(t, n);
             for (int k = 0; k < 3; k++) f[k] = d*n[k];
@@ -600,7 +601,7 @@ __device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, doub
                     double av = vi[k]-n[k]*di, bv = vj[k]-n[k]*dj;
                     f[k] += -c.YA[0]*av-c.YA[1]*bv;
                 }
-                cross3(n, C.omega, ci);
+                cross3(n, F.omega, ci);
                 for (int k = 0; k < 3; k++) f[k] += -c.YB[0]*ci[k]-c.YB[2]*ci[k];
             }
             if (side == 0) for (int k = 0; k < 3; k++) R.Fd[k] += f[k];
@@ -617,7 +618,7 @@ __device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, doub
             }
         }
     }
-    if ((q.flags & F_LUB) && C.shear_rate != 0) {
+    if ((q.flags & F_LUB) && F.rate != 0) {
         LubConst c;
         lub_constants(a0, a1, c);
         double lc, llc;
@@ -625,7 +626,7 @@ __device__ inline void pair_rhs_side(const PairRec& q, int side, double a0, doub
         double XG[4], YG[4], YH[4];
         for (int j = 0; j < 4; j++) { XG[j] = c.XG[j]*lc; YG[j] = c.YG[j]*llc; YH[j] = c.YH[j]*llc; }
         double En[3];
-        symdot(C.E, n, En);
+        symdot(F.E, n, En);
         double nnE = dot3(n, En);
         if (C.tangential) {
             double YGa = side == 0 ? YG[0]+YG[2] : YG[1]+YG[3];
@@ -725,12 +726,13 @@ __global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restric
     for (int m = 0; m < 18; m++) { od64[18*(size_t)k+m] = od[m]; od32[18*(size_t)k+m] = (float)od[m]; }
     for (int m = 0; m < 15; m++) { pdb[30*(size_t)k+m] = db0[m]; pdb[30*(size_t)k+15+m] = db1[m]; }
     const double x0[3] = {x04.x, x04.y, x04.z}, x1[3] = {x14.x, x14.y, x14.z};
+    const Flow F = sc->flow;
     for (int side = 0; side < 2; side++) {
         RhsAcc R;
         for (int m = 0; m < 3; m++) {
             R.Fc[m] = 0; R.Tc[m] = 0; R.Fd[m] = 0; R.Td[m] = 0; R.Fh[m] = 0; R.Th[m] = 0; R.Fr[m] = 0;
         }
-        pair_rhs_side(q, side, x04.w, x14.w, x0, x1, C, R);
+        pair_rhs_side(q, side, x04.w, x14.w, x0, x1, C, F, R);
         double* o = prhs+42*(size_t)k+RHS_TERMS*side;
         for (int m = 0; m < 3; m++) {
             o[m] = R.Fc[m]; o[3+m] = R.Tc[m]; o[6+m] = R.Fd[m]; o[9+m] = R.Td[m];
@@ -755,7 +757,8 @@ __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __re
     const double* __restrict__ prhs, const double4* __restrict__ pos, int n, Consts C,
     const int* __restrict__ row_ptr, const int* __restrict__ tptr, const int* __restrict__ tidx,
     double* __restrict__ diag64, float* __restrict__ diag32, double* __restrict__ minv64,
-    float* __restrict__ minv32, double* __restrict__ rhs, DevScalars* err)
+    float* __restrict__ minv32, double* __restrict__ rhs, double* __restrict__ rhs_parts, int nparts,
+    DevScalars* err)
 {
     __shared__ double sv[GD_PER][GD_VALS+1];
     const int lp = threadIdx.x/GD_LANES, c = threadIdx.x%GD_LANES;
@@ -820,15 +823,43 @@ __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __re
         rhs[6*(size_t)ip+m] = F;
         rhs[6*(size_t)ip+3+m] = T;
     }
+    // right-hand sides of the velocity parts (System::setSolverRHS, System.cpp:1858-1871)
+    const size_t m6 = 6*(size_t)n, o6 = 6*(size_t)ip;
+    for (int m = 0; m < 3 && nparts == 4; m++) {
+        rhs_parts[o6+m] = R[m];                                   // contact
+        rhs_parts[o6+3+m] = C.friction ? R[3+m] : 0.0;
+        rhs_parts[m6+o6+m] = R[6+m];                              // dashpot
+        rhs_parts[m6+o6+3+m] = R[9+m];
+        rhs_parts[2*m6+o6+m] = R[12+m];                           // hydro
+        rhs_parts[2*m6+o6+3+m] = C.tangential ? R[15+m] : 0.0;
+        rhs_parts[3*m6+o6+m] = C.repulsive ? R[18+m] : 0.0;       // repulsion
+        rhs_parts[3*m6+o6+3+m] = 0.0;
+    }
+    for (int m = 0; m < 3 && nparts == 2; m++) {
+        // rate independent (contact + repulsion), rate proportional (dashpot + hydro)
+        double F = 0, T = 0;
+        F += R[m];
+        if (C.friction) T += R[3+m];
+        if (C.repulsive) F += R[18+m];
+        rhs_parts[o6+m] = F;
+        rhs_parts[o6+3+m] = T;
+        F = 0; T = 0;
+        F += R[6+m];
+        T += R[9+m];
+        F += R[12+m];
+        if (C.tangential) T += R[15+m];
+        rhs_parts[m6+o6+m] = F;
+        rhs_parts[m6+o6+3+m] = T;
+    }
 }
 void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
                      const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
                      double* od64, float* od32, double* minv64, float* minv32, double* rhs, double* pdb,
-                     double* prhs, int pcap, DevScalars* err, hipStream_t s)
+                     double* prhs, int pcap, double* rhs_parts, int nparts, DevScalars* err, hipStream_t s)
 {
     k_assemble_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, C, sc, od64, od32, pdb, prhs, pcap);
     k_gather_diag_rhs<<<(n+GD_PER-1)/GD_PER, GD_BLOCK, 0, s>>>(pdb, prhs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
-                                                  minv64, minv32, rhs, err);
+                                                  minv64, minv32, rhs, rhs_parts, nparts, err);
 }
 
 // Matrix handed over through the StokesSolver staging protocol: FP32 copy and
@@ -856,6 +887,361 @@ void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float
     k_prepare_diag<<<(n+127)/128, 128, 0, s>>>(n, diag64, diag32, minv64, minv32, err);
 }
 
+// ------------------------------------------------------------------ velocity parts and stress
+// Stress slots, in LF_DEM's std::map order of stress components
+// (Stress.cpp:16-73): M:E, GU of up to four velocity parts, the contact xF
+// (rate control: spring + dashpot; stress control: its rate independent
+// part), the rate proportional contact xF (stress control), the repulsive xF.
+// Velocity parts: four (contact, dashpot, hydro, repulsion) or two (rate
+// independent, rate proportional).
+enum : int { SL_ME = 0, SL_GU = 1, SL_XFC = 5, SL_XFC_PROP = 6, SL_XFR = 7, NSLOT = 8 };
+static constexpr int ST_VALS = 6*NSLOT;
+
+__device__ inline void outer_sym3(const double* a, const double* b, double* o)
+{
+    // Sym2Tensor.h:256-264, (xx, xy, xz, yz, yy, zz)
+    o[0] = a[0]*b[0];
+    o[1] = 0.5*(a[0]*b[1]+a[1]*b[0]);
+    o[2] = 0.5*(a[0]*b[2]+a[2]*b[0]);
+    o[3] = 0.5*(a[1]*b[2]+a[2]*b[1]);
+    o[4] = a[1]*b[1];
+    o[5] = a[2]*b[2];
+}
+__device__ inline void outer3(const double* a, double* o)
+{
+    // Sym2Tensor.h:266-274
+    o[0] = a[0]*a[0]; o[1] = a[0]*a[1]; o[2] = a[0]*a[2];
+    o[3] = a[1]*a[2]; o[4] = a[1]*a[1]; o[5] = a[2]*a[2];
+}
+__device__ inline double ddot6(const double* a, const double* b)
+{
+    // Sym2Tensor.h:246-254
+    return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5];
+}
+
+// Force on p0 from the dashpot at the given velocities
+// (ContactDashpot::getForceOnP0 / getForceOnP0_nonaffine, ContactDashpot.cpp:342-424)
+__device__ inline void dashpot_force(const PairRec& q, const DashCoef& c, const Consts& C, const Flow& F,
+                                     const double* v0, const double* v1, const double* w0, const double* w1,
+                                     int nonaffine, double* f)
+{
+    const double* n = q.n;
+    double vi[3] = {v0[0], v0[1], v0[2]}, vj[3] = {v1[0], v1[1], v1[2]}, t[3];
+    if (!nonaffine) {
+        double zo = (double)q.zoff;
+        for (int k = 0; k < 3; k++) vj[k] += zo*F.vel_diff[k];
+    }
+    for (int k = 0; k < 3; k++) t[k] = c.XA[0]*vi[k]+c.XA[1]*vj[k];
+    double d = -dot3(t, n);
+    for (int k = 0; k < 3; k++) f[k] = d*n[k];
+    if (C.tangential_coeff > 0) {
+        double di = dot3(n, vi), dj = dot3(n, vj), ci[3], cj[3];
+        for (int k = 0; k < 3; k++) {
+            double av = vi[k]-n[k]*di, bv = vj[k]-n[k]*dj;
+            f[k] += -c.YA[0]*av-c.YA[1]*bv;
+        }
+        cross3(n, w0, ci);
+        cross3(n, w1, cj);
+        for (int k = 0; k < 3; k++) f[k] += -c.YB[0]*ci[k]-c.YB[2]*cj[k];
+    }
+}
+
+// Per pair and side, the contribution to every stress slot
+// (calcStressPerParticle, Stress.cpp:194-287):
+//   M:E and GU of each velocity part for lubricated pairs
+//   (Lubrication::addMEStresslet / addGUStresslet, Lubrication.cpp:587-650);
+//   contact xF: rate control Contact::addUpStress at the total velocities
+//   (Contact.cpp:439-458); stress control the spring part and the non-affine
+//   dashpot part (rate independent) and the dashpot part at the rate
+//   proportional velocities (calcContactXFPerParticleStressControlled,
+//   Stress.cpp:145-192);
+//   repulsive xF, half to each particle (RepulsiveForce.cpp:111-131).
+// pst[k][side][slot][6]; unused slots are written as zero.
+__global__ __launch_bounds__(128) void k_stress_pairs(const PairRec* __restrict__ pairs, const double4* __restrict__ pos,
+    Consts C, const DevScalars* sc, const double* __restrict__ parts, int nparts, const double* __restrict__ vel6,
+    int n, int control, double* __restrict__ pst, int pcap)
+{
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= sc->npairs || k >= pcap) return;
+    const PairRec q = pairs[k];
+    const Flow F = sc->flow;
+    const double4 x04 = pos[q.p0], x14 = pos[q.p1];
+    const double a0 = x04.w, a1 = x14.w;
+    const double* n3 = q.n;
+    double* o0 = pst+2*(size_t)ST_VALS*k;
+    double* o1 = o0+ST_VALS;
+    for (int m = 0; m < ST_VALS; m++) { o0[m] = 0; o1[m] = 0; }
+    const size_t m6 = 6*(size_t)n, i0 = 6*(size_t)q.p0, i1 = 6*(size_t)q.p1;
+    // interparticle vector as Interaction::calcNormalVectorDistanceGap computes it
+    double rv[3] = {x14.x-x04.x, x14.y-x04.y, x14.z-x04.z};
+    {
+        double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+        periodize_diff(C.L, C.half, sd, rv);
+    }
+    if (q.flags & F_LUB) {
+        LubConst lc;
+        lub_constants(a0, a1, lc);
+        double cXM[4], cYM[4];
+        lub_stress_constants(a0, a1, cXM, cYM);
+        double co, lco;
+        lub_coeffs(q.gap, C.lub_reduce, co, lco);
+        double XG[4], YG[4], XM[4], YM[4];
+        for (int j = 0; j < 4; j++) {
+            XG[j] = lc.XG[j]*co;
+            YG[j] = C.tangential ? lc.YG[j]*lco : 0.0;
+            XM[j] = cXM[j]*co;
+            YM[j] = C.tangential ? cYM[j]*lco : 0.0;
+        }
+        double nn[6], En[3];
+        outer3(n3, nn);
+        symdot(F.E, n3, En);
+        const double nnE = dot3(n3, En);
+        {   // M:E
+            const double coeff = 1.5*nnE;
+            const double ci = coeff*(XM[0]+XM[1]), cj = coeff*(XM[2]+XM[3]);
+            double t[6];
+            for (int m = 0; m < 6; m++) t[m] = 0;
+            if (C.tangential) {
+                double nEn[6];
+                outer_sym3(n3, En, nEn);
+                for (int m = 0; m < 6; m++) t[m] = 2*(nEn[m]-nnE*nn[m]);
+            }
+            const double yi = YM[0]+YM[1], yj = YM[2]+YM[3];
+            for (int m = 0; m < 6; m++) {
+                o0[6*SL_ME+m] = ci*nn[m]+yi*t[m];
+                o1[6*SL_ME+m] = cj*nn[m]+yj*t[m];
+            }
+        }
+        for (int p = 0; p < nparts; p++) {   // GU
+            const double* vi = parts+p*m6+i0;
+            const double* vj = parts+p*m6+i1;
+            const double nvi = dot3(n3, vi), nvj = dot3(n3, vj);
+            const double ci = -(XG[0]*nvi+XG[1]*nvj), cj = -(XG[2]*nvi+XG[3]*nvj);
+            double si[6], sj[6];
+            for (int m = 0; m < 6; m++) { si[m] = ci*nn[m]; sj[m] = cj*nn[m]; }
+            if (C.tangential) {
+                double ti[6], tj[6], t2i[6], t2j[6], wi[3], wj[3];
+                outer_sym3(n3, vi, ti);
+                outer_sym3(n3, vj, tj);
+                for (int m = 0; m < 6; m++) { ti[m] = ti[m]-nvi*nn[m]; tj[m] = tj[m]-nvj*nn[m]; }
+                cross3(vi+3, n3, wi);
+                cross3(vj+3, n3, wj);
+                outer_sym3(n3, wi, t2i);
+                outer_sym3(n3, wj, t2j);
+                for (int m = 0; m < 6; m++) {
+                    si[m] += -2*(YG[0]*ti[m]+YG[1]*tj[m]);
+                    sj[m] += -2*(YG[2]*ti[m]+YG[3]*tj[m]);
+                    si[m] += -2*(YM[0]*t2i[m]+YM[1]*t2j[m]);
+                    sj[m] += -2*(YM[2]*t2i[m]+YM[3]*t2j[m]);
+                }
+            }
+            for (int m = 0; m < 6; m++) { o0[6*(SL_GU+p)+m] = si[m]; o1[6*(SL_GU+p)+m] = sj[m]; }
+        }
+    }
+    if (q.flags & F_CONTACT) {
+        DashCoef c;
+        dashpot_coeffs(a0, a1, C.normal_coeff, C.tangential_coeff, c);
+        const double rij = a0+a1, w0 = a0/rij, w1 = a1/rij;
+        double st[6], f[3];
+        if (control == 0) {
+            dashpot_force(q, c, C, F, vel6+i0, vel6+i1, vel6+i0+3, vel6+i1+3, 0, f);
+            double Ft[3] = {q.f_total[0]+f[0], q.f_total[1]+f[1], q.f_total[2]+f[2]};
+            outer_sym3(rv, Ft, st);
+            for (int m = 0; m < 6; m++) { o0[6*SL_XFC+m] = w0*st[m]; o1[6*SL_XFC+m] = w1*st[m]; }
+        } else {
+            // rate proportional velocities (with U_inf) and rate independent ones
+            // (gatherVelocitiesByRateDependencies, Stress.cpp:116-143)
+            double vp[2][6], vi[2][6];
+            for (int s = 0; s < 2; s++) {
+                const size_t ii = s == 0 ? i0 : i1;
+                for (int m = 0; m < 6; m++) {
+                    double a = 0, b = 0;
+                    if (nparts == 4) {
+                        a += parts[m6+ii+m]; a += parts[2*m6+ii+m];
+                        b += parts[ii+m]; b += parts[3*m6+ii+m];
+                    } else {
+                        a += parts[m6+ii+m];
+                        b += parts[ii+m];
+                    }
+                    vp[s][m] = a; vi[s][m] = b;
+                }
+                const double4 xs = s == 0 ? x04 : x14;
+                double x[3] = {xs.x, xs.y, xs.z}, u[3];
+                uinf(F, x, u);
+                for (int m = 0; m < 3; m++) { vp[s][m] += u[m]; vp[s][3+m] += F.omega[m]; }
+            }
+            double sp[6], si[6];
+            outer_sym3(rv, q.f_total, st);
+            for (int m = 0; m < 6; m++) si[m] = st[m];
+            dashpot_force(q, c, C, F, vp[0], vp[1], vp[0]+3, vp[1]+3, 0, f);
+            outer_sym3(rv, f, sp);
+            dashpot_force(q, c, C, F, vi[0], vi[1], vi[0]+3, vi[1]+3, 1, f);
+            outer_sym3(rv, f, st);
+            for (int m = 0; m < 6; m++) {
+                o0[6*SL_XFC+m] = w0*si[m]+w0*st[m];
+                o1[6*SL_XFC+m] = w1*si[m]+w1*st[m];
+                o0[6*SL_XFC_PROP+m] = w0*sp[m];
+                o1[6*SL_XFC_PROP+m] = w1*sp[m];
+            }
+        }
+    }
+    if (C.repulsive) {
+        double f[3], st[6];
+        for (int m = 0; m < 3; m++) f[m] = -q.rep_norm*n3[m];
+        outer_sym3(rv, f, st);
+        for (int m = 0; m < 6; m++) { o0[6*SL_XFR+m] = 0.5*st[m]; o1[6*SL_XFR+m] = 0.5*st[m]; }
+    }
+}
+
+// Per particle and slot, the sum of its pairs' contributions in ascending
+// partner order; 64 lanes per particle, lane v < 48 owns value v.
+__global__ __launch_bounds__(256) void k_stress_gather(const double* __restrict__ pst, int n,
+    const int* __restrict__ row_ptr, const int* __restrict__ tptr, const int* __restrict__ tidx,
+    double* __restrict__ ps)
+{
+    const int i = blockIdx.x*4+(int)(threadIdx.x/WAVE);
+    const int v = threadIdx.x & (WAVE-1);
+    if (i >= n || v >= ST_VALS) return;
+    double acc = 0;
+    for (int t = tptr[i]; t < tptr[i+1]; t++) acc += pst[2*(size_t)ST_VALS*tidx[t]+ST_VALS+v];
+    for (int k = row_ptr[i]; k < row_ptr[i+1]; k++) acc += pst[2*(size_t)ST_VALS*k+v];
+    ps[(size_t)ST_VALS*i+v] = acc;
+}
+
+// Sums over particles in fixed chunks (StressComponent::getTotalStress,
+// StressComponent.h:26-33), one partial row of 48 values per chunk.
+static constexpr int ST_CHUNK = 128;
+__global__ void k_stress_chunks(const double* __restrict__ ps, int n, double* __restrict__ partial)
+{
+    const int v = threadIdx.x;
+    if (v >= ST_VALS) return;
+    const int i0 = blockIdx.x*ST_CHUNK, i1 = min(n, i0+ST_CHUNK);
+    double acc = 0;
+    for (int i = i0; i < i1; i++) acc += ps[(size_t)ST_VALS*i+v];
+    partial[(size_t)ST_VALS*blockIdx.x+v] = acc;
+}
+
+// Totals per slot, then either the stress-control shear rate
+// (System::computeShearRate + gatherStressesByRateDependencies,
+// System.cpp:2017-2055, Stress.cpp:320-340) or the output groups
+// (System::calcStress, Stress.cpp:342-384).
+__global__ void k_stress_finalize(const double* __restrict__ partial, int nchunks, Consts C, DevScalars* sc,
+                                  int nparts, int control, int mode)
+{
+    __shared__ double tot[ST_VALS];
+    const int v = threadIdx.x;
+    if (v < ST_VALS) {
+        double acc = 0;
+        for (int b = 0; b < nchunks; b++) acc += partial[(size_t)ST_VALS*b+v];
+        tot[v] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double vinv = 1.0/(C.L[0]*C.L[1]*C.L[2]);
+    const double pi = 3.14159265358979323846;
+    const Flow F = sc->flow;
+    if (mode == 0) {
+        double prop[6], indep[6];
+        for (int m = 0; m < 6; m++) {
+            double a = 0, b = 0;
+            a += tot[6*SL_ME+m];
+            if (nparts == 4) {
+                b += tot[6*SL_GU+m];
+                a += tot[6*(SL_GU+1)+m];
+                a += tot[6*(SL_GU+2)+m];
+                b += tot[6*(SL_GU+3)+m];
+            } else {
+                b += tot[6*SL_GU+m];
+                a += tot[6*(SL_GU+1)+m];
+            }
+            b += tot[6*SL_XFC+m];
+            a += tot[6*SL_XFC_PROP+m];
+            b += tot[6*SL_XFR+m];
+            prop[m] = a*vinv;
+            indep[m] = b*vinv;
+        }
+        for (int m = 0; m < 6; m++) prop[m] += (2*F.E[m])/(6*pi);
+        const double rp = ddot6(prop, F.E), ri = ddot6(indep, F.E);
+        double rate = F.rate;
+        if (rp != 0) rate = (sc->target_stress-ri)/rp;
+        if (sc->strain < sc->init_strain_limit && rate > sc->init_rate_limit) rate = sc->init_rate_limit;
+        Flow Fn;
+        make_flow(C, rate, Fn);
+        sc->flow = Fn;
+        return;
+    }
+    // groups contact, dashpot, hydro, repulsion; components added in map order
+    double g[4][6];
+    for (int m = 0; m < 6; m++) {
+        double c = 0, d = 0, h = 0, r = 0;
+        h += tot[6*SL_ME+m];
+        c += tot[6*SL_GU+m];
+        d += tot[6*(SL_GU+1)+m];
+        h += tot[6*(SL_GU+2)+m];
+        r += tot[6*(SL_GU+3)+m];
+        c += tot[6*SL_XFC+m];
+        if (control == 1) c += tot[6*SL_XFC_PROP+m];
+        r += tot[6*SL_XFR+m];
+        g[0][m] = c*vinv; g[1][m] = d*vinv; g[2][m] = h*vinv; g[3][m] = r*vinv;
+    }
+    for (int m = 0; m < 6; m++) g[2][m] += (2*F.E[m])/(6*pi);
+    for (int m = 0; m < 6; m++) {
+        double t = 0;
+        for (int q = 0; q < 4; q++) { sc->stress_groups[q][m] = g[q][m]; t += g[q][m]; }
+        sc->stress_total[m] = t;
+    }
+}
+
+// na = sum of the velocity parts in order; under stress control the rate
+// proportional parts are first rescaled by the new shear rate, in place
+// (rescaleVelHydroStressControlled + sumUpVelocityComponents, System.cpp:2236-2271).
+__global__ void k_sum_parts(double* __restrict__ parts, int nparts, double* __restrict__ na6, int n,
+                            const DevScalars* sc, int rescale)
+{
+    const size_t t = blockIdx.x*(size_t)blockDim.x+threadIdx.x, m6 = 6*(size_t)n;
+    if (t >= m6) return;
+    const double rate = sc->flow.rate;
+    double acc = 0;
+    for (int p = 0; p < nparts; p++) {
+        double v = parts[p*m6+t];
+        const bool prop = nparts == 4 ? (p == 1 || p == 2) : p == 1;
+        if (rescale && prop) { v *= rate; parts[p*m6+t] = v; }
+        acc += v;
+    }
+    na6[t] = acc;
+}
+
+__global__ void k_set_rate(DevScalars* sc, Consts C, double rate)
+{
+    Flow F;
+    make_flow(C, rate, F);
+    sc->flow = F;
+}
+
+void launch_set_rate(DevScalars* sc, Consts C, double rate, hipStream_t s)
+{
+    k_set_rate<<<1, 1, 0, s>>>(sc, C, rate);
+}
+
+void launch_stress(const PairRec* pairs, const double4* pos, Consts C, DevScalars* sc, const double* parts,
+                   int nparts, const double* vel6, int n, int control, int mode, const int* row_ptr,
+                   const int* tptr, const int* tidx, double* pst, double* ps, double* partial, int pcap,
+                   hipStream_t s)
+{
+    k_stress_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, C, sc, parts, nparts, vel6, n, control, pst, pcap);
+    k_stress_gather<<<(n+3)/4, 256, 0, s>>>(pst, n, row_ptr, tptr, tidx, ps);
+    const int nch = (n+ST_CHUNK-1)/ST_CHUNK;
+    k_stress_chunks<<<nch, 64, 0, s>>>(ps, n, partial);
+    k_stress_finalize<<<1, 64, 0, s>>>(partial, nch, C, sc, nparts, control, mode);
+}
+
+void launch_sum_parts(double* parts, int nparts, double* na6, int n, const DevScalars* sc, int rescale,
+                      hipStream_t s)
+{
+    const size_t m6 = 6*(size_t)n;
+    k_sum_parts<<<(unsigned)((m6+255)/256), 256, 0, s>>>(parts, nparts, na6, n, sc, rescale);
+}
+
 // ------------------------------------------------------------------ velocities, time step, moves
 __global__ void k_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
                                   DevScalars* sc, int in_predictor)
@@ -865,12 +1251,13 @@ __global__ void k_finish_velocity(int n, const double4* pos, const double* na6, 
     if (i >= n) return;
     double4 x4 = pos[i];
     double x[3] = {x4.x, x4.y, x4.z}, u[3];
-    uinf(C, x, u);
+    const Flow& F = sc->flow;
+    uinf(F, x, u);
     const double* v = na6+6*(size_t)i;
     if (in_predictor) atomic_max_nonneg(&sc->max_vel_bits, v[0]*v[0]+v[1]*v[1]+v[2]*v[2]);
     for (int k = 0; k < 3; k++) {
         vel6[6*(size_t)i+k] = v[k]+u[k];
-        vel6[6*(size_t)i+3+k] = v[3+k]+C.omega[k];
+        vel6[6*(size_t)i+3+k] = v[3+k]+F.omega[k];
     }
 }
 __global__ void k_reset_maxvel(DevScalars* sc) { sc->max_vel_bits = 0; }
@@ -884,7 +1271,7 @@ void launch_finish_velocity(int n, const double4* pos, const double* na6, double
 // System::adaptTimeStep (System.cpp:1013-1066), time_end disabled
 __global__ void k_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time)
 {
-    double rate = P.shear_rate;
+    double rate = sc->flow.rate;
     double mv = sqrt(bits_to_double(sc->max_vel_bits));
     double ms = sqrt(bits_to_double(sc->max_slide_bits));
     double dt;
@@ -908,9 +1295,10 @@ void launch_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_ti
 __global__ void k_boxing(DevScalars* sc, Consts C)
 {
     double dt = sc->dt;
-    sc->strain += C.shear_rate*dt;
+    const Flow& F = sc->flow;
+    sc->strain += F.rate*dt;
     double z[3] = {0, 0, 1}, e[3];
-    symdot(C.E, z, e);
+    symdot(F.E, z, e);
     double inc[3] = {2*e[0]*dt, 2*e[1]*dt, 2*e[2]*dt};
     for (int k = 0; k < 3; k++) sc->shear_disp[k] += inc[k]*C.L[2];
     int m = (int)(sc->shear_disp[0]/C.L[0]);
@@ -945,7 +1333,7 @@ __global__ void k_displace(double4* pos, double* vel6, const double* pred6, int 
     int zs = periodize(C.L, sd, x);
     if (zs != 0) {
         double f = (double)zs;
-        for (int k = 0; k < 3; k++) v[k] += f*C.vel_diff[k];
+        for (int k = 0; k < 3; k++) v[k] += f*sc->flow.vel_diff[k];
     }
     pos[i] = make_double4(x[0], x[1], x[2], x4.w);
     int l = which_box(x, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
@@ -1021,6 +1409,26 @@ __device__ inline void spmv_row(const BsrView& A, const T* od, const T* diag, co
         for (int c = 0; c < 6; c++) xi[c] = x[6*(size_t)i+c];
         dmul<T, V>(diag, xi, y);
     }
+    for (int o = 1; o < LPR; o <<= 1)
+        for (int c = 0; c < 6; c++) y[c] += __shfl_xor(y[c], o, LPR);
+}
+
+// Row i of A64 applied to the uniform translation e_a (a < 3): the sum of
+// column a of the row's blocks (LPR lanes per row, result in every lane).
+template <int LPR>
+__device__ inline void row_translation(const BsrView& A, int i, int sub, int a, double* y)
+{
+    double ea[6] = {0, 0, 0, 0, 0, 0};
+    ea[a] = 1;
+    for (int c = 0; c < 6; c++) y[c] = 0;
+    const int u0 = A.row_ptr[i], u1 = A.row_ptr[i+1];
+    const int l0 = A.tptr[i], l1 = A.tptr[i+1];
+    const int nl = l1-l0, nt = nl+(u1-u0);
+    for (int e = sub; e < nt; e += LPR) {
+        if (e < nl) omul<double, double>(A.od64+18*(size_t)A.tidx[l0+e], ea, y);
+        else omul_t<double, double>(A.od64+18*(size_t)(u0+(e-nl)), ea, y);
+    }
+    if (sub == 0) dmul<double, double>(A.diag64+15*(size_t)i, ea, y);
     for (int o = 1; o < LPR; o <<= 1)
         for (int c = 0; c < 6; c++) y[c] += __shfl_xor(y[c], o, LPR);
 }
@@ -1156,8 +1564,15 @@ __global__ __launch_bounds__(PS_BLOCK) void k_solve_mixed(BsrView A, const doubl
         }
         grid_sum2(grid, s_rr, s_bb, part, buf, G, lds);
         if (outer == 0) bb = s_bb;
+        if (bb == 0) {                     // zero right-hand side: the solution is zero
+            for (int i = row0; i < n; i += stride)
+                if (sub == 0) for (int c = 0; c < 6; c++) x[6*(size_t)i+c] = 0.0;
+            rr64 = 0;
+            converged = true;
+            break;
+        }
         rr64 = s_rr;
-        converged = bb == 0 || sqrt(rr64) <= o.tol*sqrt(bb);
+        converged = sqrt(rr64) <= o.tol*sqrt(bb);
         if (converged || outer == o.max_outer) break;
         // inner PCG on A32 d = r64 / ||r64|| ; d0 = 0, p = z = M^-1 r
         const double scale = 1.0/sqrt(rr64);
@@ -1575,6 +1990,39 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         gather3_finish(a, bs, cs);
     };
 
+    // Coarse space of the three uniform translations z_a (U_i = e_a, W_i = 0).
+    // Pair resistances vanish on a rigid translation, so only the Stokes drag
+    // holds these modes; with a small drag coefficient they are near-null and
+    // the FP32 inner solve cannot resolve them.  Each refinement pass first
+    // applies the Galerkin correction x += Z E^-1 Z^T r, E = Z^T A Z.
+    // s_az[a][thread]: (A z_a) at row i, component c; s_ei: E^-1
+    __shared__ double s_az[3][LS_BLOCK];
+    __shared__ double s_ei[9];
+    double AZ[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        double y[6] = {0, 0, 0, 0, 0, 0};
+        if (live) row_translation<LS_LPR>(A, i, c, a, y);
+        for (int m = 0; m < 6; m++) if (m == c && own) AZ[a] = y[m];
+        s_az[a][threadIdx.x] = AZ[a];
+    }
+    bool use_coarse;
+    {
+        double e00 = c == 0 ? AZ[0] : 0.0, e01 = c == 0 ? AZ[1] : 0.0, e02 = c == 0 ? AZ[2] : 0.0;
+        double e11 = c == 1 ? AZ[1] : 0.0, e12 = c == 1 ? AZ[2] : 0.0, e22 = c == 2 ? AZ[2] : 0.0;
+        gsum3(e00, e11, e22);
+        gsum3(e01, e02, e12);
+        const double c00 = e11*e22-e12*e12, c01 = e02*e12-e01*e22, c02 = e01*e12-e02*e11;
+        const double det = e00*c00+e01*c01+e02*c02;
+        use_coarse = det > 0 && isfinite(det) && e00 > 0;
+        const double id = use_coarse ? 1.0/det : 0.0;
+        if (threadIdx.x == 0) {
+            s_ei[0] = c00*id; s_ei[1] = c01*id; s_ei[2] = c02*id;
+            s_ei[4] = (e00*e22-e02*e02)*id; s_ei[5] = (e01*e02-e00*e12)*id; s_ei[8] = (e00*e11-e01*e01)*id;
+            s_ei[3] = s_ei[1]; s_ei[6] = s_ei[2]; s_ei[7] = s_ei[5];
+        }
+        __syncthreads();
+    }
+
     double bb = 0, rr64 = 0;
     int outer, inner_total = 0;
     bool converged = false, use_classic = false;
@@ -1588,14 +2036,30 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             bi = b[gi];
             r64 = bi-r64;
         }
-        double s_rr = r64*r64, s_bb = bi*bi, s_0 = 0;
-        gsum3(s_rr, s_bb, s_0);
+        double s_rr = r64*r64, s_bb = bi*bi, zr0 = c == 0 ? r64 : 0.0;
+        gsum3(s_rr, s_bb, zr0);
         if (s_fail) break;
         if (outer == 0) bb = s_bb;
+        if (bb == 0) {                     // zero right-hand side: the solution is zero
+            if (own) x[gi] = 0.0;
+            rr64 = 0;
+            converged = true;
+            break;
+        }
         if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
         rr64 = s_rr;
-        converged = bb == 0 || sqrt(rr64) <= o.tol*sqrt(bb);
+        converged = sqrt(rr64) <= o.tol*sqrt(bb);
         if (converged || outer == o.max_outer) break;
+        if (use_coarse) {
+            double zr1 = c == 1 ? r64 : 0.0, zr2 = c == 2 ? r64 : 0.0, z3 = 0;
+            gsum3(zr1, zr2, z3);
+            double cv[3];
+            for (int a = 0; a < 3; a++) cv[a] = s_ei[3*a]*zr0+s_ei[3*a+1]*zr1+s_ei[3*a+2]*zr2;
+            if (own) {
+                if (c < 3) x[gi] += cv[c];
+                r64 -= s_az[0][threadIdx.x]*cv[0]+s_az[1][threadIdx.x]*cv[1]+s_az[2][threadIdx.x]*cv[2];
+            }
+        }
         const double scale = 1.0/sqrt(rr64);
         float d = 0.f;
         int it = 0;
@@ -1839,6 +2303,12 @@ hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork&
     k_precond64<<<(A.n+255)/256, 256, 0, s>>>(A, w.pr, w.pz);
     (void)hipMemcpyAsync(w.pp, w.pz, m*sizeof(double), hipMemcpyDeviceToDevice, s);
     double bb = host_dot(b, b, m, w.partials, scal, s);
+    if (bb == 0) {                         // zero right-hand side: the solution is zero
+        launch_fill_double(x, 0.0, m, s);
+        if (iters_out) *iters_out = 0;
+        if (res_out) *res_out = 0;
+        return hipGetLastError();
+    }
     double rz = host_dot(w.pr, w.pz, m, w.partials, scal, s);
     double rr = host_dot(w.pr, w.pr, m, w.partials, scal, s);
     double stop = o.tol*o.tol*bb;

@@ -22,9 +22,9 @@ struct Params {
 // Derived constants shared by all kernels of one instance.
 struct Consts {
     double L[3], half[3];
-    double E[6], omega[3], vel_diff[3];
+    double Eh[6], Oh[3];                     // strain rate and vorticity at unit shear rate (System.cpp:2006-2015)
     double normal_coeff, tangential_coeff;   // dashpot (ContactDashpot.cpp:65-105)
-    double shear_rate, lub_max_gap, lub_reduce, kn, kt, mu_s, mu_d, sd_coeff;
+    double lub_max_gap, lub_reduce, kn, kt, mu_s, mu_d, sd_coeff;
     double repulsion, rep_len;
     int friction, repulsive, tangential;    // tangential lubrication model
 };
@@ -60,6 +60,20 @@ LFD_HD void symdot(const double* E, const double* v, double* o)
     double y = E[1]*v[0]+E[4]*v[1]+E[3]*v[2];
     double z = E[2]*v[0]+E[3]*v[1]+E[5]*v[2];
     o[0] = x; o[1] = y; o[2] = z;
+}
+
+// Imposed flow at the current shear rate: System::set_shear_rate with
+// setVelocityDifference (System.cpp:1967-1982).  Kept in device memory so that
+// stress control can change the rate between kernels.
+struct Flow { double rate, E[6], omega[3], vel_diff[3]; };
+LFD_HD void make_flow(const Consts& C, double rate, Flow& F)
+{
+    F.rate = rate;
+    for (int k = 0; k < 6; k++) F.E[k] = C.Eh[k]*rate;
+    for (int k = 0; k < 3; k++) F.omega[k] = C.Oh[k]*rate;
+    double z[3] = {0, 0, C.L[2]}, e[3];
+    symdot(F.E, z, e);
+    for (int k = 0; k < 3; k++) F.vel_diff[k] = 2*e[k];
 }
 
 // System::periodize (System.cpp:2478-2507)
@@ -169,6 +183,33 @@ LFD_HD void lub_constants(double a0, double a1, LubConst& c)
         c.YH[0] = a0a0a0_43*g; c.YH[1] = rororo_16*(8/lambda_p_1_cubic)*g5;
         c.YH[2] = rororo_16*(8*lambda_cubic/lambda_p_1_cubic)*g5i; c.YH[3] = a1a1a1_43*gi;
     }
+}
+
+// Stresslet resistance constants XM, YM of a pair (Lubrication.cpp:278-293,
+// LubricationFunctions.h); the XG, YG of LubConst enter the GU stresslet too.
+LFD_HD double g1_XM(double l) { double p = l+1; return (6./5)*l*l/(p*p*p); }
+LFD_HD double g4_XM(double l) { double p = l+1; return (6./5)*l*l*l/(p*p*p); }
+LFD_HD double g2_YM(double l) { double p = l+1; return (6./25)*l*(1-l+4*l*l)/(p*p*p); }
+LFD_HD double g5_YM(double l) { double p = l+1; return (3./50)*l*l*(7-10*l+7*l*l)/(p*p*p); }
+LFD_HD void lub_stress_constants(double a0, double a1, double* cXM, double* cYM)
+{
+    double ro = a0+a1;
+    double lambda = a1/a0;
+    double invlambda = 1/lambda;
+    double lambda_p_1 = 1+lambda;
+    double lambda_p_1_square = lambda_p_1*lambda_p_1;
+    double lambda_p_1_cubic = lambda_p_1_square*lambda_p_1;
+    double a0a0a0 = a0*a0*a0;
+    double a1a1a1 = a1*a1*a1;
+    double rororo = ro*ro*ro;
+    double a0a0a0_109 = a0a0a0*(10.0/9);
+    double a1a1a1_109 = a1a1a1*(10.0/9);
+    double rororo_536 = rororo*(5.0/36);
+    double g = g1_XM(lambda), gi = g1_XM(invlambda), g4 = g4_XM(lambda);
+    cXM[0] = a0a0a0_109*g; cXM[1] = rororo_536*(8/lambda_p_1_cubic)*g4; cXM[2] = cXM[1]; cXM[3] = a1a1a1_109*gi;
+    double g5 = g5_YM(lambda);
+    g = g2_YM(lambda); gi = g2_YM(invlambda);
+    cYM[0] = a0a0a0_109*g; cYM[1] = rororo_536*(8/lambda_p_1_cubic)*g5; cYM[2] = cYM[1]; cYM[3] = a1a1a1_109*gi;
 }
 
 // Dashpot resistances (ContactDashpot::calcDashpotResistances, ContactDashpot.cpp:107-183)
@@ -449,10 +490,10 @@ LFD_HD void pair_create(PairRec& q, int i, int j, const double* xi, const double
 
 // Interaction::updateState (Interaction.cpp:171-236); returns 1 if the pair leaves (r > range)
 LFD_HD int pair_update(PairRec& q, const double* x0, const double* x1, double a0, double a1,
-                       const double* v0, const double* v1, const Consts& C, const double* sd,
-                       double dt, int in_predictor)
+                       const double* v0, const double* v1, const Consts& C, const double* vel_diff,
+                       const double* sd, double dt, int in_predictor)
 {
-    if ((q.flags & F_CONTACT) && C.friction) contact_increment(q, v0, v1, a0, a1, C.vel_diff, dt, in_predictor);
+    if ((q.flags & F_CONTACT) && C.friction) contact_increment(q, v0, v1, a0, a1, vel_diff, dt, in_predictor);
     double ro = a0+a1;
     pair_geometry(q, x0, x1, ro, C.L, C.half, sd);
     double range = lub_range(a0, a1, C.lub_max_gap);

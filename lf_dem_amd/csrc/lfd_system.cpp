@@ -107,17 +107,13 @@ static Consts make_consts(const Params& P, const double L[3])
     double ct = std::cos(P.theta_shear), st = std::sin(P.theta_shear);
     double Eh[6] = {0, 0, ct/2, st/2, 0, 0};
     double Oh[3] = {-0.5*st, 0.5*ct, 0};
-    for (int k = 0; k < 6; k++) C.E[k] = Eh[k]*P.shear_rate;
-    for (int k = 0; k < 3; k++) C.omega[k] = Oh[k]*P.shear_rate;
-    double z[3] = {0, 0, L[2]}, e[3];
-    symdot(C.E, z, e);
-    for (int k = 0; k < 3; k++) C.vel_diff[k] = 2*e[k];
+    for (int k = 0; k < 6; k++) C.Eh[k] = Eh[k];
+    for (int k = 0; k < 3; k++) C.Oh[k] = Oh[k];
     C.normal_coeff = P.tau_n > 0 ? 4*P.kn*P.tau_n : 1/P.lub_reduce;
     int friction = P.friction_model != 0;
     if (!friction && P.lubrication_model != 2) C.tangential_coeff = 0;
     else if (P.tau_t > 0) C.tangential_coeff = 6*P.kt*P.tau_t;
     else C.tangential_coeff = P.lub_reduce < 1 ? std::log(1/P.lub_reduce) : 0;
-    C.shear_rate = P.shear_rate;
     C.lub_max_gap = P.lub_max_gap;
     C.lub_reduce = P.lub_reduce;
     C.kn = P.kn; C.kt = P.kt; C.mu_s = P.mu_static; C.mu_d = P.mu_dynamic; C.sd_coeff = P.sd_coeff;
@@ -264,6 +260,11 @@ struct lfd_system {
     int *keep = nullptr, *keep_scan = nullptr, *tcnt = nullptr, *merge_cnt = nullptr, *scan_tmp = nullptr;
     int* probe = nullptr;
     double *pdb = nullptr, *prhs = nullptr;
+    // velocity parts (contact, dashpot, hydro, repulsion / rate independent,
+    // rate proportional) with their right-hand sides, and the stress buffers
+    double *rhs_parts = nullptr, *vel_parts = nullptr, *pst = nullptr, *ps = nullptr, *st_partial = nullptr;
+    int control = LFD_RATE_CONTROL;
+    int have_stress = 0;
     DevScalars* sc = nullptr;
     Solver S;
     ~lfd_system();
@@ -274,7 +275,8 @@ lfd_system::~lfd_system()
     if (s) (void)hipStreamSynchronize(s);
     void* ptrs[] = {pos, vel6, na6, pred6, rhs, box_of, snap_start, snap_mem, home_start, home_mem, bin_count,
                     bin_cursor, found, found_cnt, pairs[0], pairs[1], keep, keep_scan, tcnt, merge_cnt,
-                    scan_tmp, sc, g.nb, g.nnb, g.nstatic, g.ctype, probe, pdb, prhs};
+                    scan_tmp, sc, g.nb, g.nnb, g.nstatic, g.ctype, probe, pdb, prhs, rhs_parts, vel_parts,
+                    pst, ps, st_partial};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     S.release();
     if (s) (void)hipStreamDestroy(s);
@@ -381,30 +383,59 @@ static int sys_update(lfd_system* y, int in_predictor, double dt_override)
     return LFD_OK;
 }
 
-static int sys_assemble(lfd_system* y)
+static int sys_assemble(lfd_system* y, int nparts = 0)
 {
     launch_assemble(y->pairs[y->cur], y->pos, y->n, y->C, y->sc, y->S.row_ptr, y->S.tptr, y->S.tidx, y->S.diag64,
                     y->S.diag32, y->S.od64, y->S.od32, y->S.minv64, y->S.minv32, y->rhs, y->pdb, y->prhs, y->pcap,
-                    y->sc, y->s);
+                    y->rhs_parts, nparts, y->sc, y->s);
     return LFD_OK;
 }
 
-static int sys_compute_velocities(lfd_system* y, int in_predictor)
+// calcStressPerParticle + calcStress (mode 1) or computeShearRate (mode 0)
+static void sys_stress(lfd_system* y, int nparts, int mode)
 {
-    // System::computeVelocities, rate control (System.cpp:2273-2319)
-    sys_assemble(y);
-    int rc = run_solve(y->S, &y->sc->npairs, y->rhs, y->na6, y->sc, y->s);
-    if (rc) return rc;
+    launch_stress(y->pairs[y->cur], y->pos, y->C, y->sc, y->vel_parts, nparts, y->vel6, y->n, y->control, mode,
+                  y->S.row_ptr, y->S.tptr, y->S.tidx, y->pst, y->ps, y->st_partial, y->pcap, y->s);
+}
+
+static int sys_compute_velocities(lfd_system* y, int in_predictor, int calc_stress = 0)
+{
+    // System::computeVelocities (System.cpp:2273-2319)
+    if (y->control == LFD_RATE_CONTROL && !calc_stress) {
+        sys_assemble(y);
+        int rc = run_solve(y->S, &y->sc->npairs, y->rhs, y->na6, y->sc, y->s);
+        if (rc) return rc;
+        launch_finish_velocity(y->n, y->pos, y->na6, y->vel6, y->C, y->sc, in_predictor, y->s);
+        return LFD_OK;
+    }
+    // velocities by parts (computeVelocityByComponents, System.cpp:1947-1965):
+    // four components when the stress is wanted, else (stress control) the
+    // rate independent and rate proportional sums
+    const int nparts = calc_stress ? 4 : 2;
+    const size_t m6 = 6*(size_t)y->n;
+    if (y->control == LFD_STRESS_CONTROL) launch_set_rate(y->sc, y->C, 1.0, y->s);
+    sys_assemble(y, nparts);
+    for (int p = 0; p < nparts; p++) {
+        int rc = run_solve(y->S, &y->sc->npairs, y->rhs_parts+p*m6, y->vel_parts+p*m6, y->sc, y->s);
+        if (rc) return rc;
+    }
+    if (y->control == LFD_STRESS_CONTROL) sys_stress(y, nparts, 0);     // new shear rate
+    launch_sum_parts(y->vel_parts, nparts, y->na6, y->n, y->sc, y->control == LFD_STRESS_CONTROL, y->s);
     launch_finish_velocity(y->n, y->pos, y->na6, y->vel6, y->C, y->sc, in_predictor, y->s);
+    if (calc_stress) {
+        sys_stress(y, nparts, 1);
+        y->have_stress = 1;
+    }
     return LFD_OK;
 }
 
-static int sys_step(lfd_system* y, double strain_end)
+static int sys_step(lfd_system* y, double strain_end, int calc_stress = 0, int adapt_first = 1)
 {
     // System::timeEvolution loop body + timeEvolutionPredictorCorrectorMethod
-    // (System.cpp:1241-1256, 888-983, 1124-1187)
+    // (System.cpp:1241-1256, 888-983, 1124-1187); the stress, when wanted, is
+    // the corrector's
     int rc;
-    launch_adapt_dt(y->sc, y->P, strain_end, 0, y->s);
+    if (adapt_first) launch_adapt_dt(y->sc, y->P, strain_end, 0, y->s);
     if ((rc = sys_compute_velocities(y, 1))) return rc;
     launch_adapt_dt(y->sc, y->P, strain_end, 1, y->s);
     launch_boxing(y->sc, y->C, y->s);
@@ -412,7 +443,7 @@ static int sys_step(lfd_system* y, double strain_end)
     launch_displace(y->pos, y->vel6, y->pred6, y->n, y->C, y->sc, y->g, y->box_of, 0, y->s);
     sys_update(y, 1, -1.0);
     launch_copy_double(y->pred6, y->vel6, 6*(size_t)y->n, y->s);
-    if ((rc = sys_compute_velocities(y, 0))) return rc;
+    if ((rc = sys_compute_velocities(y, 0, calc_stress))) return rc;
     launch_displace(y->pos, y->vel6, y->pred6, y->n, y->C, y->sc, y->g, y->box_of, 1, y->s);
     sys_check_new(y);
     sys_update(y, 0, -1.0);
@@ -471,6 +502,9 @@ extern "C" int lfd_system_create(int np, const double* pos, const double* radius
     A(dalloc(&y->g.nb, 64*(size_t)nbox)); A(dalloc(&y->g.nnb, nbox)); A(dalloc(&y->g.nstatic, nbox));
     A(dalloc(&y->g.ctype, nbox)); A(dalloc(&y->probe, 32*(size_t)nbox));
     A(dalloc(&y->pdb, 30*(size_t)pcap)); A(dalloc(&y->prhs, 42*(size_t)pcap));
+    A(dalloc(&y->rhs_parts, 4*m6)); A(dalloc(&y->vel_parts, 4*m6));
+    A(dalloc(&y->pst, 2*(size_t)STRESS_VALS*pcap)); A(dalloc(&y->ps, (size_t)STRESS_VALS*n));
+    A(dalloc(&y->st_partial, (size_t)STRESS_VALS*((n+STRESS_CHUNK-1)/STRESS_CHUNK)));
     A(y->S.alloc(n, pcap));
 #undef A
     if (e != hipSuccess) {
@@ -483,6 +517,9 @@ extern "C" int lfd_system_create(int np, const double* pos, const double* radius
     std::memset(&hs, 0, sizeof(hs));
     hs.dt = 1e-4;
     for (int k = 0; k < 3; k++) hs.shear_disp[k] = sd[k];
+    make_flow(y->C, y->P.shear_rate, hs.flow);
+    hs.init_strain_limit = 0;       // System.cpp:90-91
+    hs.init_rate_limit = 999;
     int rc = LFD_OK;
     auto up = [&](void* d, const void* h, size_t b) {
         if (e == hipSuccess) e = hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, y->s);
@@ -500,6 +537,7 @@ extern "C" int lfd_system_create(int np, const double* pos, const double* radius
     launch_fill_double(y->vel6, 0.0, m6, y->s);
     launch_fill_double(y->na6, 0.0, m6, y->s);
     launch_fill_double(y->pred6, 0.0, m6, y->s);
+    launch_fill_double(y->vel_parts, 0.0, 4*m6, y->s);
     launch_fill_int(y->S.row_ptr, 0, n+1, y->s);
     launch_box_labels(y->pos, n, y->g, y->box_of, y->sc, y->s);
     sys_bin_snapshot(y);
@@ -544,6 +582,110 @@ extern "C" int lfd_system_step(lfd_system* y, int nsteps, double strain_end)
         if (rc) return rc;
     }
     return check_scalars(y->sc, y->s);
+}
+
+extern "C" int lfd_system_set_control(lfd_system* y, int control, double target_stress)
+{
+    if (!y || (control != LFD_RATE_CONTROL && control != LFD_STRESS_CONTROL))
+        return fail(LFD_ERR_ARG, "lfd_system_set_control: bad argument");
+    y->control = control;
+    DevScalars h;
+    HIPCHK(hipMemcpyAsync(&h, y->sc, sizeof(h), hipMemcpyDeviceToHost, y->s));
+    HIPCHK(hipStreamSynchronize(y->s));
+    h.target_stress = target_stress;
+    HIPCHK(hipMemcpyAsync(y->sc, &h, sizeof(h), hipMemcpyHostToDevice, y->s));
+    HIPCHK(hipStreamSynchronize(y->s));
+    return LFD_OK;
+}
+
+extern "C" int lfd_system_set_shear_rate_limit(lfd_system* y, double init_strain_limit, double init_rate_limit)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    DevScalars h;
+    HIPCHK(hipMemcpyAsync(&h, y->sc, sizeof(h), hipMemcpyDeviceToHost, y->s));
+    HIPCHK(hipStreamSynchronize(y->s));
+    h.init_strain_limit = init_strain_limit;
+    h.init_rate_limit = init_rate_limit;
+    HIPCHK(hipMemcpyAsync(y->sc, &h, sizeof(h), hipMemcpyHostToDevice, y->s));
+    HIPCHK(hipStreamSynchronize(y->s));
+    return LFD_OK;
+}
+
+extern "C" int lfd_system_step_stress(lfd_system* y, double strain_end)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    int rc = sys_step(y, strain_end, 1, 0);
+    if (rc) return rc;
+    return check_scalars(y->sc, y->s);
+}
+
+extern "C" int lfd_system_time_evolution(lfd_system* y, double strain_end, int* nsteps)
+{
+    if (!y || !(strain_end >= 0)) return fail(LFD_ERR_ARG, "lfd_system_time_evolution: bad argument");
+    int n = 0;
+    for (;;) {
+        DevScalars h;
+        int rc = check_scalars(y->sc, y->s, &h);
+        if (rc) return rc;
+        if (h.strain > strain_end-1e-8) break;
+        if ((rc = sys_step(y, strain_end))) return rc;
+        n++;
+    }
+    int rc = sys_step(y, strain_end, 1, 0);
+    if (rc) return rc;
+    if (nsteps) *nsteps = n+1;
+    return check_scalars(y->sc, y->s);
+}
+
+extern "C" int lfd_system_compute_stress(lfd_system* y, int in_predictor)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    int rc = sys_compute_velocities(y, in_predictor, 1);
+    if (rc) return rc;
+    return check_scalars(y->sc, y->s);
+}
+
+extern "C" int lfd_system_get_stress(lfd_system* y, double* groups24, double* total6, double* shear_rate,
+                                     double* viscosity, double* particle6)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    if (!y->have_stress) return fail(LFD_ERR_ARG, "lfd_system_get_stress: no stress computed yet");
+    DevScalars h;
+    int rc = check_scalars(y->sc, y->s, &h);
+    if (rc) return rc;
+    if (groups24) std::memcpy(groups24, h.stress_groups, 24*sizeof(double));
+    if (total6) std::memcpy(total6, h.stress_total, 6*sizeof(double));
+    if (shear_rate) *shear_rate = h.flow.rate;
+    if (viscosity) {
+        // Simulation::outputData (Simulation.cpp:662-666)
+        const double* E = h.flow.E;
+        if (h.flow.rate == 0) E = y->C.Eh;
+        auto dd = [](const double* a, const double* b) {
+            return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5];
+        };
+        *viscosity = 0.5*dd(h.stress_total, E)/dd(E, E);
+    }
+    if (particle6) {
+        std::vector<double> ps((size_t)STRESS_VALS*y->n);
+        HIPCHK(hipMemcpyAsync(ps.data(), y->ps, ps.size()*sizeof(double), hipMemcpyDeviceToHost, y->s));
+        HIPCHK(hipStreamSynchronize(y->s));
+        for (int i = 0; i < y->n; i++)
+            for (int m = 0; m < 6; m++) {
+                double t = 0;
+                for (int sl = 0; sl < STRESS_VALS/6; sl++) t += ps[(size_t)STRESS_VALS*i+6*sl+m];
+                particle6[6*(size_t)i+m] = t;
+            }
+    }
+    return LFD_OK;
+}
+
+extern "C" int lfd_system_get_parts(lfd_system* y, int part, double* vel6)
+{
+    if (!y || part < 0 || part > 3 || !vel6) return fail(LFD_ERR_ARG, "lfd_system_get_parts: bad argument");
+    const size_t m6 = 6*(size_t)y->n;
+    HIPCHK(hipMemcpyAsync(vel6, y->vel_parts+part*m6, m6*sizeof(double), hipMemcpyDeviceToHost, y->s));
+    HIPCHK(hipStreamSynchronize(y->s));
+    return LFD_OK;
 }
 
 extern "C" int lfd_system_sync(lfd_system* y)

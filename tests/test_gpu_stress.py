@@ -1,0 +1,152 @@
+"""Stress path and stress control on the GPU (through the C ABI) against the
+oracle.
+
+Bars: velocity parts within 1e-8 relative of the oracle's FP64 direct solves;
+stress groups, total and per-particle stress within 1e-8 of the total stress
+norm (they are linear in those velocities); the stress-control shear rate
+within 1e-8 relative.  The oracle itself is pinned to the reference's
+stress-controlled golden row in test_oracle_stress.py.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import conf
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-8
+E_HAT = np.array([0, 0, 0.5, 0, 0, 0])
+
+
+def ddot(a, b):
+    return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5]
+
+
+def golden_params(mod):
+    # check/repulsive_stress.txt, repulsion units
+    return mod.default_params(kn=5000, kt=2500, sd_coeff=1e-8, tau_n=2e-3, tau_t=0, repulsion=1.0,
+                              repulsive_length=0.02, shear_rate=1.0)
+
+
+def cmp_stress(g, o):
+    scale = np.linalg.norm(o["total"])
+    assert np.linalg.norm(g["total"]-o["total"]) <= REL*scale
+    for k in o["groups"]:
+        assert np.linalg.norm(g["groups"][k]-o["groups"][k]) <= REL*scale, k
+    if "particle" in g:
+        ps = np.abs(o["particle"]).max()
+        assert np.abs(g["particle"]-o["particle"]).max() <= REL*ps
+
+
+def cmp_vel(S, R):
+    g, o = S.state(), R.state()
+    for k in ("na_vel", "na_ang_vel"):
+        assert np.linalg.norm(g[k]-o[k]) <= REL*np.linalg.norm(o[k]), k
+
+
+def test_stress_control_golden_row_on_gpu(golden, oracle, capi):
+    """t=0 of the reference's stress-controlled check (target 5r = 5/(6 pi))."""
+    args = (golden["dat_pos"], golden["dat_radius"], golden["dat_L"], golden["dat_disp"])
+    S = capi.System(*args, params=golden_params(capi))
+    R = oracle.System(*args, params=golden_params(oracle))
+    for X in (S, R):
+        X.set_control("stress", 5.0/(6*np.pi))
+        X.first_update()
+    S.compute_stress(True)
+    R.set_in_predictor(True)
+    R.compute_velocities_components()
+    R.calc_stress()
+    g, o = S.stress(particle=True), R.stress()
+    print("rate gpu", g["shear_rate"], "oracle", o["shear_rate"], flush=True)
+    assert g["shear_rate"] == pytest.approx(o["shear_rate"], rel=REL)
+    assert g["shear_rate"] == pytest.approx(1.08058, abs=5e-6)
+    assert 6*np.pi*g["viscosity"] == pytest.approx(4.62714, abs=5e-6)
+    cmp_stress(g, o)
+    cmp_vel(S, R)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    pos, rad, L = conf(500, 0.5, 1)
+    return pos, rad, L
+
+
+def test_rate_control_stress_matches_oracle(c1, oracle, capi):
+    """N=500 phi=0.5 at t=0 (812 contacts): parts, groups, per-particle stress."""
+    S = capi.System(*c1)
+    R = oracle.System(*c1)
+    S.first_update()
+    R.first_update()
+    S.compute_stress(False)
+    R.set_in_predictor(False)
+    R.compute_velocities_components()
+    R.calc_stress()
+    for k, name in enumerate(S.PARTS):
+        v = S.part(name)
+        ov, ow = R.components(name)
+        ref = np.hstack([ov, ow])
+        assert np.linalg.norm(v-ref) <= REL*max(np.linalg.norm(ref), 1e-300), name
+    g, o = S.stress(particle=True), R.stress()
+    assert np.any(o["groups"]["contact"] != 0)
+    cmp_stress(g, o)
+    assert g["viscosity"] == pytest.approx(o["viscosity"], rel=REL)
+    cmp_vel(S, R)
+
+
+def test_stress_control_matches_oracle(c1, oracle, capi):
+    """Stress control at twice the rate-1 shear stress: rate, stress, velocities."""
+    S = capi.System(*c1)
+    R = oracle.System(*c1)
+    for X in (S, R):
+        X.first_update()
+    R.set_in_predictor(False)
+    R.compute_velocities_components()
+    R.calc_stress()
+    target = 2*ddot(R.stress()["total"], E_HAT)
+    for X in (S, R):
+        X.set_control("stress", target)
+    S.compute_stress(False)
+    R.compute_velocities_components()
+    R.calc_stress()
+    g, o = S.stress(particle=True), R.stress()
+    assert g["shear_rate"] == pytest.approx(o["shear_rate"], rel=REL)
+    assert ddot(g["total"], E_HAT) == pytest.approx(target, rel=REL)
+    cmp_stress(g, o)
+    cmp_vel(S, R)
+
+
+def test_stress_control_trajectory(c1, oracle, capi):
+    """Five stress-controlled predictor-corrector steps: positions, strain,
+    shear rate (two velocity parts per solve on the GPU, four in the oracle)."""
+    S = capi.System(*c1)
+    R = oracle.System(*c1)
+    target = 0.05
+    for X in (S, R):
+        X.set_control("stress", target)
+        X.first_update()
+    for _ in range(5):
+        S.step(1)
+        R.step()
+    g, o = S.state(), R.state()
+    assert g["strain"] == pytest.approx(o["strain"], rel=1e-8)
+    assert g["dt"] == pytest.approx(o["dt"], rel=1e-7)
+    d = g["pos"]-o["pos"]
+    d -= np.round(d/np.asarray(c1[2]))*np.asarray(c1[2])
+    assert np.abs(d).max() <= 1e-8
+    assert S.npairs == R.npairs
+
+
+def test_time_evolution_with_final_stress(c1, oracle, capi):
+    """System::timeEvolution to a strain end: step count, landing strain and
+    the final step's stress (rate control)."""
+    S = capi.System(*c1)
+    R = oracle.System(*c1)
+    for X in (S, R):
+        X.first_update()
+    strain_end = 0.004
+    ns = S.time_evolution(strain_end)
+    no = R.time_evolution(strain_end)
+    assert ns == no
+    g, o = S.stress(), R.stress()
+    cmp_stress(g, o)
+    assert S.state()["strain"] == pytest.approx(R.state()["strain"], rel=1e-10)
