@@ -110,6 +110,56 @@ def run(args):
     print(json.dumps(out), flush=True)
 
 
+def run_c5(args):
+    """Stress-controlled near-jamming workload: N=32000 bidisperse, phi=0.58,
+    mu=1, fixed shear stress; every velocity computation solves the rate
+    independent and rate proportional right-hand sides on one preconditioner
+    and sets the shear rate on the device.  The target stress is the shear
+    stress of the initial configuration at unit shear rate, so the run starts
+    at a rate near 1."""
+    from lf_dem_amd import capi, confgen
+    if capi.lib().lfd_device_count() < 1:
+        raise SystemExit("no GPU visible")
+    n, phi = 32000, 0.58
+    t0 = time.time()
+    pos, rad, L = confgen.generate(n, phi, args.seed)
+    gen_s = time.time() - t0
+    print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
+    params = capi.default_params()
+    opts = capi.default_opts(kind=0, tol=args.tol)
+    S = capi.System(pos, rad, L, params=params, opts=opts)
+    S.first_update()
+    S.compute_stress(False)
+    st = S.stress()
+    target = st["total"][2]*2*0.5          # S:E_hat with E_hat = (0, 0, 1/2, 0, 0, 0)
+    S.set_control("stress", target)
+    S.step(args.warmup)
+    st0 = S.solve_stats()
+    S.profile(True)
+    t0 = time.perf_counter()
+    S.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    solve_ms, launches = S.profile(False)
+    st1 = S.solve_stats()
+    S.step_stress()
+    sr = S.stress()
+    sps = args.steps/elapsed
+    out = {
+        "metric": "steps/s at N=32000 phi=0.58 stress control (2 RHS per solve)", "value": round(sps, 3),
+        "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3/sps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "N=32000 bidisperse 1.0/1.4 phi=0.58 stress-controlled shear", "N": n, "phi": phi,
+                   "seed": args.seed, "pairs": S.npairs, "target_stress": target, "tol": args.tol,
+                   "parallelism": "single GPU"},
+        "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],
+        "solver": {"solve_launches": launches, "avg_solve_ms": round(solve_ms/max(launches, 1), 4),
+                   "inner_iters_per_solve": round((st1["total_inner"]-st0["total_inner"])/max(launches, 1), 1)},
+        "confgen_s": round(gen_s, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_leg(pos, rad, L, params, opts, args):
     """FP64 direct (oracle) solve at t=0: error of the GPU velocities and the
     host-core time of one assembly + sparse Cholesky factorisation + solve."""
@@ -145,10 +195,15 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("c2", "c5"), default="c2",
+                    help="c2: N=4000 rate control (headline); c5: N=32000 stress control")
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)")
-    run(args)
+    if args.workload == "c5":
+        run_c5(args)
+    else:
+        run(args)
 
 
 if __name__ == "__main__":
