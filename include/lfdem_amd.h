@@ -181,6 +181,48 @@ int lfd_system_get_stress(lfd_system *s, double *groups24, double *total6, doubl
  * already scaled by the shear rate under stress control. */
 int lfd_system_get_parts(lfd_system *s, int part, double *vel6);
 
+/* ------------------------------------------------------------------ */
+/* Batches of independent instances (sweeps and served simulations)    */
+/* ------------------------------------------------------------------ */
+/* No LF_DEM counterpart: LF_DEM runs one System per process.  A batch steps
+ * many independent Systems (each with its own N, box, parameters, shear
+ * displacement and pair list) with one set of kernels per phase and one
+ * cooperative solver launch per 256 workgroups; lfd_system_* is a batch of
+ * one.  Every instance's trajectory is the one it has alone: instances share
+ * no arithmetic.  Capacities are fixed at creation: particle slots (an
+ * instance takes N rounded up to 128), cells and instance handles. */
+typedef struct lfd_batch lfd_batch;
+int lfd_batch_create(int slot_cap, int cell_cap, int inst_cap, int control, const lfd_solve_opts *o,
+                     lfd_batch **out);
+void lfd_batch_destroy(lfd_batch *b);
+/* Slots, cells and device bytes an instance needs (admission by HBM
+ * footprint); host-only, no device call. */
+int lfd_batch_footprint(int np, const double *radius, const double L[3], const lfd_params *p,
+                        int *slots, int *cells, long long *bytes);
+/* 1 when an instance of that many slots and cells can be admitted now, else 0 */
+int lfd_batch_room(lfd_batch *b, int slots, int cells);
+/* Admission: System::setupSystem + the first-call block of timeEvolution
+ * (System.cpp:468-611, 1219-1228) for the new instance only, between steps.
+ * strain_end < 0: no end-of-interval clamp; target_stress is used under
+ * stress control.  LFD_ERR_CAPACITY (nothing changed) when it does not fit. */
+int lfd_batch_add(lfd_batch *b, int np, const double *pos, const double *radius, const double L[3],
+                  const double shear_disp[3], const lfd_params *p, double strain_end, double target_stress,
+                  int *id);
+/* Retirement: frees the instance's slots, cells and handle. */
+int lfd_batch_remove(lfd_batch *b, int id);
+/* nsteps predictor-corrector steps of every live instance (enqueued). */
+int lfd_batch_step(lfd_batch *b, int nsteps);
+int lfd_batch_sync(lfd_batch *b);
+/* scalars8 as lfd_system_get_state; status: the instance's own lfd_status */
+int lfd_batch_query(lfd_batch *b, int id, double *scalars8, lfd_solve_stats *st, int *status);
+int lfd_batch_get_state(lfd_batch *b, int id, double *pos, double *vel6, double *na6, double *scalars8);
+int lfd_batch_get_pairs(lfd_batch *b, int id, int cap, int *p0, int *p1, int *state, double *geom5,
+                        double *disp_tan);
+int lfd_batch_num_pairs(lfd_batch *b, int id);
+int lfd_batch_info(lfd_batch *b, int *live, int *slot_cap, int *cell_cap, int *inst_cap, int *solver_launches,
+                   long long *bytes);
+int lfd_batch_profile(lfd_batch *b, int enable, double *solve_ms_total, int *solve_launches);
+
 #ifdef __cplusplus
 }
 #endif

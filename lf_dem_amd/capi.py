@@ -71,6 +71,23 @@ SIGNATURES = {
     "lfd_system_compute_stress": ([_vp, ctypes.c_int], ctypes.c_int),
     "lfd_system_get_stress": ([_vp, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
     "lfd_system_get_parts": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_batch_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(_vp)],
+                         ctypes.c_int),
+    "lfd_batch_destroy": ([_vp], None),
+    "lfd_batch_footprint": ([ctypes.c_int, _dp, _dp, _vp, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)],
+                            ctypes.c_int),
+    "lfd_batch_room": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "lfd_batch_add": ([_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _vp, ctypes.c_double, ctypes.c_double, _ip],
+                      ctypes.c_int),
+    "lfd_batch_remove": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_batch_step": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_batch_sync": ([_vp], ctypes.c_int),
+    "lfd_batch_query": ([_vp, ctypes.c_int, _dp, _vp, _ip], ctypes.c_int),
+    "lfd_batch_get_state": ([_vp, ctypes.c_int, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "lfd_batch_get_pairs": ([_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
+    "lfd_batch_num_pairs": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_batch_info": ([_vp, _ip, _ip, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "lfd_batch_profile": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
 }
 
 
@@ -374,3 +391,108 @@ class System:
         _check(lib().lfd_system_get_solve_stats(self.h, ctypes.byref(st), ctypes.byref(tot)))
         return dict(iters_inner=st.iters_inner, iters_outer=st.iters_outer, res_norm=st.res_norm,
                     rhs_norm=st.rhs_norm, total_inner=tot.value)
+
+
+def _state_dict(pos, vel6, na6, sc):
+    return dict(pos=pos, vel=vel6[:, :3].copy(), ang_vel=vel6[:, 3:].copy(), na_vel=na6[:, :3].copy(),
+                na_ang_vel=na6[:, 3:].copy(), shear_disp=sc[:3].copy(), dt=sc[3], time=sc[4],
+                strain=sc[5], max_velocity=sc[6], max_sliding_velocity=sc[7])
+
+
+def footprint(pos_or_np, radius, L, params=None):
+    """(slots, cells, device bytes) an instance needs in a Batch."""
+    rad = np.ascontiguousarray(radius, dtype=np.float64)
+    Lv = np.ascontiguousarray(L, dtype=np.float64)
+    p = params if params is not None else default_params()
+    sl = ctypes.c_int(0)
+    ce = ctypes.c_int(0)
+    by = ctypes.c_longlong(0)
+    _check(lib().lfd_batch_footprint(len(rad), _d(rad), _d(Lv), ctypes.byref(p), ctypes.byref(sl),
+                                     ctypes.byref(ce), ctypes.byref(by)))
+    return sl.value, ce.value, by.value
+
+
+class Batch:
+    """Many independent Systems stepped together (lfd_batch_* of include/lfdem_amd.h)."""
+
+    def __init__(self, slot_cap, cell_cap, inst_cap, control="rate", opts=None):
+        self.opts = opts if opts is not None else default_opts()
+        self.control = control
+        h = _vp()
+        _check(lib().lfd_batch_create(int(slot_cap), int(cell_cap), int(inst_cap), {"rate": 0, "stress": 1}[control],
+                                      ctypes.byref(self.opts), ctypes.byref(h)))
+        self.h = h
+        self.np = {}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lfd_batch_destroy(self.h)
+            self.h = None
+
+    def room(self, slots, cells):
+        return bool(_check(lib().lfd_batch_room(self.h, int(slots), int(cells))))
+
+    def add(self, pos, radius, L, shear_disp=(0.0, 0.0, 0.0), params=None, strain_end=-1.0, target_stress=0.0):
+        """Admit one instance; returns its id (LfdError with LFD_ERR_CAPACITY when it does not fit)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        rad = np.ascontiguousarray(radius, dtype=np.float64)
+        Lv = np.ascontiguousarray(L, dtype=np.float64)
+        sd = np.ascontiguousarray(shear_disp, dtype=np.float64)
+        p = params if params is not None else default_params()
+        i = ctypes.c_int(-1)
+        _check(lib().lfd_batch_add(self.h, len(rad), _d(pos), _d(rad), _d(Lv), _d(sd), ctypes.byref(p),
+                                   float(strain_end), float(target_stress), ctypes.byref(i)))
+        self.np[i.value] = len(rad)
+        return i.value
+
+    def remove(self, iid):
+        _check(lib().lfd_batch_remove(self.h, int(iid)))
+        self.np.pop(iid, None)
+
+    def step(self, nsteps=1):
+        _check(lib().lfd_batch_step(self.h, int(nsteps)))
+
+    def sync(self):
+        _check(lib().lfd_batch_sync(self.h))
+
+    def query(self, iid):
+        sc = np.zeros(8)
+        st = SolveStats()
+        status = ctypes.c_int(0)
+        _check(lib().lfd_batch_query(self.h, int(iid), _d(sc), ctypes.byref(st), ctypes.byref(status)))
+        return dict(shear_disp=sc[:3].copy(), dt=sc[3], time=sc[4], strain=sc[5], status=status.value,
+                    iters_inner=st.iters_inner, iters_outer=st.iters_outer)
+
+    def state(self, iid):
+        n = self.np[iid]
+        pos = np.zeros((n, 3))
+        vel6 = np.zeros((n, 6))
+        na6 = np.zeros((n, 6))
+        sc = np.zeros(8)
+        _check(lib().lfd_batch_get_state(self.h, int(iid), _d(pos), _d(vel6), _d(na6), _d(sc)))
+        return _state_dict(pos, vel6, na6, sc)
+
+    def pairs(self, iid):
+        n = _check(lib().lfd_batch_num_pairs(self.h, int(iid)))
+        cap = max(n, 1)
+        p0 = np.zeros(cap, dtype=np.int32)
+        p1 = np.zeros(cap, dtype=np.int32)
+        st = np.zeros(cap, dtype=np.int32)
+        geom = np.zeros((cap, 5))
+        dt = np.zeros((cap, 3))
+        n = _check(lib().lfd_batch_get_pairs(self.h, int(iid), cap, _i(p0), _i(p1), _i(st), _d(geom), _d(dt)))
+        return dict(p0=p0[:n], p1=p1[:n], state=st[:n], nvec=geom[:n, :3], r=geom[:n, 3],
+                    gap=geom[:n, 4], disp_tan=dt[:n])
+
+    def info(self):
+        v = [ctypes.c_int(0) for _ in range(5)]
+        by = ctypes.c_longlong(0)
+        _check(lib().lfd_batch_info(self.h, *[ctypes.byref(x) for x in v], ctypes.byref(by)))
+        return dict(live=v[0].value, slot_cap=v[1].value, cell_cap=v[2].value, inst_cap=v[3].value,
+                    solver_launches=v[4].value, bytes=by.value)
+
+    def profile(self, enable):
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int(0)
+        _check(lib().lfd_batch_profile(self.h, int(enable), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -26,7 +26,8 @@ struct DevScalars {
     unsigned long long max_vel_bits;    // max |U_na| (bit pattern of a non-negative double)
     unsigned long long max_slide_bits;  // max sliding velocity squared
     double res_norm, rhs_norm;          // last solve: ||b - A x||, ||b||
-    int npairs, npairs_tmp;
+    double strain_end;                  // the time step is clipped to reach it (< 0: no limit)
+    int npairs;                         // StokesSolver staging: block count
     int error;
     int in_predictor;
     int iters_inner, iters_outer;       // last solve
@@ -38,6 +39,46 @@ struct DevScalars {
     // last stress computation: groups contact, dashpot, hydro, repulsion
     // (already / volume, hydro with the solvent 2E/(6 pi)) and their total
     double stress_groups[4][6], stress_total[6];
+};
+
+// Counters of the whole batch.
+struct BatchScalars {
+    int npairs;       // pairs of all instances (one list sorted by (p0, p1))
+    int error;        // errors not owned by one instance (pair capacity)
+    int pad[2];
+};
+
+// Everything constant about one instance between admissions: its physics,
+// cell grid and where its particles and cells sit in the batch arrays.
+// Particles of instance k occupy slots [poff, poff + n) (poff a multiple of
+// 128), its cells [coff, coff + nbox).
+struct InstConst {
+    Consts C;
+    Params P;
+    double bx, by, bz;
+    int nx, ny, nz, yz, nbox, boxed;
+    int coff, poff, n, pad;
+    double probe_top[16][3], probe_bot[16][3];
+};
+
+// Cell arrays of the batch (global cell index; -1 in cell_inst for a free cell).
+struct CellArrays {
+    int ncell;     // cells in use (grid bound)
+    int* nb;       // [ncell*64] neighbour cells incl. self, deduplicated, global index
+    int* nnb;      // [ncell]
+    int* nstatic;  // [ncell] count of self+static entries (moving ones follow)
+    int* ctype;    // [ncell] 0 bulk, 1 bottom, 2 top, 3 top-bottom
+    int* cell_inst;
+};
+
+// Work item of one workgroup of the batched LDS solver.
+struct SolveJob {
+    int inst;      // instance (index into the DevScalars array)
+    int lwg;       // workgroup index within the instance
+    int G;         // workgroups of the instance
+    int row0;      // first slot of the instance
+    int row_end;   // one past its last particle
+    int pad[3];
 };
 
 // 6x6-block sparse symmetric matrix.  Off-diagonal blocks are stored once per
@@ -87,64 +128,66 @@ void launch_fill_double(double* p, double v, size_t n, hipStream_t s);
 void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, int* tmp, hipStream_t s,
                         int* out2 = nullptr);
 
-// cell list
-struct CellGrid {
-    int nbox, nx, ny, nz, yz, boxed;
-    double bx, by, bz;
-    int* nb;       // [nbox*64] neighbour cells incl. self, deduplicated
-    int* nnb;      // [nbox]
-    int* nstatic;  // [nbox] count of self+static entries (moving ones follow)
-    int* ctype;    // [nbox] 0 bulk, 1 bottom, 2 top, 3 top-bottom
-    double probe_top[16][3], probe_bot[16][3];
-};
-void launch_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc, hipStream_t s);
+// cell list and pair discovery; `only` >= 0 restricts the search to one instance
+void launch_box_labels(const double4* pos, const int* inst_of, int n, const InstConst* IC, int* box_of,
+                       DevScalars* sc, hipStream_t s);
 void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int* mem, int* cursor,
                 int* scan_tmp, hipStream_t s);
-void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, int* probe, DevScalars* err,
-                             hipStream_t s);
-void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_start, const int* home_mem,
-                       const int* snap_start, const int* snap_mem, const DevScalars* sc, Consts C,
-                       int* found, int* found_cnt, int K, DevScalars* err, hipStream_t s);
+void launch_moving_neighbors(CellArrays ca, const InstConst* IC, const DevScalars* sc, int* probe, hipStream_t s);
+void launch_find_pairs(const double4* pos, int n, CellArrays ca, const InstConst* IC, const int* home_start,
+                       const int* home_mem, const int* snap_start, const int* snap_mem, DevScalars* sc,
+                       int* found, int* found_cnt, int K, int only, hipStream_t s);
 void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_p1, const int* old_row_ptr, const int* found,
                         const int* found_cnt, int K, int* cnt, int* new_row_ptr, int* src, PairRec* new_pairs,
-                        const double4* pos, Consts C, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
-void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, Consts C, DevScalars* sc,
-                         int* keep, int pcap, int in_predictor, double dt_override, hipStream_t s);
+                        const double4* pos, const int* inst_of, const InstConst* IC, const DevScalars* sc,
+                        BatchScalars* bs, int pcap, int* scan_tmp, hipStream_t s);
+// only >= 0: other instances' pairs are kept unchanged; only == -2: drop the
+// pairs of free slots and change nothing else
+void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, const int* inst_of,
+                         const InstConst* IC, DevScalars* sc, int ninst, const BatchScalars* bs, int* keep,
+                         int pcap, int in_predictor, double dt_override, int only, hipStream_t s);
 void launch_compact_pairs(const PairRec* in, PairRec* out, const int* keep, int* keep_scan, int n,
-                          int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, DevScalars* sc,
+                          int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, BatchScalars* bs,
                           int pcap, int* scan_tmp, hipStream_t s);
 void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
-                       int* tcnt, int* tmp, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s);
-void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
-                     const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
-                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, double* pdb,
-                     double* prhs, int pcap, double* rhs_parts, int nparts, DevScalars* err, hipStream_t s);
+                       int* tcnt, int* tmp, const BatchScalars* bs, int pcap, int* scan_tmp, hipStream_t s);
+void launch_assemble(const PairRec* pairs, const double4* pos, int n, const int* inst_of, const InstConst* IC,
+                     DevScalars* sc, const BatchScalars* bs, const int* row_ptr, const int* tptr, const int* tidx,
+                     double* diag64, float* diag32, double* od64, float* od32, double* minv64, float* minv32,
+                     double* rhs, double* pdb, double* prhs, int pcap, double* rhs_parts, int nparts, hipStream_t s);
 // stress (mode 0: stress-control shear rate, mode 1: output groups) and velocity parts
 static constexpr int STRESS_VALS = 48;      // 8 slots x 6 per particle / pair side
 static constexpr int STRESS_CHUNK = 128;
-void launch_stress(const PairRec* pairs, const double4* pos, Consts C, DevScalars* sc, const double* parts,
-                   int nparts, const double* vel6, int n, int control, int mode, const int* row_ptr,
-                   const int* tptr, const int* tidx, double* pst, double* ps, double* partial, int pcap,
-                   hipStream_t s);
-void launch_sum_parts(double* parts, int nparts, double* na6, int n, const DevScalars* sc, int rescale,
-                      hipStream_t s);
-void launch_set_rate(DevScalars* sc, Consts C, double rate, hipStream_t s);
+void launch_stress(const PairRec* pairs, const double4* pos, const int* inst_of, const InstConst* IC, int ninst,
+                   DevScalars* sc, const BatchScalars* bs, const double* parts, int nparts, const double* vel6,
+                   int n, int control, int mode, const int* row_ptr, const int* tptr, const int* tidx, double* pst,
+                   double* ps, double* partial, int pcap, hipStream_t s);
+void launch_sum_parts(double* parts, int nparts, double* na6, int n, const int* inst_of, const DevScalars* sc,
+                      int rescale, hipStream_t s);
+void launch_set_rate(DevScalars* sc, const InstConst* IC, int ninst, double rate, hipStream_t s);
 void launch_prepare_matrix(int n, const int* npairs, const double* diag64, float* diag32, const double* od64,
                            float* od32, double* minv64, float* minv32, int pcap, DevScalars* err, hipStream_t s);
-void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
-                            DevScalars* sc, int in_predictor, hipStream_t s);
-void launch_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time, hipStream_t s);
-void launch_boxing(DevScalars* sc, Consts C, hipStream_t s);
-void launch_displace(double4* pos, double* vel6, const double* pred6, int n, Consts C, DevScalars* sc,
-                     CellGrid g, int* box_of, int mode, hipStream_t s);
+void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, const int* inst_of,
+                            DevScalars* sc, int ninst, int in_predictor, hipStream_t s);
+void launch_adapt_dt(DevScalars* sc, const InstConst* IC, int ninst, int advance_time, hipStream_t s);
+void launch_boxing(DevScalars* sc, const InstConst* IC, int ninst, hipStream_t s);
+void launch_set_strain_end(DevScalars* sc, int ninst, double strain_end, hipStream_t s);
+void launch_displace(double4* pos, double* vel6, const double* pred6, int n, const int* inst_of,
+                     const InstConst* IC, DevScalars* sc, int* box_of, int mode, hipStream_t s);
 void launch_copy_double(double* dst, const double* src, size_t n, hipStream_t s);
 
 // solver
 int persistent_groups(const SolveOpts& o);
 hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s);
+// Batched LDS-resident solver: one cooperative launch of nwg workgroups
+// described by jobs[0..nwg) (all instances of the launch).  Partial records
+// and barrier words of an instance sit at 6 and 128 per row block from its
+// first row block (row0 / 128).
+static constexpr int LDS_ROWS = 128;
 hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
-                     const SolveOpts& o, hipStream_t s);
+                     const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
+int lds_solver_max_groups();
 hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
 void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);

@@ -168,30 +168,36 @@ void exclusive_scan_int(const int* in, int* out, const int* count_dev, int cap, 
 }
 
 // ------------------------------------------------------------------ cells
-__global__ void k_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc)
+// Global cell label of every particle slot (-1 for a free slot).
+__global__ void k_box_labels(const double4* pos, const int* inst_of, int n, const InstConst* IC, int* box_of,
+                             DevScalars* sc)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
+    const int k = inst_of[i];
+    if (k < 0) { box_of[i] = -1; return; }
+    const InstConst& ic = IC[k];
     double4 q = pos[i];
     double p[3] = {q.x, q.y, q.z};
-    int l = which_box(p, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
-    if (l < 0) { raise_error(sc, E_BOX_OOB); l = 0; }
-    box_of[i] = l;
+    int l = which_box(p, ic.bx, ic.by, ic.bz, (unsigned)ic.yz, (unsigned)ic.nz, (unsigned)ic.nbox);
+    if (l < 0) { raise_error(sc+k, E_BOX_OOB); l = 0; }
+    box_of[i] = ic.coff+l;
 }
-void launch_box_labels(const double4* pos, int n, CellGrid g, int* box_of, DevScalars* sc, hipStream_t s)
+void launch_box_labels(const double4* pos, const int* inst_of, int n, const InstConst* IC, int* box_of,
+                       DevScalars* sc, hipStream_t s)
 {
-    k_box_labels<<<(n+255)/256, 256, 0, s>>>(pos, n, g, box_of, sc);
+    k_box_labels<<<(n+255)/256, 256, 0, s>>>(pos, inst_of, n, IC, box_of, sc);
 }
 
 __global__ void k_bin_count(const int* box_of, int n, int* count)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i < n) atomicAdd(&count[box_of[i]], 1);
+    if (i < n && box_of[i] >= 0) atomicAdd(&count[box_of[i]], 1);
 }
 __global__ void k_bin_scatter(const int* box_of, int n, int* cursor, int* mem)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i < n) mem[atomicAdd(&cursor[box_of[i]], 1)] = i;
+    if (i < n && box_of[i] >= 0) mem[atomicAdd(&cursor[box_of[i]], 1)] = i;
 }
 // members of a cell in ascending particle order (the order of Box's std::set<int>)
 __global__ void k_bin_sort(const int* start, int nbox, int* mem)
@@ -224,40 +230,47 @@ void launch_bin(const int* box_of, int n, int nbox, int* count, int* start, int*
 // bottom cells from the 16 probing positions, with the current LE shift.
 // One thread per (cell, probe) computes the probe's cell label; a second pass
 // appends the new labels to each cell's list (top probes first).
-__global__ void k_probe_labels(CellGrid g, const DevScalars* sc, Consts C, int* probe, DevScalars* err)
+__global__ void k_probe_labels(CellArrays ca, const InstConst* IC, const DevScalars* sc, int* probe)
 {
     int t = blockIdx.x*blockDim.x+threadIdx.x;
     int b = t >> 5, k = t & 31;
-    if (b >= g.nbox) return;
-    int ty = g.ctype[b];
-    bool top = k < 16;
-    bool want = g.boxed && ((top && (ty == 2 || ty == 3)) || (!top && (ty == 1 || ty == 3)));
+    if (b >= ca.ncell) return;
+    const int inst = ca.cell_inst[b];
     int l = -1;
-    if (want) {
-        unsigned ix = b/g.yz, iy = (b%g.yz)/g.nz, iz = b%g.nz;
-        double c[3] = {g.bx*(ix+0.5), g.by*(iy+0.5), g.bz*(iz+0.5)};
-        double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
-        const double* pr = top ? g.probe_top[k] : g.probe_bot[k-16];
-        double q[3] = {c[0]+pr[0], c[1]+pr[1], c[2]+pr[2]};
-        periodize(C.L, sd, q);
-        l = which_box(q, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
-        if (l < 0) raise_error(err, E_BOX_OOB);
-        if (l == b) l = -1;
+    if (inst >= 0) {
+        const InstConst& ic = IC[inst];
+        const int lb = b-ic.coff, ty = ca.ctype[b];
+        const bool top = k < 16;
+        const bool want = ic.boxed && ((top && (ty == 2 || ty == 3)) || (!top && (ty == 1 || ty == 3)));
+        if (want) {
+            unsigned ix = lb/ic.yz, iy = (lb%ic.yz)/ic.nz, iz = lb%ic.nz;
+            double c[3] = {ic.bx*(ix+0.5), ic.by*(iy+0.5), ic.bz*(iz+0.5)};
+            double sd[3] = {sc[inst].shear_disp[0], sc[inst].shear_disp[1], sc[inst].shear_disp[2]};
+            const double* pr = top ? ic.probe_top[k] : ic.probe_bot[k-16];
+            double q[3] = {c[0]+pr[0], c[1]+pr[1], c[2]+pr[2]};
+            periodize(ic.C.L, sd, q);
+            l = which_box(q, ic.bx, ic.by, ic.bz, (unsigned)ic.yz, (unsigned)ic.nz, (unsigned)ic.nbox);
+            if (l < 0) raise_error(const_cast<DevScalars*>(sc)+inst, E_BOX_OOB);
+            else if (l+ic.coff == b) l = -1;
+            else l += ic.coff;
+        }
     }
     probe[t] = l;
 }
 // One wave per cell: the static neighbours sit one per lane, the 32 probe
 // labels are appended in probe order when no lane already holds them.
-__global__ __launch_bounds__(256) void k_moving_merge(CellGrid g, const int* probe)
+__global__ __launch_bounds__(256) void k_moving_merge(CellArrays ca, const InstConst* IC, const int* probe)
 {
     const int b = blockIdx.x*(256/WAVE)+(int)(threadIdx.x/WAVE);
     const int lane = threadIdx.x & (WAVE-1);
-    if (b >= g.nbox) return;
-    int cnt = g.nstatic[b];
-    int* nb = g.nb+64*b;
-    if (g.ctype[b] != 0 && g.boxed) {
+    if (b >= ca.ncell) return;
+    const int inst = ca.cell_inst[b];
+    if (inst < 0) return;
+    int cnt = ca.nstatic[b];
+    int* nb = ca.nb+64*(size_t)b;
+    if (ca.ctype[b] != 0 && IC[inst].boxed) {
         int mine = lane < cnt ? nb[lane] : -1;
-        const int pl = lane < 32 ? probe[32*b+lane] : -1;
+        const int pl = lane < 32 ? probe[32*(size_t)b+lane] : -1;
         for (int k = 0; k < 32; k++) {
             const int l = __shfl(pl, k);
             if (l < 0) continue;
@@ -269,15 +282,13 @@ __global__ __launch_bounds__(256) void k_moving_merge(CellGrid g, const int* pro
         }
         if (lane < cnt) nb[lane] = mine;
     }
-    if (lane == 0) g.nnb[b] = cnt;
+    if (lane == 0) ca.nnb[b] = cnt;
 }
-void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, int* probe, DevScalars* err,
-                             hipStream_t s)
+void launch_moving_neighbors(CellArrays ca, const InstConst* IC, const DevScalars* sc, int* probe, hipStream_t s)
 {
-    Consts C{};
-    C.L[0] = L[0]; C.L[1] = L[1]; C.L[2] = L[2];
-    k_probe_labels<<<(32*g.nbox+255)/256, 256, 0, s>>>(g, sc, C, probe, err);
-    k_moving_merge<<<(g.nbox+3)/4, 256, 0, s>>>(g, probe);
+    if (ca.ncell == 0) return;
+    k_probe_labels<<<(unsigned)((32*(size_t)ca.ncell+255)/256), 256, 0, s>>>(ca, IC, sc, probe);
+    k_moving_merge<<<(ca.ncell+3)/4, 256, 0, s>>>(ca, IC, probe);
 }
 
 // ------------------------------------------------------------------ neighbour search
@@ -288,10 +299,10 @@ void launch_moving_neighbors(CellGrid g, const DevScalars* sc, const double* L, 
 // System.cpp:1328-1353, over BoxSet::neighborhood, Box.cpp:45-87).
 static constexpr int FP_BLOCK = 256, FP_CHUNK = 512;
 
-__global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restrict__ pos, CellGrid g,
-    const int* __restrict__ home_start, const int* __restrict__ home_mem, const int* __restrict__ snap_start,
-    const int* __restrict__ snap_mem, const DevScalars* sc, Consts C, int* __restrict__ found,
-    int* __restrict__ found_cnt, int K, DevScalars* err)
+__global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restrict__ pos, CellArrays ca,
+    const InstConst* IC, const int* __restrict__ home_start, const int* __restrict__ home_mem,
+    const int* __restrict__ snap_start, const int* __restrict__ snap_mem, DevScalars* sc,
+    int* __restrict__ found, int* __restrict__ found_cnt, int K, int only)
 {
     __shared__ int cells[64];
     __shared__ int coff[65];
@@ -301,11 +312,14 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restri
     const int cell = blockIdx.x;
     const int h0 = home_start[cell], nh = home_start[cell+1]-h0;
     if (nh == 0) return;
+    const int inst = ca.cell_inst[cell];
+    if (inst < 0 || (only >= 0 && inst != only)) return;
+    const Consts& C = IC[inst].C;
     const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE, nwave = FP_BLOCK/WAVE;
     if (threadIdx.x == 0) {
-        int nc = g.nnb[cell], acc = 0;
+        int nc = ca.nnb[cell], acc = 0;
         for (int c = 0; c < nc; c++) {
-            int b = g.nb[64*cell+c];
+            int b = ca.nb[64*(size_t)cell+c];
             cells[c] = b;
             coff[c] = acc;
             acc += snap_start[b+1]-snap_start[b];
@@ -316,7 +330,9 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restri
     __syncthreads();
     const int nc = ncell;
     const int total = coff[nc];
-    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    double sd[3] = {sc[inst].shear_disp[0], sc[inst].shear_disp[1], sc[inst].shear_disp[2]};
+    const double L[3] = {C.L[0], C.L[1], C.L[2]}, half[3] = {C.half[0], C.half[1], C.half[2]};
+    const double lmg = C.lub_max_gap;
     for (int base = 0; base < total; base += FP_CHUNK) {
         int nchunk = min(FP_CHUNK, total-base);
         for (int t = threadIdx.x; t < nchunk; t += FP_BLOCK) {
@@ -340,9 +356,9 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restri
                     if (j > i) {
                         double4 xj = cpos[t];
                         double d[3] = {xj.x-xi.x, xj.y-xi.y, xj.z-xi.z};
-                        periodize_diff(C.L, C.half, sd, d);
+                        periodize_diff(L, half, sd, d);
                         double sq = d[0]*d[0]+d[1]*d[1]+d[2]*d[2];
-                        double range = lub_range(xi.w, xj.w, C.lub_max_gap);
+                        double range = lub_range(xi.w, xj.w, lmg);
                         hit = sq < range*range;
                     }
                 }
@@ -355,19 +371,20 @@ __global__ __launch_bounds__(FP_BLOCK) void k_find_pairs(const double4* __restri
             }
             if (lane == 0) {
                 found_cnt[i] = cnt;
-                if (cnt > K) raise_error(err, E_FOUND_OVERFLOW);
+                if (cnt > K) raise_error(sc+inst, E_FOUND_OVERFLOW);
             }
         }
         __syncthreads();
     }
 }
-void launch_find_pairs(const double4* pos, int n, CellGrid g, const int* home_start, const int* home_mem,
-                       const int* snap_start, const int* snap_mem, const DevScalars* sc, Consts C,
-                       int* found, int* found_cnt, int K, DevScalars* err, hipStream_t s)
+void launch_find_pairs(const double4* pos, int n, CellArrays ca, const InstConst* IC, const int* home_start,
+                       const int* home_mem, const int* snap_start, const int* snap_mem, DevScalars* sc,
+                       int* found, int* found_cnt, int K, int only, hipStream_t s)
 {
     launch_fill_int(found_cnt, 0, n, s);
-    k_find_pairs<<<g.nbox, FP_BLOCK, 0, s>>>(pos, g, home_start, home_mem, snap_start, snap_mem, sc, C,
-                                             found, found_cnt, K, err);
+    if (ca.ncell > 0)
+        k_find_pairs<<<ca.ncell, FP_BLOCK, 0, s>>>(pos, ca, IC, home_start, home_mem, snap_start, snap_mem, sc,
+                                                   found, found_cnt, K, only);
 }
 
 // ------------------------------------------------------------------ merge new pairs
@@ -402,7 +419,7 @@ __global__ void k_merge_count(int n, const int* __restrict__ old_p1, const int* 
 // ~(slot in the found list) for a new pair.
 __global__ void k_merge_index(int n, const int* __restrict__ old_p1, const int* __restrict__ old_row_ptr,
                               const int* __restrict__ found, const int* __restrict__ found_cnt, int K,
-                              const int* __restrict__ new_row_ptr, int* __restrict__ src, int pcap, DevScalars* err)
+                              const int* __restrict__ new_row_ptr, int* __restrict__ src, int pcap, BatchScalars* err)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
@@ -420,14 +437,15 @@ __global__ void k_merge_index(int n, const int* __restrict__ old_p1, const int* 
             else if (jo < jf) v = o++;
             else { v = ~(i*K+a); a++; }
         }
-        if (w >= pcap) { raise_error(err, E_PAIR_OVERFLOW); return; }
+        if (w >= pcap) { atomicOr(&err->error, E_PAIR_OVERFLOW); return; }
         src[w++] = v;
     }
 }
 // One thread per merged slot: copy the old record or create the new pair.
 __global__ __launch_bounds__(256) void k_merge_copy(const PairRec* __restrict__ old_pairs, const int* __restrict__ src,
     const int* __restrict__ found, int K, const int* __restrict__ total, PairRec* __restrict__ new_pairs,
-    const double4* __restrict__ pos, Consts C, const DevScalars* sc, int pcap)
+    const double4* __restrict__ pos, const int* __restrict__ inst_of, const InstConst* IC, const DevScalars* sc,
+    int pcap)
 {
     int w = blockIdx.x*blockDim.x+threadIdx.x;
     if (w >= min(*total, pcap)) return;
@@ -435,81 +453,100 @@ __global__ __launch_bounds__(256) void k_merge_copy(const PairRec* __restrict__ 
     if (v >= 0) { new_pairs[w] = old_pairs[v]; return; }
     v = ~v;
     int i = v/K, j = found[v];
-    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+    const int inst = inst_of[i];
+    const Consts& C = IC[inst].C;
+    double sd[3] = {sc[inst].shear_disp[0], sc[inst].shear_disp[1], sc[inst].shear_disp[2]};
     double4 xi = pos[i], xj = pos[j];
     double pi[3] = {xi.x, xi.y, xi.z}, pj[3] = {xj.x, xj.y, xj.z};
     PairRec q;
     pair_create(q, i, j, pi, pj, xi.w, xj.w, C, sd);
     new_pairs[w] = q;
 }
-__global__ void k_set_npairs_from(DevScalars* sc, const int* total, int pcap)
+__global__ void k_set_npairs_from(BatchScalars* bs, const int* total, int pcap)
 {
     int t = *total;
-    if (t > pcap) { atomicOr(&sc->error, E_PAIR_OVERFLOW); t = pcap; }
-    sc->npairs = t;
+    if (t > pcap) { atomicOr(&bs->error, E_PAIR_OVERFLOW); t = pcap; }
+    bs->npairs = t;
 }
 void launch_merge_pairs(int n, const PairRec* old_pairs, const int* old_p1, const int* old_row_ptr, const int* found,
                         const int* found_cnt, int K, int* cnt, int* new_row_ptr, int* src, PairRec* new_pairs,
-                        const double4* pos, Consts C, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
+                        const double4* pos, const int* inst_of, const InstConst* IC, const DevScalars* sc,
+                        BatchScalars* bs, int pcap, int* scan_tmp, hipStream_t s)
 {
     k_merge_count<<<(n+255)/256, 256, 0, s>>>(n, old_p1, old_row_ptr, const_cast<int*>(found), found_cnt, K, cnt);
     exclusive_scan_int(cnt, new_row_ptr, nullptr, n, scan_tmp, s);
-    k_merge_index<<<(n+255)/256, 256, 0, s>>>(n, old_p1, old_row_ptr, found, found_cnt, K, new_row_ptr, src, pcap, sc);
-    k_merge_copy<<<(pcap+255)/256, 256, 0, s>>>(old_pairs, src, found, K, new_row_ptr+n, new_pairs, pos, C, sc, pcap);
-    k_set_npairs_from<<<1, 1, 0, s>>>(sc, new_row_ptr+n, pcap);
+    k_merge_index<<<(n+255)/256, 256, 0, s>>>(n, old_p1, old_row_ptr, found, found_cnt, K, new_row_ptr, src, pcap, bs);
+    k_merge_copy<<<(pcap+255)/256, 256, 0, s>>>(old_pairs, src, found, K, new_row_ptr+n, new_pairs, pos, inst_of,
+                                                IC, sc, pcap);
+    k_set_npairs_from<<<1, 1, 0, s>>>(bs, new_row_ptr+n, pcap);
 }
 
 // ------------------------------------------------------------------ pair update
 // System::updateInteractions (System.cpp:1375-1403) without the swap-with-last
 // removal: every pair is updated once and removals are compacted afterwards.
-__global__ __launch_bounds__(256) void k_update_pairs(PairRec* pairs, const double4* __restrict__ pos, const double* __restrict__ vel6,
-                               Consts C, DevScalars* sc, int* keep, int pcap, int in_predictor, double dt_override)
+__global__ __launch_bounds__(256) void k_update_pairs(PairRec* pairs, const double4* __restrict__ pos,
+    const double* __restrict__ vel6, const int* __restrict__ inst_of, const InstConst* IC, DevScalars* sc,
+    const BatchScalars* bs, int* keep, int pcap, int in_predictor, double dt_override, int only)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
-    int np = sc->npairs;
+    int np = bs->npairs;
     if (k >= np || k >= pcap) return;
+    const int p0 = pairs[k].p0;
+    const int inst = inst_of[p0];
+    if (only == -2 || (only >= 0 && inst != only)) { keep[k] = inst >= 0 ? 1 : 0; return; }
     PairRec q = pairs[k];
+    const Consts& C = IC[inst].C;
+    DevScalars* si = sc+inst;
     double4 x0 = pos[q.p0], x1 = pos[q.p1];
     double a[3] = {x0.x, x0.y, x0.z}, b[3] = {x1.x, x1.y, x1.z};
-    double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
-    double dt = dt_override >= 0 ? dt_override : sc->dt;
-    const double vd[3] = {sc->flow.vel_diff[0], sc->flow.vel_diff[1], sc->flow.vel_diff[2]};
+    double sd[3] = {si->shear_disp[0], si->shear_disp[1], si->shear_disp[2]};
+    double dt = dt_override >= 0 ? dt_override : si->dt;
+    const double vd[3] = {si->flow.vel_diff[0], si->flow.vel_diff[1], si->flow.vel_diff[2]};
     int drop = pair_update(q, a, b, x0.w, x1.w, vel6+6*(size_t)q.p0, vel6+6*(size_t)q.p1, C, vd, sd, dt,
                            in_predictor);
-    if (C.friction && (q.flags & F_CONTACT)) atomic_max_nonneg(&sc->max_slide_bits, q.rsv_sq);
+    if (C.friction && (q.flags & F_CONTACT)) atomic_max_nonneg(&si->max_slide_bits, q.rsv_sq);
     pairs[k] = q;
     keep[k] = drop ? 0 : 1;
 }
-__global__ void k_reset_slide(DevScalars* sc) { sc->max_slide_bits = 0; }
-void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, Consts C, DevScalars* sc,
-                         int* keep, int pcap, int in_predictor, double dt_override, hipStream_t s)
+// per-instance resets: bit 0 max sliding velocity, bit 1 max velocity
+__global__ void k_reset_inst(DevScalars* sc, int ninst, int what, int only)
 {
-    k_reset_slide<<<1, 1, 0, s>>>(sc);
-    k_update_pairs<<<(pcap+255)/256, 256, 0, s>>>(pairs, pos, vel6, C, sc, keep, pcap, in_predictor, dt_override);
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= ninst || (only >= 0 && k != only)) return;
+    if (what & 1) sc[k].max_slide_bits = 0;
+    if (what & 2) sc[k].max_vel_bits = 0;
+}
+void launch_update_pairs(PairRec* pairs, const double4* pos, const double* vel6, const int* inst_of,
+                         const InstConst* IC, DevScalars* sc, int ninst, const BatchScalars* bs, int* keep,
+                         int pcap, int in_predictor, double dt_override, int only, hipStream_t s)
+{
+    if (only != -2) k_reset_inst<<<(ninst+63)/64, 64, 0, s>>>(sc, ninst, 1, only);
+    k_update_pairs<<<(pcap+255)/256, 256, 0, s>>>(pairs, pos, vel6, inst_of, IC, sc, bs, keep, pcap, in_predictor,
+                                                  dt_override, only);
 }
 
-__global__ void k_compact(const PairRec* in, PairRec* out, const int* keep, const int* ks, const DevScalars* sc)
+__global__ void k_compact(const PairRec* in, PairRec* out, const int* keep, const int* ks, const BatchScalars* bs)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
-    if (k >= sc->npairs) return;
+    if (k >= bs->npairs) return;
     if (keep[k]) out[ks[k]] = in[k];
 }
-__global__ void k_set_npairs(DevScalars* sc, const int* ks)
+__global__ void k_set_npairs(BatchScalars* bs, const int* ks)
 {
-    sc->npairs = ks[sc->npairs];
+    bs->npairs = ks[bs->npairs];
 }
 void launch_compact_pairs(const PairRec* in, PairRec* out, const int* keep, int* keep_scan, int n,
-                          int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, DevScalars* sc,
+                          int* row_ptr, int* p0, int* p1, int* tptr, int* tidx, int* tcnt, BatchScalars* bs,
                           int pcap, int* scan_tmp, hipStream_t s)
 {
-    exclusive_scan_int(keep, keep_scan, &sc->npairs, pcap, scan_tmp, s);
-    k_compact<<<(pcap+255)/256, 256, 0, s>>>(in, out, keep, keep_scan, sc);
-    k_set_npairs<<<1, 1, 0, s>>>(sc, keep_scan);
-    launch_pair_index(out, n, row_ptr, p0, p1, tptr, tidx, tcnt, const_cast<int*>(keep), sc, pcap, scan_tmp, s);
+    exclusive_scan_int(keep, keep_scan, &bs->npairs, pcap, scan_tmp, s);
+    k_compact<<<(pcap+255)/256, 256, 0, s>>>(in, out, keep, keep_scan, bs);
+    k_set_npairs<<<1, 1, 0, s>>>(bs, keep_scan);
+    launch_pair_index(out, n, row_ptr, p0, p1, tptr, tidx, tcnt, const_cast<int*>(keep), bs, pcap, scan_tmp, s);
 }
 
 // ------------------------------------------------------------------ pair index
-__global__ void k_pair_arrays(const PairRec* pairs, int* p0, int* p1, int* tcnt, const DevScalars* sc)
+__global__ void k_pair_arrays(const PairRec* pairs, int* p0, int* p1, int* tcnt, const BatchScalars* sc)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= sc->npairs) return;
@@ -517,7 +554,7 @@ __global__ void k_pair_arrays(const PairRec* pairs, int* p0, int* p1, int* tcnt,
     p0[k] = a; p1[k] = b;
     atomicAdd(&tcnt[b], 1);
 }
-__global__ void k_row_ptr(const int* p0, int n, int* row_ptr, const DevScalars* sc)
+__global__ void k_row_ptr(const int* p0, int n, int* row_ptr, const BatchScalars* sc)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i > n) return;
@@ -525,7 +562,7 @@ __global__ void k_row_ptr(const int* p0, int n, int* row_ptr, const DevScalars* 
     while (lo < hi) { int m = (lo+hi) >> 1; if (p0[m] < i) lo = m+1; else hi = m; }
     row_ptr[i] = lo;
 }
-__global__ void k_tfill(const int* p1, int* cursor, int* tmp, const DevScalars* sc)
+__global__ void k_tfill(const int* p1, int* cursor, int* tmp, const BatchScalars* sc)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= sc->npairs) return;
@@ -534,7 +571,7 @@ __global__ void k_tfill(const int* p1, int* cursor, int* tmp, const DevScalars* 
 // lower incidence lists in ascending pair index (= ascending p0): one thread
 // per entry places it by its rank within its list
 __global__ void k_tsort(const int* __restrict__ tptr, const int* __restrict__ p1, const int* __restrict__ tmp,
-                        int* __restrict__ tidx, const DevScalars* sc)
+                        int* __restrict__ tidx, const BatchScalars* sc)
 {
     const int a = blockIdx.x*blockDim.x+threadIdx.x;
     if (a >= sc->npairs) return;
@@ -545,7 +582,7 @@ __global__ void k_tsort(const int* __restrict__ tptr, const int* __restrict__ p1
     tidx[s0+rank] = x;
 }
 void launch_pair_index(const PairRec* pairs, int n, int* row_ptr, int* p0, int* p1, int* tptr, int* tidx,
-                       int* tcnt, int* tmp, DevScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
+                       int* tcnt, int* tmp, const BatchScalars* sc, int pcap, int* scan_tmp, hipStream_t s)
 {
     launch_fill_int(tcnt, 0, n, s);
     k_pair_arrays<<<(pcap+255)/256, 256, 0, s>>>(pairs, p0, p1, tcnt, sc);
@@ -713,20 +750,23 @@ __device__ inline bool dblock_inverse(const double* b, double* inv64, float* inv
 // diagonal contributions and its right-hand-side terms for both particles.
 // pdb[30*k]: DBlock of p0 then of p1; prhs[42*k]: RhsAcc of p0 then of p1.
 static constexpr int RHS_TERMS = 21;
-__global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restrict__ pairs, const double4* __restrict__ pos, Consts C,
-                                 const DevScalars* sc, double* __restrict__ od64, float* __restrict__ od32,
-                                 double* __restrict__ pdb, double* __restrict__ prhs, int pcap)
+__global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restrict__ pairs,
+    const double4* __restrict__ pos, const int* __restrict__ inst_of, const InstConst* IC, const DevScalars* sc,
+    const BatchScalars* bs, double* __restrict__ od64, float* __restrict__ od32, double* __restrict__ pdb,
+    double* __restrict__ prhs, int pcap)
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
-    if (k >= sc->npairs || k >= pcap) return;
+    if (k >= bs->npairs || k >= pcap) return;
     const PairRec q = pairs[k];
+    const int inst = inst_of[q.p0];
+    const Consts& C = IC[inst].C;
     const double4 x04 = pos[q.p0], x14 = pos[q.p1];
     double db0[15], db1[15], od[18];
     pair_blocks(q, x04.w, x14.w, C, db0, db1, od);
     for (int m = 0; m < 18; m++) { od64[18*(size_t)k+m] = od[m]; od32[18*(size_t)k+m] = (float)od[m]; }
     for (int m = 0; m < 15; m++) { pdb[30*(size_t)k+m] = db0[m]; pdb[30*(size_t)k+15+m] = db1[m]; }
     const double x0[3] = {x04.x, x04.y, x04.z}, x1[3] = {x14.x, x14.y, x14.z};
-    const Flow F = sc->flow;
+    const Flow F = sc[inst].flow;
     for (int side = 0; side < 2; side++) {
         RhsAcc R;
         for (int m = 0; m < 3; m++) {
@@ -754,16 +794,18 @@ __global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restric
 static constexpr int GD_LANES = 8, GD_BLOCK = 256, GD_PER = GD_BLOCK/GD_LANES, GD_VALS = 15+RHS_TERMS;
 
 __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __restrict__ pdb,
-    const double* __restrict__ prhs, const double4* __restrict__ pos, int n, Consts C,
-    const int* __restrict__ row_ptr, const int* __restrict__ tptr, const int* __restrict__ tidx,
-    double* __restrict__ diag64, float* __restrict__ diag32, double* __restrict__ minv64,
-    float* __restrict__ minv32, double* __restrict__ rhs, double* __restrict__ rhs_parts, int nparts,
-    DevScalars* err)
+    const double* __restrict__ prhs, const double4* __restrict__ pos, int n, const int* __restrict__ inst_of,
+    const InstConst* IC, const int* __restrict__ row_ptr, const int* __restrict__ tptr,
+    const int* __restrict__ tidx, double* __restrict__ diag64, float* __restrict__ diag32,
+    double* __restrict__ minv64, float* __restrict__ minv32, double* __restrict__ rhs,
+    double* __restrict__ rhs_parts, int nparts, DevScalars* err)
 {
     __shared__ double sv[GD_PER][GD_VALS+1];
     const int lp = threadIdx.x/GD_LANES, c = threadIdx.x%GD_LANES;
     const int i = blockIdx.x*GD_PER+lp;
-    if (i < n) {
+    const int inst = i < n ? inst_of[i] : -1;
+    if (inst >= 0) {
+        const Consts& C = IC[inst].C;
         const double a = pos[i].w;
         const double FU = C.sd_coeff*a;
         const double TW = C.sd_coeff*(a*a*a)*4.0/3;
@@ -801,16 +843,35 @@ __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __re
             if (q < GD_VALS) sv[lp][q] = acc[u];
             if (q < 15) { diag64[15*(size_t)i+q] = acc[u]; diag32[15*(size_t)i+q] = (float)acc[u]; }
         }
+    } else if (i < n) {
+        // free slot: unit diagonal, zero right-hand sides
+        #pragma unroll
+        for (int u = 0; u < 5; u++) {
+            const int q = c+GD_LANES*u;
+            const double v = (q == 0 || q == 5 || q == 8 || q == 9 || q == 12 || q == 14) ? 1.0 : 0.0;
+            if (q < GD_VALS) sv[lp][q] = q < 15 ? v : 0.0;
+            if (q < 15) { diag64[15*(size_t)i+q] = v; diag32[15*(size_t)i+q] = (float)v; }
+        }
     }
     __syncthreads();
     if (threadIdx.x >= GD_PER) return;
     const int ip = blockIdx.x*GD_PER+threadIdx.x;
     if (ip >= n) return;
+    const int ik = inst_of[ip];
     const double* D = sv[threadIdx.x];
     const double* R = sv[threadIdx.x]+15;
     double Dl[15];
     for (int m = 0; m < 15; m++) Dl[m] = D[m];
-    if (!dblock_inverse(Dl, minv64+36*(size_t)ip, minv32+36*(size_t)ip)) raise_error(err, E_NOT_SPD);
+    if (!dblock_inverse(Dl, minv64+36*(size_t)ip, minv32+36*(size_t)ip) && ik >= 0) raise_error(err+ik, E_NOT_SPD);
+    const size_t m6 = 6*(size_t)n, o6 = 6*(size_t)ip;
+    if (ik < 0) {
+        for (int m = 0; m < 6; m++) {
+            rhs[o6+m] = 0.0;
+            for (int p = 0; p < nparts; p++) rhs_parts[p*m6+o6+m] = 0.0;
+        }
+        return;
+    }
+    const Consts& C = IC[ik].C;
     for (int m = 0; m < 3; m++) {
         double F = 0, T = 0;
         F += R[m];                           // contact
@@ -824,7 +885,6 @@ __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __re
         rhs[6*(size_t)ip+3+m] = T;
     }
     // right-hand sides of the velocity parts (System::setSolverRHS, System.cpp:1858-1871)
-    const size_t m6 = 6*(size_t)n, o6 = 6*(size_t)ip;
     for (int m = 0; m < 3 && nparts == 4; m++) {
         rhs_parts[o6+m] = R[m];                                   // contact
         rhs_parts[o6+3+m] = C.friction ? R[3+m] : 0.0;
@@ -852,14 +912,15 @@ __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __re
         rhs_parts[m6+o6+3+m] = T;
     }
 }
-void launch_assemble(const PairRec* pairs, const double4* pos, int n, Consts C, const DevScalars* sc,
-                     const int* row_ptr, const int* tptr, const int* tidx, double* diag64, float* diag32,
-                     double* od64, float* od32, double* minv64, float* minv32, double* rhs, double* pdb,
-                     double* prhs, int pcap, double* rhs_parts, int nparts, DevScalars* err, hipStream_t s)
+void launch_assemble(const PairRec* pairs, const double4* pos, int n, const int* inst_of, const InstConst* IC,
+                     DevScalars* sc, const BatchScalars* bs, const int* row_ptr, const int* tptr, const int* tidx,
+                     double* diag64, float* diag32, double* od64, float* od32, double* minv64, float* minv32,
+                     double* rhs, double* pdb, double* prhs, int pcap, double* rhs_parts, int nparts, hipStream_t s)
 {
-    k_assemble_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, C, sc, od64, od32, pdb, prhs, pcap);
-    k_gather_diag_rhs<<<(n+GD_PER-1)/GD_PER, GD_BLOCK, 0, s>>>(pdb, prhs, pos, n, C, row_ptr, tptr, tidx, diag64, diag32,
-                                                  minv64, minv32, rhs, rhs_parts, nparts, err);
+    k_assemble_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, inst_of, IC, sc, bs, od64, od32, pdb, prhs, pcap);
+    k_gather_diag_rhs<<<(n+GD_PER-1)/GD_PER, GD_BLOCK, 0, s>>>(pdb, prhs, pos, n, inst_of, IC, row_ptr, tptr, tidx,
+                                                               diag64, diag32, minv64, minv32, rhs, rhs_parts,
+                                                               nparts, sc);
 }
 
 // Matrix handed over through the StokesSolver staging protocol: FP32 copy and
@@ -958,13 +1019,16 @@ __device__ inline void dashpot_force(const PairRec& q, const DashCoef& c, const 
 //   repulsive xF, half to each particle (RepulsiveForce.cpp:111-131).
 // pst[k][side][slot][6]; unused slots are written as zero.
 __global__ __launch_bounds__(128) void k_stress_pairs(const PairRec* __restrict__ pairs, const double4* __restrict__ pos,
-    Consts C, const DevScalars* sc, const double* __restrict__ parts, int nparts, const double* __restrict__ vel6,
-    int n, int control, double* __restrict__ pst, int pcap)
+    const int* __restrict__ inst_of, const InstConst* IC, const DevScalars* sc, const BatchScalars* bs,
+    const double* __restrict__ parts, int nparts, const double* __restrict__ vel6, int n, int control,
+    double* __restrict__ pst, int pcap)
 {
     const int k = blockIdx.x*blockDim.x+threadIdx.x;
-    if (k >= sc->npairs || k >= pcap) return;
+    if (k >= bs->npairs || k >= pcap) return;
     const PairRec q = pairs[k];
-    const Flow F = sc->flow;
+    const int inst = inst_of[q.p0];
+    const Consts& C = IC[inst].C;
+    const Flow F = sc[inst].flow;
     const double4 x04 = pos[q.p0], x14 = pos[q.p1];
     const double a0 = x04.w, a1 = x14.w;
     const double* n3 = q.n;
@@ -975,7 +1039,7 @@ __global__ __launch_bounds__(128) void k_stress_pairs(const PairRec* __restrict_
     // interparticle vector as Interaction::calcNormalVectorDistanceGap computes it
     double rv[3] = {x14.x-x04.x, x14.y-x04.y, x14.z-x04.z};
     {
-        double sd[3] = {sc->shear_disp[0], sc->shear_disp[1], sc->shear_disp[2]};
+        double sd[3] = {sc[inst].shear_disp[0], sc[inst].shear_disp[1], sc[inst].shear_disp[2]};
         periodize_diff(C.L, C.half, sd, rv);
     }
     if (q.flags & F_LUB) {
@@ -1125,14 +1189,20 @@ __global__ void k_stress_chunks(const double* __restrict__ ps, int n, double* __
 // (System::computeShearRate + gatherStressesByRateDependencies,
 // System.cpp:2017-2055, Stress.cpp:320-340) or the output groups
 // (System::calcStress, Stress.cpp:342-384).
-__global__ void k_stress_finalize(const double* __restrict__ partial, int nchunks, Consts C, DevScalars* sc,
+// One workgroup per instance, over the chunks of its slots.
+__global__ void k_stress_finalize(const double* __restrict__ partial, const InstConst* IC, DevScalars* scs,
                                   int nparts, int control, int mode)
 {
     __shared__ double tot[ST_VALS];
     const int v = threadIdx.x;
+    const InstConst& ic = IC[blockIdx.x];
+    DevScalars* sc = scs+blockIdx.x;
+    if (ic.n <= 0) return;
+    const Consts& C = ic.C;
+    const int b0 = ic.poff/ST_CHUNK, b1 = (ic.poff+ic.n+ST_CHUNK-1)/ST_CHUNK;
     if (v < ST_VALS) {
         double acc = 0;
-        for (int b = 0; b < nchunks; b++) acc += partial[(size_t)ST_VALS*b+v];
+        for (int b = b0; b < b1; b++) acc += partial[(size_t)ST_VALS*b+v];
         tot[v] = acc;
     }
     __syncthreads();
@@ -1196,11 +1266,13 @@ __global__ void k_stress_finalize(const double* __restrict__ partial, int nchunk
 // proportional parts are first rescaled by the new shear rate, in place
 // (rescaleVelHydroStressControlled + sumUpVelocityComponents, System.cpp:2236-2271).
 __global__ void k_sum_parts(double* __restrict__ parts, int nparts, double* __restrict__ na6, int n,
-                            const DevScalars* sc, int rescale)
+                            const int* __restrict__ inst_of, const DevScalars* sc, int rescale)
 {
     const size_t t = blockIdx.x*(size_t)blockDim.x+threadIdx.x, m6 = 6*(size_t)n;
     if (t >= m6) return;
-    const double rate = sc->flow.rate;
+    const int inst = inst_of[t/6];
+    if (inst < 0) { na6[t] = 0.0; return; }
+    const double rate = sc[inst].flow.rate;
     double acc = 0;
     for (int p = 0; p < nparts; p++) {
         double v = parts[p*m6+t];
@@ -1211,44 +1283,50 @@ __global__ void k_sum_parts(double* __restrict__ parts, int nparts, double* __re
     na6[t] = acc;
 }
 
-__global__ void k_set_rate(DevScalars* sc, Consts C, double rate)
+__global__ void k_set_rate(DevScalars* sc, const InstConst* IC, int ninst, double rate)
 {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= ninst || IC[k].n <= 0) return;
     Flow F;
-    make_flow(C, rate, F);
-    sc->flow = F;
+    make_flow(IC[k].C, rate, F);
+    sc[k].flow = F;
 }
 
-void launch_set_rate(DevScalars* sc, Consts C, double rate, hipStream_t s)
+void launch_set_rate(DevScalars* sc, const InstConst* IC, int ninst, double rate, hipStream_t s)
 {
-    k_set_rate<<<1, 1, 0, s>>>(sc, C, rate);
+    k_set_rate<<<(ninst+63)/64, 64, 0, s>>>(sc, IC, ninst, rate);
 }
 
-void launch_stress(const PairRec* pairs, const double4* pos, Consts C, DevScalars* sc, const double* parts,
-                   int nparts, const double* vel6, int n, int control, int mode, const int* row_ptr,
-                   const int* tptr, const int* tidx, double* pst, double* ps, double* partial, int pcap,
-                   hipStream_t s)
+void launch_stress(const PairRec* pairs, const double4* pos, const int* inst_of, const InstConst* IC, int ninst,
+                   DevScalars* sc, const BatchScalars* bs, const double* parts, int nparts, const double* vel6,
+                   int n, int control, int mode, const int* row_ptr, const int* tptr, const int* tidx, double* pst,
+                   double* ps, double* partial, int pcap, hipStream_t s)
 {
-    k_stress_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, C, sc, parts, nparts, vel6, n, control, pst, pcap);
+    k_stress_pairs<<<(pcap+127)/128, 128, 0, s>>>(pairs, pos, inst_of, IC, sc, bs, parts, nparts, vel6, n, control,
+                                                  pst, pcap);
     k_stress_gather<<<(n+3)/4, 256, 0, s>>>(pst, n, row_ptr, tptr, tidx, ps);
     const int nch = (n+ST_CHUNK-1)/ST_CHUNK;
     k_stress_chunks<<<nch, 64, 0, s>>>(ps, n, partial);
-    k_stress_finalize<<<1, 64, 0, s>>>(partial, nch, C, sc, nparts, control, mode);
+    k_stress_finalize<<<ninst, 64, 0, s>>>(partial, IC, sc, nparts, control, mode);
 }
 
-void launch_sum_parts(double* parts, int nparts, double* na6, int n, const DevScalars* sc, int rescale,
-                      hipStream_t s)
+void launch_sum_parts(double* parts, int nparts, double* na6, int n, const int* inst_of, const DevScalars* sc,
+                      int rescale, hipStream_t s)
 {
     const size_t m6 = 6*(size_t)n;
-    k_sum_parts<<<(unsigned)((m6+255)/256), 256, 0, s>>>(parts, nparts, na6, n, sc, rescale);
+    k_sum_parts<<<(unsigned)((m6+255)/256), 256, 0, s>>>(parts, nparts, na6, n, inst_of, sc, rescale);
 }
 
 // ------------------------------------------------------------------ velocities, time step, moves
-__global__ void k_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
-                                  DevScalars* sc, int in_predictor)
+__global__ void k_finish_velocity(int n, const double4* pos, const double* na6, double* vel6,
+                                  const int* __restrict__ inst_of, DevScalars* scs, int in_predictor)
 {
     // System::computeMaxNAVelocity + adjustVelocityPeriodicBoundary (System.cpp:1887-1902, 2369-2392)
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
+    const int inst = inst_of[i];
+    if (inst < 0) return;
+    DevScalars* sc = scs+inst;
     double4 x4 = pos[i];
     double x[3] = {x4.x, x4.y, x4.z}, u[3];
     const Flow& F = sc->flow;
@@ -1260,17 +1338,22 @@ __global__ void k_finish_velocity(int n, const double4* pos, const double* na6, 
         vel6[6*(size_t)i+3+k] = v[3+k]+F.omega[k];
     }
 }
-__global__ void k_reset_maxvel(DevScalars* sc) { sc->max_vel_bits = 0; }
-void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, Consts C,
-                            DevScalars* sc, int in_predictor, hipStream_t s)
+void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, const int* inst_of,
+                            DevScalars* sc, int ninst, int in_predictor, hipStream_t s)
 {
-    if (in_predictor) k_reset_maxvel<<<1, 1, 0, s>>>(sc);
-    k_finish_velocity<<<(n+255)/256, 256, 0, s>>>(n, pos, na6, vel6, C, sc, in_predictor);
+    if (in_predictor) k_reset_inst<<<(ninst+63)/64, 64, 0, s>>>(sc, ninst, 2, -1);
+    k_finish_velocity<<<(n+255)/256, 256, 0, s>>>(n, pos, na6, vel6, inst_of, sc, in_predictor);
 }
 
-// System::adaptTimeStep (System.cpp:1013-1066), time_end disabled
-__global__ void k_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time)
+// System::adaptTimeStep (System.cpp:1013-1066), time_end disabled; one
+// thread per instance
+__global__ void k_adapt_dt(DevScalars* scs, const InstConst* IC, int ninst, int advance_time)
 {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= ninst || IC[k].n <= 0) return;
+    DevScalars* sc = scs+k;
+    const Params& P = IC[k].P;
+    const double strain_end = sc->strain_end;
     double rate = sc->flow.rate;
     double mv = sqrt(bits_to_double(sc->max_vel_bits));
     double ms = sqrt(bits_to_double(sc->max_slide_bits));
@@ -1286,14 +1369,27 @@ __global__ void k_adapt_dt(DevScalars* sc, Params P, double strain_end, int adva
     sc->dt = dt;
     if (advance_time) sc->time += dt;
 }
-void launch_adapt_dt(DevScalars* sc, Params P, double strain_end, int advance_time, hipStream_t s)
+void launch_adapt_dt(DevScalars* sc, const InstConst* IC, int ninst, int advance_time, hipStream_t s)
 {
-    k_adapt_dt<<<1, 1, 0, s>>>(sc, P, strain_end, advance_time);
+    k_adapt_dt<<<(ninst+63)/64, 64, 0, s>>>(sc, IC, ninst, advance_time);
+}
+__global__ void k_set_strain_end(DevScalars* sc, int ninst, double strain_end)
+{
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k < ninst) sc[k].strain_end = strain_end;
+}
+void launch_set_strain_end(DevScalars* sc, int ninst, double strain_end, hipStream_t s)
+{
+    k_set_strain_end<<<(ninst+63)/64, 64, 0, s>>>(sc, ninst, strain_end);
 }
 
-// System::timeStepBoxing (System.cpp:672-717), simple shear
-__global__ void k_boxing(DevScalars* sc, Consts C)
+// System::timeStepBoxing (System.cpp:672-717), simple shear; one thread per instance
+__global__ void k_boxing(DevScalars* scs, const InstConst* IC, int ninst)
 {
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= ninst || IC[k].n <= 0) return;
+    DevScalars* sc = scs+k;
+    const Consts& C = IC[k].C;
     double dt = sc->dt;
     const Flow& F = sc->flow;
     sc->strain += F.rate*dt;
@@ -1308,15 +1404,23 @@ __global__ void k_boxing(DevScalars* sc, Consts C)
     if (sc->shear_disp[1] < 0) m--;
     sc->shear_disp[1] = sc->shear_disp[1]-m*C.L[1];
 }
-void launch_boxing(DevScalars* sc, Consts C, hipStream_t s) { k_boxing<<<1, 1, 0, s>>>(sc, C); }
+void launch_boxing(DevScalars* sc, const InstConst* IC, int ninst, hipStream_t s)
+{
+    k_boxing<<<(ninst+63)/64, 64, 0, s>>>(sc, IC, ninst);
+}
 
 // System::displacement (System.cpp:2418-2443).  mode 0: predictor move by
 // U dt; mode 1: corrector (velocity average, System.cpp:1164-1187).
-__global__ void k_displace(double4* pos, double* vel6, const double* pred6, int n, Consts C, DevScalars* sc,
-                           CellGrid g, int* box_of, int mode)
+__global__ void k_displace(double4* pos, double* vel6, const double* pred6, int n, const int* __restrict__ inst_of,
+                           const InstConst* IC, DevScalars* scs, int* box_of, int mode)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
+    const int inst = inst_of[i];
+    if (inst < 0) return;
+    DevScalars* sc = scs+inst;
+    const InstConst& g = IC[inst];
+    const Consts& C = g.C;
     double dt = sc->dt;
     double* v = vel6+6*(size_t)i;
     double dr[3];
@@ -1338,12 +1442,12 @@ __global__ void k_displace(double4* pos, double* vel6, const double* pred6, int 
     pos[i] = make_double4(x[0], x[1], x[2], x4.w);
     int l = which_box(x, g.bx, g.by, g.bz, (unsigned)g.yz, (unsigned)g.nz, (unsigned)g.nbox);
     if (l < 0) { raise_error(sc, E_BOX_OOB); l = 0; }
-    box_of[i] = l;
+    box_of[i] = g.coff+l;
 }
-void launch_displace(double4* pos, double* vel6, const double* pred6, int n, Consts C, DevScalars* sc,
-                     CellGrid g, int* box_of, int mode, hipStream_t s)
+void launch_displace(double4* pos, double* vel6, const double* pred6, int n, const int* inst_of,
+                     const InstConst* IC, DevScalars* sc, int* box_of, int mode, hipStream_t s)
 {
-    k_displace<<<(n+255)/256, 256, 0, s>>>(pos, vel6, pred6, n, C, sc, g, box_of, mode);
+    k_displace<<<(n+255)/256, 256, 0, s>>>(pos, vel6, pred6, n, inst_of, IC, sc, box_of, mode);
 }
 
 // ------------------------------------------------------------------ block SpMV
@@ -1821,19 +1925,24 @@ __device__ inline void tree_barrier(unsigned* bar, int G, int wg, unsigned& gen,
 struct LdsRow { int e0, ne; };      // staged entries of a row (ne < 0: not staged)
 
 __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double* __restrict__ b,
-    double* __restrict__ x, SolverWork w, DevScalars* sc, SolveOpts o, unsigned* bar)
+    double* __restrict__ x, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
 {
     __shared__ int ent_j[LS_ENTRY_CAP];             // partner; ~partner for the transposed block
     __shared__ float ent_b[LS_ENTRY_CAP*18];
     __shared__ LdsRow rows[LS_ROWS];
     __shared__ double red[3*LS_WAVES+4];
     __shared__ int s_fail;
-    const int wg = blockIdx.x, G = gridDim.x, n = A.n;
+    // this workgroup's instance: rows [row0 + wg*LS_ROWS, ...) below row_end;
+    // its partial records and barrier words follow from its first row block
+    const SolveJob J = jobs[blockIdx.x];
+    const int wg = J.lwg, G = J.G, n = J.row_end, rbase = J.row0+wg*LS_ROWS;
+    DevScalars* sc = scs+J.inst;
+    unsigned* bar = w.barrier+128*(size_t)(J.row0/LS_ROWS);
     unsigned seq = 0;
     double g0 = 0, g1 = 0, g2 = 0;        // wave 0: this lane's share of the polled partial sums
     const int lr = threadIdx.x/LS_LPR, c = threadIdx.x%LS_LPR;
     const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
-    const int i = wg*LS_ROWS+lr;
+    const int i = rbase+lr;
     const bool live = i < n;
     const bool own = live && c < 6;
     const size_t gi = 6*(size_t)i+c;
@@ -1841,14 +1950,14 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     if (threadIdx.x == 0) s_fail = 0;
     // ---- stage this workgroup's rows: degrees in parallel, serial prefix in LDS
     if (threadIdx.x < LS_ROWS) {
-        int ii = wg*LS_ROWS+threadIdx.x;
+        int ii = rbase+threadIdx.x;
         rows[threadIdx.x].ne = ii < n ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int used = 0;
         for (int t = 0; t < LS_ROWS; t++) {
-            int ii = wg*LS_ROWS+t, deg = rows[t].ne;
+            int ii = rbase+t, deg = rows[t].ne;
             if (ii < n && used+deg <= LS_ENTRY_CAP) { rows[t].e0 = used; used += deg; }
             else { rows[t].e0 = 0; rows[t].ne = ii < n ? -1 : 0; }
         }
@@ -1930,7 +2039,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         return own ? z : 0.f;
     };
 
-    double* part = w.partials;
+    double* part = w.partials+6*(size_t)(J.row0/LS_ROWS);
     int buf = 0;
     // Grid sums of three values.  Block sums (DPP tree, fixed order) are
     // published as tagged partial records in the current buffer after every
@@ -2170,19 +2279,19 @@ int lds_solver_max_groups()
     return cached;
 }
 
-// One workgroup of 1024 threads per 128 block rows, all co-resident.
+// One workgroup of 1024 threads per 128 block rows; all workgroups of the
+// launch (every instance in the job table) co-resident.
 hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
-                     const SolveOpts& o, hipStream_t s)
+                     const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s)
 {
-    int G = (A.n+LS_ROWS-1)/LS_ROWS;
-    if (G > lds_solver_max_groups() || G > w.max_groups) return hipErrorCooperativeLaunchTooLarge;
+    if (nwg > lds_solver_max_groups()) return hipErrorCooperativeLaunchTooLarge;
     BsrView Av = A;
     SolverWork wv = w;
     SolveOpts ov = o;
-    unsigned* bar = w.barrier;
-    launch_fill_double(w.partials, 0.0, 6*(size_t)G, s);     // no stale round tags
-    void* args[] = {&Av, (void*)&b, (void*)&x, &wv, &sc, &ov, &bar};
-    return hipLaunchCooperativeKernel((const void*)k_solve_lds, dim3(G), dim3(LS_BLOCK), args, 0, s);
+    // no stale round tags in any instance's partial records
+    launch_fill_double(w.partials, 0.0, 6*(size_t)((A.n+LS_ROWS-1)/LS_ROWS), s);
+    void* args[] = {&Av, (void*)&b, (void*)&x, &wv, &sc, &ov, (void*)&jobs};
+    return hipLaunchCooperativeKernel((const void*)k_solve_lds, dim3(nwg), dim3(LS_BLOCK), args, 0, s);
 }
 
 int persistent_groups(const SolveOpts& o)

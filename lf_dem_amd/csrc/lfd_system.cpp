@@ -135,10 +135,17 @@ struct Solver {
     float *diag32 = nullptr, *od32 = nullptr, *minv32 = nullptr;
     int *p0 = nullptr, *p1 = nullptr, *row_ptr = nullptr, *tptr = nullptr, *tidx = nullptr;
     SolverWork w{};
+    // batched LDS solver: job table and its cooperative launches (first job, count);
+    // no launch means the single instance `single` is solved by the global-memory form
+    SolveJob* jobs = nullptr;
+    int jobs_cap = 0;
+    std::vector<std::pair<int, int>> launches;
+    int single = -1;          // the only instance, or -1 when there are several
     hipError_t alloc(int n_, int pcap_)
     {
         n = n_; pcap = pcap_;
         size_t m = 6*(size_t)n;
+        const int rb = (n+LDS_ROWS-1)/LDS_ROWS;
         hipError_t e = hipSuccess;
 #define A(x) if (e == hipSuccess) e = (x)
         A(dalloc(&diag64, 15*(size_t)n)); A(dalloc(&diag32, 15*(size_t)n));
@@ -151,9 +158,13 @@ struct Solver {
         A(dalloc(&w.d, m));
         A(dalloc(&w.pr, m)); A(dalloc(&w.pz, m)); A(dalloc(&w.pp, m)); A(dalloc(&w.pq, m));
         w.max_groups = 2048;
-        A(dalloc(&w.partials, 8*(size_t)w.max_groups));
-        A(dalloc(&w.barrier, 16384));
-        A(hipMemset(w.barrier, 0, 16384*sizeof(unsigned)));
+        A(dalloc(&w.partials, std::max<size_t>(8*(size_t)w.max_groups, 6*(size_t)rb+64)));
+        // tree-barrier words: 128 per row block for the batched solver, 16384 for the global form
+        const size_t nbar = std::max<size_t>(16384, 128*(size_t)rb+256);
+        A(dalloc(&w.barrier, nbar));
+        A(hipMemset(w.barrier, 0, nbar*sizeof(unsigned)));
+        jobs_cap = rb+1;
+        A(dalloc(&jobs, jobs_cap));
 #undef A
         return e;
     }
@@ -161,7 +172,7 @@ struct Solver {
     {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
                         w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials,
-                        w.barrier};
+                        w.barrier, jobs};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (auto& e : prof) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         prof.clear();
@@ -175,24 +186,71 @@ struct Solver {
     }
 };
 
+struct InstSpan { int inst, row0, n; };
+
+// Job table of the batched solver: ceil(n/128) workgroups per instance, packed
+// in slot order into cooperative launches that fit the device.  An instance
+// too large for one launch can only run alone, on the global-memory form.
+static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s)
+{
+    S.launches.clear();
+    S.single = spans.size() == 1 ? spans[0].inst : -1;
+    if (spans.empty()) return LFD_OK;
+    const int maxg = std::min(lds_solver_max_groups(), S.w.max_groups);
+    std::vector<SolveJob> jobs;
+    int first = 0;
+    for (const InstSpan& sp : spans) {
+        const int G = (sp.n+LDS_ROWS-1)/LDS_ROWS;
+        if (G > maxg) {
+            if (spans.size() == 1) return LFD_OK;
+            return fail(LFD_ERR_CAPACITY, "an instance of this size only runs alone in its batch");
+        }
+        if ((int)jobs.size()-first+G > maxg) {
+            S.launches.emplace_back(first, (int)jobs.size()-first);
+            first = (int)jobs.size();
+        }
+        for (int g = 0; g < G; g++) {
+            SolveJob j{};
+            j.inst = sp.inst; j.lwg = g; j.G = G; j.row0 = sp.row0; j.row_end = sp.row0+sp.n;
+            jobs.push_back(j);
+        }
+    }
+    S.launches.emplace_back(first, (int)jobs.size()-first);
+    if ((int)jobs.size() > S.jobs_cap) return fail(LFD_ERR_CAPACITY, "solver job table capacity exceeded");
+    HIPCHK(hipMemcpyAsync(S.jobs, jobs.data(), jobs.size()*sizeof(SolveJob), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return LFD_OK;
+}
+
+static int status_of(int err)
+{
+    if (err & E_BOX_OOB) return fail(LFD_ERR_BOX, "BoxSet: trying to box position out of boundaries");
+    if (err & (E_FOUND_OVERFLOW | E_PAIR_OVERFLOW)) return fail(LFD_ERR_CAPACITY, "pair capacity exceeded");
+    if (err & E_NOT_SPD) return fail(LFD_ERR_NOT_SPD, "resistance diagonal block not positive definite");
+    if (err & E_SYNC_TIMEOUT) return fail(LFD_ERR_HIP, "solver workgroups lost synchronisation (spin limit)");
+    if (err & E_NOT_CONVERGED) return fail(LFD_ERR_NOT_CONVERGED, "solver did not reach the tolerance");
+    return LFD_OK;
+}
+
 static int check_scalars(DevScalars* sc_dev, hipStream_t s, DevScalars* out = nullptr)
 {
     DevScalars h;
     HIPCHK(hipMemcpyAsync(&h, sc_dev, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (out) *out = h;
-    if (h.error & E_BOX_OOB) return fail(LFD_ERR_BOX, "BoxSet: trying to box position out of boundaries");
-    if (h.error & (E_FOUND_OVERFLOW | E_PAIR_OVERFLOW)) return fail(LFD_ERR_CAPACITY, "pair capacity exceeded");
-    if (h.error & E_NOT_SPD) return fail(LFD_ERR_NOT_SPD, "resistance diagonal block not positive definite");
-    if (h.error & E_SYNC_TIMEOUT) return fail(LFD_ERR_HIP, "solver workgroups lost synchronisation (spin limit)");
-    if (h.error & E_NOT_CONVERGED) return fail(LFD_ERR_NOT_CONVERGED, "solver did not reach the tolerance");
-    return LFD_OK;
+    return status_of(h.error);
 }
 
+// One solve of every instance: the batched LDS launches, or for a lone
+// instance the plain form or the global-memory form.
 static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* x, DevScalars* sc,
                      hipStream_t s)
 {
     BsrView A = S.view(npairs_dev);
+    if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL || S.launches.empty()) {
+        if (S.single < 0) return fail(LFD_ERR_UNSUPPORTED, "this solver form takes a single instance");
+        sc += S.single;
+    }
     if (S.kind == LFD_SOLVER_PLAIN) {
         int it = 0;
         double res = 0;
@@ -203,6 +261,7 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         h.iters_inner = it; h.iters_outer = 1; h.res_norm = res; h.rhs_norm = 1; h.total_inner += it;
         if (!(res <= S.opts.tol)) h.error |= E_NOT_CONVERGED;
         HIPCHK(hipMemcpyAsync(sc, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
         return LFD_OK;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -211,13 +270,11 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, s));
     }
-    hipError_t e = hipErrorCooperativeLaunchTooLarge;
-    if (S.kind == LFD_SOLVER_MIXED) e = solve_lds(A, b, x, S.w, sc, S.opts, s);
-    if (e == hipErrorCooperativeLaunchTooLarge) {
-        (void)hipGetLastError();
-        e = solve_mixed(A, b, x, S.w, sc, S.opts, s);
+    if (S.kind == LFD_SOLVER_MIXED && !S.launches.empty()) {
+        for (auto& L : S.launches) HIPCHK(solve_lds(A, b, x, S.w, sc, S.jobs+L.first, L.second, S.opts, s));
+    } else {
+        HIPCHK(solve_mixed(A, b, x, S.w, sc, S.opts, s));
     }
-    HIPCHK(e);
     if (S.profiling) {
         HIPCHK(hipEventRecord(e1, s));
         S.prof.emplace_back(e0, e1);
@@ -244,16 +301,28 @@ static int collect_profile(Solver& S, hipStream_t s, double* ms_total, int* laun
     return LFD_OK;
 }
 
-// ===================================================================== System
-struct lfd_system {
-    int n = 0, pcap = 0, K = 64;
-    Params P{};
-    Consts C{};
-    CellGrid g{};
+// ===================================================================== batch of instances
+// Every array is indexed by a global particle slot, cell or pair; instance k
+// owns the slots [poff, poff + round_up(n, 128)) and the cells
+// [coff, coff + nbox).  Pairs never cross instances, so the one (p0, p1)-sorted
+// pair list is the concatenation of the instances' own sorted lists and every
+// per-instance result is independent of the other instances.
+struct InstHost {
+    bool live = false;
+    int slots = 0;
+    int have_stress = 0;
+    InstConst ic{};
+};
+
+struct lfd_batch {
+    int M = 0, ccap = 0, icap = 0, pcap = 0, K = 64;
+    int ninst = 0;                // instance handles in use (high-water mark)
+    int control = LFD_RATE_CONTROL;
     hipStream_t s = nullptr;
     double4* pos = nullptr;
     double *vel6 = nullptr, *na6 = nullptr, *pred6 = nullptr, *rhs = nullptr;
-    int *box_of = nullptr, *snap_start = nullptr, *snap_mem = nullptr, *home_start = nullptr, *home_mem = nullptr;
+    int *inst_of = nullptr, *box_of = nullptr;
+    int *snap_start = nullptr, *snap_mem = nullptr, *home_start = nullptr, *home_mem = nullptr;
     int *bin_count = nullptr, *bin_cursor = nullptr, *found = nullptr, *found_cnt = nullptr;
     PairRec* pairs[2] = {nullptr, nullptr};
     int cur = 0;
@@ -263,31 +332,107 @@ struct lfd_system {
     // velocity parts (contact, dashpot, hydro, repulsion / rate independent,
     // rate proportional) with their right-hand sides, and the stress buffers
     double *rhs_parts = nullptr, *vel_parts = nullptr, *pst = nullptr, *ps = nullptr, *st_partial = nullptr;
-    int control = LFD_RATE_CONTROL;
-    int have_stress = 0;
+    CellArrays ca{};
+    InstConst* d_ic = nullptr;
     DevScalars* sc = nullptr;
+    BatchScalars* bs = nullptr;
     Solver S;
-    ~lfd_system();
+    std::vector<InstHost> inst;
+    ~lfd_batch();
 };
 
-lfd_system::~lfd_system()
+// The single-instance System is a batch holding instance 0.
+struct lfd_system : lfd_batch {};
+
+lfd_batch::~lfd_batch()
 {
     if (s) (void)hipStreamSynchronize(s);
-    void* ptrs[] = {pos, vel6, na6, pred6, rhs, box_of, snap_start, snap_mem, home_start, home_mem, bin_count,
-                    bin_cursor, found, found_cnt, pairs[0], pairs[1], keep, keep_scan, tcnt, merge_cnt,
-                    scan_tmp, sc, g.nb, g.nnb, g.nstatic, g.ctype, probe, pdb, prhs, rhs_parts, vel_parts,
-                    pst, ps, st_partial};
+    void* ptrs[] = {pos, vel6, na6, pred6, rhs, inst_of, box_of, snap_start, snap_mem, home_start, home_mem,
+                    bin_count, bin_cursor, found, found_cnt, pairs[0], pairs[1], keep, keep_scan, tcnt, merge_cnt,
+                    scan_tmp, probe, pdb, prhs, rhs_parts, vel_parts, pst, ps, st_partial, ca.nb, ca.nnb,
+                    ca.nstatic, ca.ctype, ca.cell_inst, d_ic, sc, bs};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     S.release();
     if (s) (void)hipStreamDestroy(s);
 }
 
+static int round_up(int a, int m) { return (a+m-1)/m*m; }
+
+// Device bytes per particle slot and per cell of a batch (pair storage is
+// sized per slot), used for admission by HBM footprint.
+static void batch_unit_bytes(size_t* per_slot, size_t* per_cell)
+{
+    const size_t pp = 16;     // pair capacity per slot
+    size_t slot = sizeof(double4)+4*6*8+2*4+2*4+(size_t)64*4+4*4    // pos, vel6/na6/pred6/rhs, labels, found
+                  +8*6*8+STRESS_VALS*8                               // velocity parts, rhs parts, particle stress
+                  +(15+36)*(8+4)+6*8*2+6*4*6+6*8*4+3*4;              // solver rows and vectors
+    slot += pp*(2*sizeof(PairRec)+2*4+(30+42)*8+2*STRESS_VALS*8+18*(8+4)+3*4);
+    if (per_slot) *per_slot = slot;
+    if (per_cell) *per_cell = (size_t)(64+32+8)*4;
+}
+
+static int batch_init(lfd_batch* B, int slot_cap, int cell_cap, int inst_cap, const lfd_solve_opts* o, int control)
+{
+    lfd_solve_opts od;
+    lfd_default_solve_opts(&od);
+    if (!o) o = &od;
+    B->S.opts = to_opts(o);
+    B->S.kind = o->kind;
+    B->control = control;
+    B->M = round_up(std::max(slot_cap, 1), LDS_ROWS);
+    B->ccap = std::max(cell_cap, 1);
+    B->icap = std::max(inst_cap, 1);
+    B->pcap = 16*B->M+1024;
+    B->inst.assign(B->icap, InstHost());
+    const int M = B->M, ccap = B->ccap, pcap = B->pcap;
+    const size_t m6 = 6*(size_t)M;
+    hipError_t e = hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking);
+#define A(x) if (e == hipSuccess) e = (x)
+    A(dalloc(&B->pos, M)); A(dalloc(&B->vel6, m6)); A(dalloc(&B->na6, m6)); A(dalloc(&B->pred6, m6));
+    A(dalloc(&B->rhs, m6)); A(dalloc(&B->inst_of, M)); A(dalloc(&B->box_of, M));
+    A(dalloc(&B->snap_start, ccap+1)); A(dalloc(&B->snap_mem, M)); A(dalloc(&B->home_start, ccap+1));
+    A(dalloc(&B->home_mem, M)); A(dalloc(&B->bin_count, ccap)); A(dalloc(&B->bin_cursor, ccap));
+    A(dalloc(&B->found, (size_t)M*B->K)); A(dalloc(&B->found_cnt, M));
+    A(dalloc(&B->pairs[0], pcap)); A(dalloc(&B->pairs[1], pcap));
+    A(dalloc(&B->keep, pcap)); A(dalloc(&B->keep_scan, (size_t)std::max(pcap, M)+1)); A(dalloc(&B->tcnt, M));
+    A(dalloc(&B->merge_cnt, M)); A(dalloc(&B->scan_tmp, pcap/2048+64)); A(dalloc(&B->probe, 32*(size_t)ccap));
+    A(dalloc(&B->pdb, 30*(size_t)pcap)); A(dalloc(&B->prhs, 42*(size_t)pcap));
+    A(dalloc(&B->rhs_parts, 4*m6)); A(dalloc(&B->vel_parts, 4*m6));
+    A(dalloc(&B->pst, 2*(size_t)STRESS_VALS*pcap)); A(dalloc(&B->ps, (size_t)STRESS_VALS*M));
+    A(dalloc(&B->st_partial, (size_t)STRESS_VALS*(M/STRESS_CHUNK+1)));
+    A(dalloc(&B->ca.nb, 64*(size_t)ccap)); A(dalloc(&B->ca.nnb, ccap)); A(dalloc(&B->ca.nstatic, ccap));
+    A(dalloc(&B->ca.ctype, ccap)); A(dalloc(&B->ca.cell_inst, ccap));
+    A(dalloc(&B->d_ic, B->icap)); A(dalloc(&B->sc, B->icap)); A(dalloc(&B->bs, 1));
+    A(B->S.alloc(M, pcap));
+#undef A
+    if (e != hipSuccess) return fail(LFD_ERR_HIP, std::string("batch allocation: ")+hipGetErrorString(e));
+    B->ca.ncell = ccap;
+    hipStream_t s = B->s;
+    launch_fill_int(B->inst_of, -1, M, s);
+    launch_fill_int(B->box_of, -1, M, s);
+    launch_fill_int(B->ca.cell_inst, -1, ccap, s);
+    launch_fill_int(B->ca.nnb, 0, ccap, s);
+    launch_fill_int(B->ca.nstatic, 0, ccap, s);
+    launch_fill_int(B->ca.ctype, 0, ccap, s);
+    launch_fill_int(B->S.row_ptr, 0, M+1, s);
+    launch_fill_int(B->S.tptr, 0, M+1, s);
+    launch_fill_int((int*)B->bs, 0, sizeof(BatchScalars)/sizeof(int), s);
+    launch_fill_int((int*)B->d_ic, 0, B->icap*sizeof(InstConst)/sizeof(int), s);
+    launch_fill_int((int*)B->sc, 0, B->icap*sizeof(DevScalars)/sizeof(int), s);
+    launch_fill_double(B->vel6, 0.0, m6, s);
+    launch_fill_double(B->na6, 0.0, m6, s);
+    launch_fill_double(B->pred6, 0.0, m6, s);
+    launch_fill_double(B->vel_parts, 0.0, 4*m6, s);
+    HIPCHK(hipStreamSynchronize(s));
+    return LFD_OK;
+}
+
 // BoxSet::init (BoxSet.cpp:7-70, 170-306): grid, probes and static neighbours
-static void build_grid_host(lfd_system* y, double max_range, const double sd[3], std::vector<int>& nb,
+// of one instance (local cell labels)
+static void build_grid_host(InstConst& g, double max_range, const double sd[3], std::vector<int>& nb,
                             std::vector<int>& nnb, std::vector<int>& ctype)
 {
-    CellGrid& g = y->g;
-    const double* L = y->C.L;
+    const double* L = g.C.L;
     unsigned nx = (unsigned)(L[0]/max_range), ny = (unsigned)(L[1]/max_range), nz = (unsigned)(L[2]/max_range);
     if (nx == 0) nx = 1;
     if (ny == 0) ny = 1;
@@ -346,128 +491,18 @@ static void build_grid_host(lfd_system* y, double max_range, const double sd[3],
     }
 }
 
-static int sys_bin_snapshot(lfd_system* y)
+// Constants, cell grid and static neighbour lists (local cell labels, at the
+// initial shear displacement sd) of a prospective instance.
+static int describe_instance(int np, const double* radius, const double L[3], const lfd_params* p,
+                             const double sd[3], InstConst& ic, std::vector<int>& nb, std::vector<int>& nnb,
+                             std::vector<int>& ctype)
 {
-    // BoxSet::update (BoxSet.cpp:498-506): moving neighbours, then the
-    // neighbourhood snapshot (cell membership at this instant)
-    launch_moving_neighbors(y->g, y->sc, y->C.L, y->probe, y->sc, y->s);
-    launch_bin(y->box_of, y->n, y->g.nbox, y->bin_count, y->snap_start, y->snap_mem, y->bin_cursor, y->scan_tmp, y->s);
-    return LFD_OK;
-}
-
-static int sys_check_new(lfd_system* y)
-{
-    // System::checkNewInteraction (System.cpp:1328-1353)
-    launch_bin(y->box_of, y->n, y->g.nbox, y->bin_count, y->home_start, y->home_mem, y->bin_cursor, y->scan_tmp, y->s);
-    launch_find_pairs(y->pos, y->n, y->g, y->home_start, y->home_mem, y->snap_start, y->snap_mem, y->sc, y->C,
-                      y->found, y->found_cnt, y->K, y->sc, y->s);
-    PairRec* oldp = y->pairs[y->cur];
-    PairRec* newp = y->pairs[y->cur^1];
-    launch_merge_pairs(y->n, oldp, y->S.p1, y->S.row_ptr, y->found, y->found_cnt, y->K, y->merge_cnt, y->keep_scan,
-                       y->keep, newp, y->pos, y->C, y->sc, y->pcap, y->scan_tmp, y->s);
-    y->cur ^= 1;
-    launch_pair_index(newp, y->n, y->S.row_ptr, y->S.p0, y->S.p1, y->S.tptr, y->S.tidx, y->tcnt, y->keep, y->sc,
-                      y->pcap, y->scan_tmp, y->s);
-    return LFD_OK;
-}
-
-static int sys_update(lfd_system* y, int in_predictor, double dt_override)
-{
-    // System::updateInteractions (System.cpp:1375-1403)
-    PairRec* cp = y->pairs[y->cur];
-    PairRec* np_ = y->pairs[y->cur^1];
-    launch_update_pairs(cp, y->pos, y->vel6, y->C, y->sc, y->keep, y->pcap, in_predictor, dt_override, y->s);
-    launch_compact_pairs(cp, np_, y->keep, y->keep_scan, y->n, y->S.row_ptr, y->S.p0, y->S.p1, y->S.tptr,
-                         y->S.tidx, y->tcnt, y->sc, y->pcap, y->scan_tmp, y->s);
-    y->cur ^= 1;
-    return LFD_OK;
-}
-
-static int sys_assemble(lfd_system* y, int nparts = 0)
-{
-    launch_assemble(y->pairs[y->cur], y->pos, y->n, y->C, y->sc, y->S.row_ptr, y->S.tptr, y->S.tidx, y->S.diag64,
-                    y->S.diag32, y->S.od64, y->S.od32, y->S.minv64, y->S.minv32, y->rhs, y->pdb, y->prhs, y->pcap,
-                    y->rhs_parts, nparts, y->sc, y->s);
-    return LFD_OK;
-}
-
-// calcStressPerParticle + calcStress (mode 1) or computeShearRate (mode 0)
-static void sys_stress(lfd_system* y, int nparts, int mode)
-{
-    launch_stress(y->pairs[y->cur], y->pos, y->C, y->sc, y->vel_parts, nparts, y->vel6, y->n, y->control, mode,
-                  y->S.row_ptr, y->S.tptr, y->S.tidx, y->pst, y->ps, y->st_partial, y->pcap, y->s);
-}
-
-static int sys_compute_velocities(lfd_system* y, int in_predictor, int calc_stress = 0)
-{
-    // System::computeVelocities (System.cpp:2273-2319)
-    if (y->control == LFD_RATE_CONTROL && !calc_stress) {
-        sys_assemble(y);
-        int rc = run_solve(y->S, &y->sc->npairs, y->rhs, y->na6, y->sc, y->s);
-        if (rc) return rc;
-        launch_finish_velocity(y->n, y->pos, y->na6, y->vel6, y->C, y->sc, in_predictor, y->s);
-        return LFD_OK;
-    }
-    // velocities by parts (computeVelocityByComponents, System.cpp:1947-1965):
-    // four components when the stress is wanted, else (stress control) the
-    // rate independent and rate proportional sums
-    const int nparts = calc_stress ? 4 : 2;
-    const size_t m6 = 6*(size_t)y->n;
-    if (y->control == LFD_STRESS_CONTROL) launch_set_rate(y->sc, y->C, 1.0, y->s);
-    sys_assemble(y, nparts);
-    for (int p = 0; p < nparts; p++) {
-        int rc = run_solve(y->S, &y->sc->npairs, y->rhs_parts+p*m6, y->vel_parts+p*m6, y->sc, y->s);
-        if (rc) return rc;
-    }
-    if (y->control == LFD_STRESS_CONTROL) sys_stress(y, nparts, 0);     // new shear rate
-    launch_sum_parts(y->vel_parts, nparts, y->na6, y->n, y->sc, y->control == LFD_STRESS_CONTROL, y->s);
-    launch_finish_velocity(y->n, y->pos, y->na6, y->vel6, y->C, y->sc, in_predictor, y->s);
-    if (calc_stress) {
-        sys_stress(y, nparts, 1);
-        y->have_stress = 1;
-    }
-    return LFD_OK;
-}
-
-static int sys_step(lfd_system* y, double strain_end, int calc_stress = 0, int adapt_first = 1)
-{
-    // System::timeEvolution loop body + timeEvolutionPredictorCorrectorMethod
-    // (System.cpp:1241-1256, 888-983, 1124-1187); the stress, when wanted, is
-    // the corrector's
-    int rc;
-    if (adapt_first) launch_adapt_dt(y->sc, y->P, strain_end, 0, y->s);
-    if ((rc = sys_compute_velocities(y, 1))) return rc;
-    launch_adapt_dt(y->sc, y->P, strain_end, 1, y->s);
-    launch_boxing(y->sc, y->C, y->s);
-    sys_bin_snapshot(y);
-    launch_displace(y->pos, y->vel6, y->pred6, y->n, y->C, y->sc, y->g, y->box_of, 0, y->s);
-    sys_update(y, 1, -1.0);
-    launch_copy_double(y->pred6, y->vel6, 6*(size_t)y->n, y->s);
-    if ((rc = sys_compute_velocities(y, 0, calc_stress))) return rc;
-    launch_displace(y->pos, y->vel6, y->pred6, y->n, y->C, y->sc, y->g, y->box_of, 1, y->s);
-    sys_check_new(y);
-    sys_update(y, 0, -1.0);
-    return LFD_OK;
-}
-
-extern "C" int lfd_system_create(int np, const double* pos, const double* radius, const double L[3],
-                                 const double shear_disp[3], const lfd_params* p, const lfd_solve_opts* o,
-                                 lfd_system** out)
-{
-    if (!out || np < 2 || !pos || !radius || !L || !p) return fail(LFD_ERR_ARG, "lfd_system_create: bad argument");
-    *out = nullptr;
-    lfd_system* y = new lfd_system();
-    y->n = np;
-    y->P = to_params(p);
-    y->C = make_consts(y->P, L);
-    lfd_solve_opts od;
-    lfd_default_solve_opts(&od);
-    if (!o) o = &od;
-    y->S.opts = to_opts(o);
-    y->S.kind = o->kind;
-    y->pcap = 24*np+1024;
-    double sd[3] = {0, 0, 0};
-    if (shear_disp) for (int k = 0; k < 3; k++) sd[k] = shear_disp[k];
+    if (np < 2 || !radius || !L || !p) return fail(LFD_ERR_ARG, "instance: bad argument");
+    for (int k = 0; k < 3; k++)
+        if (!(L[k] > 0)) return fail(LFD_ERR_ARG, "instance: box lengths must be positive");
+    std::memset(&ic, 0, sizeof(ic));
+    ic.P = to_params(p);
+    ic.C = make_consts(ic.P, L);
     // max interaction range over the (radius_i, radius_j), i < j, that occur
     // (System::initializeBoxing, System.cpp:613-652)
     std::vector<double> uniq(radius, radius+np);
@@ -483,66 +518,410 @@ extern "C" int lfd_system_create(int np, const double* pos, const double* radius
     for (size_t a = 0; a < uniq.size(); a++)
         for (size_t b = 0; b < uniq.size(); b++) {
             if (!(first[a] < last[b])) continue;     // some i (radius a) < j (radius b)
-            double r = lub_range(uniq[a], uniq[b], y->P.lub_max_gap);
+            double r = lub_range(uniq[a], uniq[b], ic.P.lub_max_gap);
             if (r > max_range) max_range = r;
         }
-    std::vector<int> nb, nnb, ctype;
-    build_grid_host(y, max_range, sd, nb, nnb, ctype);
-    hipError_t e = hipStreamCreateWithFlags(&y->s, hipStreamNonBlocking);
-    int n = np, nbox = y->g.nbox, pcap = y->pcap;
-    size_t m6 = 6*(size_t)n;
-#define A(x) if (e == hipSuccess) e = (x)
-    A(dalloc(&y->pos, n)); A(dalloc(&y->vel6, m6)); A(dalloc(&y->na6, m6)); A(dalloc(&y->pred6, m6));
-    A(dalloc(&y->rhs, m6)); A(dalloc(&y->box_of, n)); A(dalloc(&y->snap_start, nbox+1));
-    A(dalloc(&y->snap_mem, n)); A(dalloc(&y->home_start, nbox+1)); A(dalloc(&y->home_mem, n));
-    A(dalloc(&y->bin_count, nbox)); A(dalloc(&y->bin_cursor, nbox)); A(dalloc(&y->found, (size_t)n*y->K));
-    A(dalloc(&y->found_cnt, n)); A(dalloc(&y->pairs[0], pcap)); A(dalloc(&y->pairs[1], pcap));
-    A(dalloc(&y->keep, pcap)); A(dalloc(&y->keep_scan, (size_t)std::max(pcap, n)+1)); A(dalloc(&y->tcnt, n));
-    A(dalloc(&y->merge_cnt, n)); A(dalloc(&y->scan_tmp, 4096)); A(dalloc(&y->sc, 1));
-    A(dalloc(&y->g.nb, 64*(size_t)nbox)); A(dalloc(&y->g.nnb, nbox)); A(dalloc(&y->g.nstatic, nbox));
-    A(dalloc(&y->g.ctype, nbox)); A(dalloc(&y->probe, 32*(size_t)nbox));
-    A(dalloc(&y->pdb, 30*(size_t)pcap)); A(dalloc(&y->prhs, 42*(size_t)pcap));
-    A(dalloc(&y->rhs_parts, 4*m6)); A(dalloc(&y->vel_parts, 4*m6));
-    A(dalloc(&y->pst, 2*(size_t)STRESS_VALS*pcap)); A(dalloc(&y->ps, (size_t)STRESS_VALS*n));
-    A(dalloc(&y->st_partial, (size_t)STRESS_VALS*((n+STRESS_CHUNK-1)/STRESS_CHUNK)));
-    A(y->S.alloc(n, pcap));
-#undef A
-    if (e != hipSuccess) {
-        delete y;
-        return fail(LFD_ERR_HIP, std::string("lfd_system_create: ") + hipGetErrorString(e));
+    build_grid_host(ic, max_range, sd, nb, nnb, ctype);
+    ic.n = np;
+    return LFD_OK;
+}
+
+// First fit of `len` units among the live ranges (start, length) in [0, cap).
+static int first_fit(std::vector<std::pair<int, int>> used, int cap, int len)
+{
+    std::sort(used.begin(), used.end());
+    int at = 0;
+    for (auto& u : used) {
+        if (u.first-at >= len) return at;
+        at = std::max(at, u.first+u.second);
     }
-    std::vector<double4> h4(n);
-    for (int i = 0; i < n; i++) h4[i] = make_double4(pos[3*i], pos[3*i+1], pos[3*i+2], radius[i]);
+    return cap-at >= len ? at : -1;
+}
+
+static std::vector<InstSpan> live_spans(const lfd_batch* B)
+{
+    std::vector<InstSpan> v;
+    for (int k = 0; k < B->ninst; k++)
+        if (B->inst[k].live) v.push_back({k, B->inst[k].ic.poff, B->inst[k].ic.n});
+    std::sort(v.begin(), v.end(), [](const InstSpan& a, const InstSpan& b) { return a.row0 < b.row0; });
+    return v;
+}
+
+static int batch_check(lfd_batch* B, int id, DevScalars* out = nullptr)
+{
+    BatchScalars hb;
+    HIPCHK(hipMemcpyAsync(&hb, B->bs, sizeof(hb), hipMemcpyDeviceToHost, B->s));
+    std::vector<DevScalars> h(std::max(B->ninst, 1));
+    if (B->ninst) HIPCHK(hipMemcpyAsync(h.data(), B->sc, B->ninst*sizeof(DevScalars), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    if (out && id >= 0 && id < B->ninst) *out = h[id];
+    int rc = status_of(hb.error);
+    if (rc) return rc;
+    for (int k = 0; k < B->ninst; k++) {
+        if (!B->inst[k].live || (id >= 0 && k != id)) continue;
+        if ((rc = status_of(h[k].error))) return rc;
+    }
+    return LFD_OK;
+}
+
+static void bat_bin_snapshot(lfd_batch* B)
+{
+    // BoxSet::update (BoxSet.cpp:498-506): moving neighbours, then the
+    // neighbourhood snapshot (cell membership at this instant)
+    launch_moving_neighbors(B->ca, B->d_ic, B->sc, B->probe, B->s);
+    launch_bin(B->box_of, B->M, B->ccap, B->bin_count, B->snap_start, B->snap_mem, B->bin_cursor, B->scan_tmp, B->s);
+}
+
+static void bat_check_new(lfd_batch* B, int only = -1)
+{
+    // System::checkNewInteraction (System.cpp:1328-1353)
+    launch_bin(B->box_of, B->M, B->ccap, B->bin_count, B->home_start, B->home_mem, B->bin_cursor, B->scan_tmp, B->s);
+    launch_find_pairs(B->pos, B->M, B->ca, B->d_ic, B->home_start, B->home_mem, B->snap_start, B->snap_mem, B->sc,
+                      B->found, B->found_cnt, B->K, only, B->s);
+    PairRec* oldp = B->pairs[B->cur];
+    PairRec* newp = B->pairs[B->cur^1];
+    launch_merge_pairs(B->M, oldp, B->S.p1, B->S.row_ptr, B->found, B->found_cnt, B->K, B->merge_cnt, B->keep_scan,
+                       B->keep, newp, B->pos, B->inst_of, B->d_ic, B->sc, B->bs, B->pcap, B->scan_tmp, B->s);
+    B->cur ^= 1;
+    launch_pair_index(newp, B->M, B->S.row_ptr, B->S.p0, B->S.p1, B->S.tptr, B->S.tidx, B->tcnt, B->keep, B->bs,
+                      B->pcap, B->scan_tmp, B->s);
+}
+
+static void bat_update(lfd_batch* B, int in_predictor, double dt_override, int only = -1)
+{
+    // System::updateInteractions (System.cpp:1375-1403)
+    PairRec* cp = B->pairs[B->cur];
+    PairRec* np_ = B->pairs[B->cur^1];
+    launch_update_pairs(cp, B->pos, B->vel6, B->inst_of, B->d_ic, B->sc, B->ninst, B->bs, B->keep, B->pcap,
+                        in_predictor, dt_override, only, B->s);
+    launch_compact_pairs(cp, np_, B->keep, B->keep_scan, B->M, B->S.row_ptr, B->S.p0, B->S.p1, B->S.tptr,
+                         B->S.tidx, B->tcnt, B->bs, B->pcap, B->scan_tmp, B->s);
+    B->cur ^= 1;
+}
+
+static void bat_assemble(lfd_batch* B, int nparts = 0)
+{
+    launch_assemble(B->pairs[B->cur], B->pos, B->M, B->inst_of, B->d_ic, B->sc, B->bs, B->S.row_ptr, B->S.tptr,
+                    B->S.tidx, B->S.diag64, B->S.diag32, B->S.od64, B->S.od32, B->S.minv64, B->S.minv32, B->rhs,
+                    B->pdb, B->prhs, B->pcap, B->rhs_parts, nparts, B->s);
+}
+
+// calcStressPerParticle + calcStress (mode 1) or computeShearRate (mode 0)
+static void bat_stress(lfd_batch* B, int nparts, int mode)
+{
+    launch_stress(B->pairs[B->cur], B->pos, B->inst_of, B->d_ic, B->ninst, B->sc, B->bs, B->vel_parts, nparts,
+                  B->vel6, B->M, B->control, mode, B->S.row_ptr, B->S.tptr, B->S.tidx, B->pst, B->ps,
+                  B->st_partial, B->pcap, B->s);
+}
+
+static int bat_velocities(lfd_batch* B, int in_predictor, int calc_stress = 0)
+{
+    // System::computeVelocities (System.cpp:2273-2319)
+    if (B->control == LFD_RATE_CONTROL && !calc_stress) {
+        bat_assemble(B);
+        int rc = run_solve(B->S, &B->bs->npairs, B->rhs, B->na6, B->sc, B->s);
+        if (rc) return rc;
+        launch_finish_velocity(B->M, B->pos, B->na6, B->vel6, B->inst_of, B->sc, B->ninst, in_predictor, B->s);
+        return LFD_OK;
+    }
+    // velocities by parts (computeVelocityByComponents, System.cpp:1947-1965):
+    // four components when the stress is wanted, else (stress control) the
+    // rate independent and rate proportional sums
+    const int nparts = calc_stress ? 4 : 2;
+    const size_t m6 = 6*(size_t)B->M;
+    if (B->control == LFD_STRESS_CONTROL) launch_set_rate(B->sc, B->d_ic, B->ninst, 1.0, B->s);
+    bat_assemble(B, nparts);
+    for (int p = 0; p < nparts; p++) {
+        int rc = run_solve(B->S, &B->bs->npairs, B->rhs_parts+p*m6, B->vel_parts+p*m6, B->sc, B->s);
+        if (rc) return rc;
+    }
+    if (B->control == LFD_STRESS_CONTROL) bat_stress(B, nparts, 0);     // new shear rate
+    launch_sum_parts(B->vel_parts, nparts, B->na6, B->M, B->inst_of, B->sc, B->control == LFD_STRESS_CONTROL, B->s);
+    launch_finish_velocity(B->M, B->pos, B->na6, B->vel6, B->inst_of, B->sc, B->ninst, in_predictor, B->s);
+    if (calc_stress) {
+        bat_stress(B, nparts, 1);
+        for (auto& h : B->inst) if (h.live) h.have_stress = 1;
+    }
+    return LFD_OK;
+}
+
+static int bat_step(lfd_batch* B, int calc_stress = 0, int adapt_first = 1)
+{
+    // System::timeEvolution loop body + timeEvolutionPredictorCorrectorMethod
+    // (System.cpp:1241-1256, 888-983, 1124-1187); the stress, when wanted, is
+    // the corrector's
+    int rc;
+    if (adapt_first) launch_adapt_dt(B->sc, B->d_ic, B->ninst, 0, B->s);
+    if ((rc = bat_velocities(B, 1))) return rc;
+    launch_adapt_dt(B->sc, B->d_ic, B->ninst, 1, B->s);
+    launch_boxing(B->sc, B->d_ic, B->ninst, B->s);
+    bat_bin_snapshot(B);
+    launch_displace(B->pos, B->vel6, B->pred6, B->M, B->inst_of, B->d_ic, B->sc, B->box_of, 0, B->s);
+    bat_update(B, 1, -1.0);
+    launch_copy_double(B->pred6, B->vel6, 6*(size_t)B->M, B->s);
+    if ((rc = bat_velocities(B, 0, calc_stress))) return rc;
+    launch_displace(B->pos, B->vel6, B->pred6, B->M, B->inst_of, B->d_ic, B->sc, B->box_of, 1, B->s);
+    bat_check_new(B);
+    bat_update(B, 0, -1.0);
+    return LFD_OK;
+}
+
+// Admission of one instance: slots, cells and a handle are taken first fit;
+// the particles are binned and the instance's pairs found (System::setupSystem,
+// System.cpp:468-611), touching no other instance's pairs.  first_update also
+// runs the first-call block of System::timeEvolution (System.cpp:1219-1228).
+static int bat_add(lfd_batch* B, int np, const double* pos, const double* radius, const double L[3],
+                   const double shear_disp[3], const lfd_params* p, double strain_end, int first_update, int* id_out)
+{
+    if (!pos) return fail(LFD_ERR_ARG, "instance: bad argument");
+    double sd[3] = {0, 0, 0};
+    if (shear_disp) for (int k = 0; k < 3; k++) sd[k] = shear_disp[k];
+    InstConst ic;
+    std::vector<int> nb, nnb, ctype;
+    int rc = describe_instance(np, radius, L, p, sd, ic, nb, nnb, ctype);
+    if (rc) return rc;
+    const int slots = round_up(np, LDS_ROWS);
+    std::vector<std::pair<int, int>> us, uc;
+    int id = -1;
+    for (int k = 0; k < B->icap; k++) {
+        if (B->inst[k].live) {
+            us.emplace_back(B->inst[k].ic.poff, B->inst[k].slots);
+            uc.emplace_back(B->inst[k].ic.coff, B->inst[k].ic.nbox);
+        } else if (id < 0) id = k;
+    }
+    const int poff = first_fit(us, B->M, slots);
+    const int coff = first_fit(uc, B->ccap, ic.nbox);
+    if (id < 0 || poff < 0 || coff < 0) return fail(LFD_ERR_CAPACITY, "batch: no room for the instance");
+    ic.poff = poff;
+    ic.coff = coff;
+    // the solver must be able to take the new set of instances
+    std::vector<InstSpan> spans = live_spans(B);
+    spans.push_back({id, poff, np});
+    std::sort(spans.begin(), spans.end(), [](const InstSpan& a, const InstSpan& b) { return a.row0 < b.row0; });
+    if ((rc = set_jobs(B->S, spans, B->s))) {
+        set_jobs(B->S, live_spans(B), B->s);
+        return rc;
+    }
+    hipStream_t s = B->s;
+    std::vector<double4> h4(np);
+    for (int i = 0; i < np; i++) h4[i] = make_double4(pos[3*i], pos[3*i+1], pos[3*i+2], radius[i]);
+    for (auto& v : nb) v += coff;
     DevScalars hs;
     std::memset(&hs, 0, sizeof(hs));
     hs.dt = 1e-4;
     for (int k = 0; k < 3; k++) hs.shear_disp[k] = sd[k];
-    make_flow(y->C, y->P.shear_rate, hs.flow);
+    make_flow(ic.C, ic.P.shear_rate, hs.flow);
     hs.init_strain_limit = 0;       // System.cpp:90-91
     hs.init_rate_limit = 999;
-    int rc = LFD_OK;
-    auto up = [&](void* d, const void* h, size_t b) {
-        if (e == hipSuccess) e = hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, y->s);
-    };
-    up(y->pos, h4.data(), n*sizeof(double4));
-    up(y->sc, &hs, sizeof(hs));
-    up(y->g.nb, nb.data(), nb.size()*sizeof(int));
-    up(y->g.nnb, nnb.data(), nnb.size()*sizeof(int));
-    up(y->g.nstatic, nnb.data(), nnb.size()*sizeof(int));
-    up(y->g.ctype, ctype.data(), ctype.size()*sizeof(int));
-    if (e != hipSuccess) {
-        delete y;
-        return fail(LFD_ERR_HIP, std::string("lfd_system_create: ") + hipGetErrorString(e));
+    hs.strain_end = strain_end;
+    const size_t o6 = 6*(size_t)poff, s6 = 6*(size_t)slots;
+    HIPCHK(hipMemcpyAsync(B->pos+poff, h4.data(), np*sizeof(double4), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->sc+id, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->d_ic+id, &ic, sizeof(ic), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->ca.nb+64*(size_t)coff, nb.data(), nb.size()*sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->ca.nnb+coff, nnb.data(), nnb.size()*sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->ca.nstatic+coff, nnb.data(), nnb.size()*sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B->ca.ctype+coff, ctype.data(), ctype.size()*sizeof(int), hipMemcpyHostToDevice, s));
+    launch_fill_int(B->ca.cell_inst+coff, id, ic.nbox, s);
+    launch_fill_int(B->inst_of+poff, id, np, s);
+    launch_fill_double(B->vel6+o6, 0.0, s6, s);
+    launch_fill_double(B->na6+o6, 0.0, s6, s);
+    launch_fill_double(B->pred6+o6, 0.0, s6, s);
+    for (int q = 0; q < 4; q++) launch_fill_double(B->vel_parts+q*6*(size_t)B->M+o6, 0.0, s6, s);
+    InstHost& h = B->inst[id];
+    h.live = true;
+    h.slots = slots;
+    h.have_stress = 0;
+    h.ic = ic;
+    B->ninst = std::max(B->ninst, id+1);
+    launch_box_labels(B->pos+poff, B->inst_of+poff, np, B->d_ic, B->box_of+poff, B->sc, s);
+    bat_bin_snapshot(B);
+    bat_check_new(B, id);
+    if (first_update) {
+        bat_check_new(B, id);
+        bat_update(B, 1, 0.0, id);
     }
-    launch_fill_double(y->vel6, 0.0, m6, y->s);
-    launch_fill_double(y->na6, 0.0, m6, y->s);
-    launch_fill_double(y->pred6, 0.0, m6, y->s);
-    launch_fill_double(y->vel_parts, 0.0, 4*m6, y->s);
-    launch_fill_int(y->S.row_ptr, 0, n+1, y->s);
-    launch_box_labels(y->pos, n, y->g, y->box_of, y->sc, y->s);
-    sys_bin_snapshot(y);
-    sys_check_new(y);
-    rc = check_scalars(y->sc, y->s);
+    if ((rc = batch_check(B, id))) return rc;
+    if (id_out) *id_out = id;
+    return LFD_OK;
+}
+
+// Retirement: the instance's slots and cells are freed and its pairs dropped;
+// the other instances' pairs keep their order.
+static int bat_remove(lfd_batch* B, int id)
+{
+    if (id < 0 || id >= B->ninst || !B->inst[id].live) return fail(LFD_ERR_ARG, "batch: no such instance");
+    InstHost& h = B->inst[id];
+    hipStream_t s = B->s;
+    launch_fill_int(B->inst_of+h.ic.poff, -1, h.slots, s);
+    launch_fill_int(B->box_of+h.ic.poff, -1, h.slots, s);
+    launch_fill_int(B->ca.cell_inst+h.ic.coff, -1, h.ic.nbox, s);
+    launch_fill_int(B->ca.nnb+h.ic.coff, 0, h.ic.nbox, s);
+    launch_fill_int((int*)(B->d_ic+id), 0, sizeof(InstConst)/sizeof(int), s);
+    bat_update(B, 0, 0.0, -2);
+    h = InstHost();
+    while (B->ninst > 0 && !B->inst[B->ninst-1].live) B->ninst--;
+    int rc = set_jobs(B->S, live_spans(B), s);
+    if (rc) return rc;
+    BatchScalars hb;
+    HIPCHK(hipMemcpyAsync(&hb, B->bs, sizeof(hb), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return status_of(hb.error);
+}
+
+// ---------------------------------------------------------------- per-instance views
+static const InstHost* live_inst(lfd_batch* B, int id)
+{
+    if (!B || id < 0 || id >= B->ninst || !B->inst[id].live) {
+        fail(LFD_ERR_ARG, "no such instance");
+        return nullptr;
+    }
+    return &B->inst[id];
+}
+
+// pair range [k0, k1) of an instance in the global sorted list
+static int pair_range(lfd_batch* B, const InstHost& h, int& k0, int& k1)
+{
+    HIPCHK(hipMemcpyAsync(&k0, B->S.row_ptr+h.ic.poff, sizeof(int), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipMemcpyAsync(&k1, B->S.row_ptr+h.ic.poff+h.ic.n, sizeof(int), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    return LFD_OK;
+}
+
+static int inst_num_pairs(lfd_batch* B, int id)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h) return LFD_ERR_ARG;
+    int rc = batch_check(B, id);
+    if (rc) return rc;
+    int k0 = 0, k1 = 0;
+    if ((rc = pair_range(B, *h, k0, k1))) return rc;
+    return k1-k0;
+}
+
+static int inst_get_pairs(lfd_batch* B, int id, int cap, int* p0, int* p1, int* state, double* geom5,
+                          double* disp_tan)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h) return LFD_ERR_ARG;
+    int rc = batch_check(B, id);
+    if (rc) return rc;
+    int k0 = 0, k1 = 0;
+    if ((rc = pair_range(B, *h, k0, k1))) return rc;
+    const int np_ = k1-k0, off = h->ic.poff;
+    if (np_ > cap) return fail(LFD_ERR_ARG, "get_pairs: cap too small");
+    std::vector<PairRec> v(np_);
+    if (np_) HIPCHK(hipMemcpy(v.data(), B->pairs[B->cur]+k0, np_*sizeof(PairRec), hipMemcpyDeviceToHost));
+    for (int k = 0; k < np_; k++) {
+        const PairRec& q = v[k];
+        if (p0) p0[k] = q.p0-off;
+        if (p1) p1[k] = q.p1-off;
+        if (state) state[k] = pair_state(q.flags);
+        if (geom5) {
+            geom5[5*k] = q.n[0]; geom5[5*k+1] = q.n[1]; geom5[5*k+2] = q.n[2];
+            geom5[5*k+3] = q.r; geom5[5*k+4] = q.gap;
+        }
+        if (disp_tan) for (int c = 0; c < 3; c++) disp_tan[3*k+c] = q.disp_tan[c];
+    }
+    return np_;
+}
+
+static void fill_scalars8(const DevScalars& h, double* sc8)
+{
+    sc8[0] = h.shear_disp[0]; sc8[1] = h.shear_disp[1]; sc8[2] = h.shear_disp[2];
+    sc8[3] = h.dt; sc8[4] = h.time; sc8[5] = h.strain;
+    sc8[6] = std::sqrt(__builtin_bit_cast(double, h.max_vel_bits));
+    sc8[7] = std::sqrt(__builtin_bit_cast(double, h.max_slide_bits));
+}
+
+static int inst_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* sc8)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h) return LFD_ERR_ARG;
+    DevScalars hs;
+    int rc = batch_check(B, id, &hs);
+    if (rc) return rc;
+    const int n = h->ic.n, off = h->ic.poff;
+    if (pos) {
+        std::vector<double4> v(n);
+        HIPCHK(hipMemcpy(v.data(), B->pos+off, n*sizeof(double4), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) { pos[3*i] = v[i].x; pos[3*i+1] = v[i].y; pos[3*i+2] = v[i].z; }
+    }
+    if (vel6) HIPCHK(hipMemcpy(vel6, B->vel6+6*(size_t)off, 6*(size_t)n*sizeof(double), hipMemcpyDeviceToHost));
+    if (na6) HIPCHK(hipMemcpy(na6, B->na6+6*(size_t)off, 6*(size_t)n*sizeof(double), hipMemcpyDeviceToHost));
+    if (sc8) fill_scalars8(hs, sc8);
+    return LFD_OK;
+}
+
+static int inst_get_stress(lfd_batch* B, int id, double* groups24, double* total6, double* shear_rate,
+                           double* viscosity, double* particle6)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h) return LFD_ERR_ARG;
+    if (!h->have_stress) return fail(LFD_ERR_ARG, "get_stress: no stress computed yet");
+    DevScalars hs;
+    int rc = batch_check(B, id, &hs);
+    if (rc) return rc;
+    if (groups24) std::memcpy(groups24, hs.stress_groups, 24*sizeof(double));
+    if (total6) std::memcpy(total6, hs.stress_total, 6*sizeof(double));
+    if (shear_rate) *shear_rate = hs.flow.rate;
+    if (viscosity) {
+        // Simulation::outputData (Simulation.cpp:662-666)
+        const double* E = hs.flow.E;
+        if (hs.flow.rate == 0) E = h->ic.C.Eh;
+        auto dd = [](const double* a, const double* b) {
+            return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5];
+        };
+        *viscosity = 0.5*dd(hs.stress_total, E)/dd(E, E);
+    }
+    if (particle6) {
+        const int n = h->ic.n;
+        std::vector<double> ps((size_t)STRESS_VALS*n);
+        HIPCHK(hipMemcpyAsync(ps.data(), B->ps+(size_t)STRESS_VALS*h->ic.poff, ps.size()*sizeof(double),
+                              hipMemcpyDeviceToHost, B->s));
+        HIPCHK(hipStreamSynchronize(B->s));
+        for (int i = 0; i < n; i++)
+            for (int m = 0; m < 6; m++) {
+                double t = 0;
+                for (int sl = 0; sl < STRESS_VALS/6; sl++) t += ps[(size_t)STRESS_VALS*i+6*sl+m];
+                particle6[6*(size_t)i+m] = t;
+            }
+    }
+    return LFD_OK;
+}
+
+static int inst_set_scalars(lfd_batch* B, int id, const double* target_stress, const double* strain_limit,
+                            const double* rate_limit, const double* strain_end)
+{
+    if (!live_inst(B, id)) return LFD_ERR_ARG;
+    DevScalars h;
+    HIPCHK(hipMemcpyAsync(&h, B->sc+id, sizeof(h), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    if (target_stress) h.target_stress = *target_stress;
+    if (strain_limit) h.init_strain_limit = *strain_limit;
+    if (rate_limit) h.init_rate_limit = *rate_limit;
+    if (strain_end) h.strain_end = *strain_end;
+    HIPCHK(hipMemcpyAsync(B->sc+id, &h, sizeof(h), hipMemcpyHostToDevice, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    return LFD_OK;
+}
+
+// ===================================================================== System (a batch of one)
+extern "C" int lfd_system_create(int np, const double* pos, const double* radius, const double L[3],
+                                 const double shear_disp[3], const lfd_params* p, const lfd_solve_opts* o,
+                                 lfd_system** out)
+{
+    if (!out || np < 2 || !pos || !radius || !L || !p) return fail(LFD_ERR_ARG, "lfd_system_create: bad argument");
+    *out = nullptr;
+    InstConst ic;
+    std::vector<int> nb, nnb, ctype;
+    const double sd0[3] = {0, 0, 0};
+    int rc = describe_instance(np, radius, L, p, shear_disp ? shear_disp : sd0, ic, nb, nnb, ctype);
+    if (rc) return rc;
+    lfd_system* y = new lfd_system();
+    rc = batch_init(y, np, ic.nbox, 1, o, LFD_RATE_CONTROL);
+    int id = -1;
+    if (!rc) rc = bat_add(y, np, pos, radius, L, shear_disp, p, -1.0, 0, &id);
     if (rc) { delete y; return rc; }
     *out = y;
     return LFD_OK;
@@ -569,19 +948,20 @@ extern "C" int lfd_system_set_opts(lfd_system* y, const lfd_solve_opts* o)
 extern "C" int lfd_system_first_update(lfd_system* y)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    sys_check_new(y);
-    sys_update(y, 1, 0.0);
-    return check_scalars(y->sc, y->s);
+    bat_check_new(y);
+    bat_update(y, 1, 0.0);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_step(lfd_system* y, int nsteps, double strain_end)
 {
     if (!y || nsteps < 0) return fail(LFD_ERR_ARG, "lfd_system_step: bad argument");
+    launch_set_strain_end(y->sc, y->ninst, strain_end, y->s);
     for (int k = 0; k < nsteps; k++) {
-        int rc = sys_step(y, strain_end);
+        int rc = bat_step(y);
         if (rc) return rc;
     }
-    return check_scalars(y->sc, y->s);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_set_control(lfd_system* y, int control, double target_stress)
@@ -589,101 +969,65 @@ extern "C" int lfd_system_set_control(lfd_system* y, int control, double target_
     if (!y || (control != LFD_RATE_CONTROL && control != LFD_STRESS_CONTROL))
         return fail(LFD_ERR_ARG, "lfd_system_set_control: bad argument");
     y->control = control;
-    DevScalars h;
-    HIPCHK(hipMemcpyAsync(&h, y->sc, sizeof(h), hipMemcpyDeviceToHost, y->s));
-    HIPCHK(hipStreamSynchronize(y->s));
-    h.target_stress = target_stress;
-    HIPCHK(hipMemcpyAsync(y->sc, &h, sizeof(h), hipMemcpyHostToDevice, y->s));
-    HIPCHK(hipStreamSynchronize(y->s));
-    return LFD_OK;
+    return inst_set_scalars(y, 0, &target_stress, nullptr, nullptr, nullptr);
 }
 
 extern "C" int lfd_system_set_shear_rate_limit(lfd_system* y, double init_strain_limit, double init_rate_limit)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    DevScalars h;
-    HIPCHK(hipMemcpyAsync(&h, y->sc, sizeof(h), hipMemcpyDeviceToHost, y->s));
-    HIPCHK(hipStreamSynchronize(y->s));
-    h.init_strain_limit = init_strain_limit;
-    h.init_rate_limit = init_rate_limit;
-    HIPCHK(hipMemcpyAsync(y->sc, &h, sizeof(h), hipMemcpyHostToDevice, y->s));
-    HIPCHK(hipStreamSynchronize(y->s));
-    return LFD_OK;
+    return inst_set_scalars(y, 0, nullptr, &init_strain_limit, &init_rate_limit, nullptr);
 }
 
 extern "C" int lfd_system_step_stress(lfd_system* y, double strain_end)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    int rc = sys_step(y, strain_end, 1, 0);
+    launch_set_strain_end(y->sc, y->ninst, strain_end, y->s);
+    int rc = bat_step(y, 1, 0);
     if (rc) return rc;
-    return check_scalars(y->sc, y->s);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_time_evolution(lfd_system* y, double strain_end, int* nsteps)
 {
     if (!y || !(strain_end >= 0)) return fail(LFD_ERR_ARG, "lfd_system_time_evolution: bad argument");
+    launch_set_strain_end(y->sc, y->ninst, strain_end, y->s);
     int n = 0;
     for (;;) {
         DevScalars h;
-        int rc = check_scalars(y->sc, y->s, &h);
+        int rc = batch_check(y, 0, &h);
         if (rc) return rc;
         if (h.strain > strain_end-1e-8) break;
-        if ((rc = sys_step(y, strain_end))) return rc;
+        if ((rc = bat_step(y))) return rc;
         n++;
     }
-    int rc = sys_step(y, strain_end, 1, 0);
+    int rc = bat_step(y, 1, 0);
     if (rc) return rc;
     if (nsteps) *nsteps = n+1;
-    return check_scalars(y->sc, y->s);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_compute_stress(lfd_system* y, int in_predictor)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    int rc = sys_compute_velocities(y, in_predictor, 1);
+    int rc = bat_velocities(y, in_predictor, 1);
     if (rc) return rc;
-    return check_scalars(y->sc, y->s);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_get_stress(lfd_system* y, double* groups24, double* total6, double* shear_rate,
                                      double* viscosity, double* particle6)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    if (!y->have_stress) return fail(LFD_ERR_ARG, "lfd_system_get_stress: no stress computed yet");
-    DevScalars h;
-    int rc = check_scalars(y->sc, y->s, &h);
-    if (rc) return rc;
-    if (groups24) std::memcpy(groups24, h.stress_groups, 24*sizeof(double));
-    if (total6) std::memcpy(total6, h.stress_total, 6*sizeof(double));
-    if (shear_rate) *shear_rate = h.flow.rate;
-    if (viscosity) {
-        // Simulation::outputData (Simulation.cpp:662-666)
-        const double* E = h.flow.E;
-        if (h.flow.rate == 0) E = y->C.Eh;
-        auto dd = [](const double* a, const double* b) {
-            return a[0]*b[0]+2*a[1]*b[1]+2*a[2]*b[2]+2*a[3]*b[3]+a[4]*b[4]+a[5]*b[5];
-        };
-        *viscosity = 0.5*dd(h.stress_total, E)/dd(E, E);
-    }
-    if (particle6) {
-        std::vector<double> ps((size_t)STRESS_VALS*y->n);
-        HIPCHK(hipMemcpyAsync(ps.data(), y->ps, ps.size()*sizeof(double), hipMemcpyDeviceToHost, y->s));
-        HIPCHK(hipStreamSynchronize(y->s));
-        for (int i = 0; i < y->n; i++)
-            for (int m = 0; m < 6; m++) {
-                double t = 0;
-                for (int sl = 0; sl < STRESS_VALS/6; sl++) t += ps[(size_t)STRESS_VALS*i+6*sl+m];
-                particle6[6*(size_t)i+m] = t;
-            }
-    }
-    return LFD_OK;
+    return inst_get_stress(y, 0, groups24, total6, shear_rate, viscosity, particle6);
 }
 
 extern "C" int lfd_system_get_parts(lfd_system* y, int part, double* vel6)
 {
     if (!y || part < 0 || part > 3 || !vel6) return fail(LFD_ERR_ARG, "lfd_system_get_parts: bad argument");
-    const size_t m6 = 6*(size_t)y->n;
-    HIPCHK(hipMemcpyAsync(vel6, y->vel_parts+part*m6, m6*sizeof(double), hipMemcpyDeviceToHost, y->s));
+    const InstHost& h = y->inst[0];
+    const size_t m6 = 6*(size_t)y->M;
+    HIPCHK(hipMemcpyAsync(vel6, y->vel_parts+part*m6+6*(size_t)h.ic.poff, 6*(size_t)h.ic.n*sizeof(double),
+                          hipMemcpyDeviceToHost, y->s));
     HIPCHK(hipStreamSynchronize(y->s));
     return LFD_OK;
 }
@@ -691,125 +1035,103 @@ extern "C" int lfd_system_get_parts(lfd_system* y, int part, double* vel6)
 extern "C" int lfd_system_sync(lfd_system* y)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    return check_scalars(y->sc, y->s);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_check_new_interactions(lfd_system* y)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    sys_check_new(y);
-    return check_scalars(y->sc, y->s);
+    bat_check_new(y);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_update_interactions(lfd_system* y, int in_predictor, double dt)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    sys_update(y, in_predictor, dt);
-    return check_scalars(y->sc, y->s);
+    bat_update(y, in_predictor, dt);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_assemble(lfd_system* y)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    sys_assemble(y);
-    return check_scalars(y->sc, y->s);
+    bat_assemble(y);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_compute_velocities(lfd_system* y, int in_predictor)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    int rc = sys_compute_velocities(y, in_predictor);
+    int rc = bat_velocities(y, in_predictor);
     if (rc) return rc;
-    return check_scalars(y->sc, y->s);
+    return batch_check(y, 0);
 }
 
 extern "C" int lfd_system_num_pairs(lfd_system* y)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    DevScalars h;
-    int rc = check_scalars(y->sc, y->s, &h);
-    if (rc) return rc;
-    return h.npairs;
+    return inst_num_pairs(y, 0);
 }
 
 extern "C" int lfd_system_get_pairs(lfd_system* y, int cap, int* p0, int* p1, int* state, double* geom5,
                                     double* disp_tan)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    DevScalars h;
-    int rc = check_scalars(y->sc, y->s, &h);
-    if (rc) return rc;
-    int np_ = h.npairs;
-    if (np_ > cap) return fail(LFD_ERR_ARG, "lfd_system_get_pairs: cap too small");
-    std::vector<PairRec> v(np_);
-    if (np_) HIPCHK(hipMemcpy(v.data(), y->pairs[y->cur], np_*sizeof(PairRec), hipMemcpyDeviceToHost));
-    for (int k = 0; k < np_; k++) {
-        const PairRec& q = v[k];
-        if (p0) p0[k] = q.p0;
-        if (p1) p1[k] = q.p1;
-        if (state) state[k] = pair_state(q.flags);
-        if (geom5) {
-            geom5[5*k] = q.n[0]; geom5[5*k+1] = q.n[1]; geom5[5*k+2] = q.n[2];
-            geom5[5*k+3] = q.r; geom5[5*k+4] = q.gap;
-        }
-        if (disp_tan) for (int c = 0; c < 3; c++) disp_tan[3*k+c] = q.disp_tan[c];
-    }
-    return np_;
+    return inst_get_pairs(y, 0, cap, p0, p1, state, geom5, disp_tan);
 }
 
 extern "C" int lfd_system_get_state(lfd_system* y, double* pos, double* vel6, double* na6, double* sc8)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    DevScalars h;
-    int rc = check_scalars(y->sc, y->s, &h);
-    if (rc) return rc;
-    if (pos) {
-        std::vector<double4> v(y->n);
-        HIPCHK(hipMemcpy(v.data(), y->pos, y->n*sizeof(double4), hipMemcpyDeviceToHost));
-        for (int i = 0; i < y->n; i++) { pos[3*i] = v[i].x; pos[3*i+1] = v[i].y; pos[3*i+2] = v[i].z; }
-    }
-    if (vel6) HIPCHK(hipMemcpy(vel6, y->vel6, 6*(size_t)y->n*sizeof(double), hipMemcpyDeviceToHost));
-    if (na6) HIPCHK(hipMemcpy(na6, y->na6, 6*(size_t)y->n*sizeof(double), hipMemcpyDeviceToHost));
-    if (sc8) {
-        sc8[0] = h.shear_disp[0]; sc8[1] = h.shear_disp[1]; sc8[2] = h.shear_disp[2];
-        sc8[3] = h.dt; sc8[4] = h.time; sc8[5] = h.strain;
-        sc8[6] = std::sqrt(__builtin_bit_cast(double, h.max_vel_bits));
-        sc8[7] = std::sqrt(__builtin_bit_cast(double, h.max_slide_bits));
-    }
-    return LFD_OK;
+    return inst_get_state(y, 0, pos, vel6, na6, sc8);
 }
 
 extern "C" int lfd_system_get_cells(lfd_system* y, int* dims4, int* labels)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    int rc = check_scalars(y->sc, y->s);
+    int rc = batch_check(y, 0);
     if (rc) return rc;
-    if (dims4) { dims4[0] = y->g.nx; dims4[1] = y->g.ny; dims4[2] = y->g.nz; dims4[3] = y->g.boxed; }
-    if (labels) HIPCHK(hipMemcpy(labels, y->box_of, y->n*sizeof(int), hipMemcpyDeviceToHost));
-    return y->g.nbox;
+    const InstConst& g = y->inst[0].ic;
+    if (dims4) { dims4[0] = g.nx; dims4[1] = g.ny; dims4[2] = g.nz; dims4[3] = g.boxed; }
+    if (labels) {
+        HIPCHK(hipMemcpy(labels, y->box_of+g.poff, g.n*sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < g.n; i++) labels[i] -= g.coff;
+    }
+    return g.nbox;
 }
 
 extern "C" int lfd_system_get_blocks(lfd_system* y, int cap, double* dblocks, double* odblocks, int* p0, int* p1)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
-    DevScalars h;
-    int rc = check_scalars(y->sc, y->s, &h);
+    int rc = batch_check(y, 0);
     if (rc) return rc;
-    int np_ = h.npairs;
+    const InstHost& h = y->inst[0];
+    int k0 = 0, k1 = 0;
+    if ((rc = pair_range(y, h, k0, k1))) return rc;
+    const int np_ = k1-k0, off = h.ic.poff;
     if (np_ > cap) return fail(LFD_ERR_ARG, "lfd_system_get_blocks: cap too small");
-    if (dblocks) HIPCHK(hipMemcpy(dblocks, y->S.diag64, 15*(size_t)y->n*sizeof(double), hipMemcpyDeviceToHost));
-    if (odblocks && np_) HIPCHK(hipMemcpy(odblocks, y->S.od64, 18*(size_t)np_*sizeof(double), hipMemcpyDeviceToHost));
-    if (p0 && np_) HIPCHK(hipMemcpy(p0, y->S.p0, np_*sizeof(int), hipMemcpyDeviceToHost));
-    if (p1 && np_) HIPCHK(hipMemcpy(p1, y->S.p1, np_*sizeof(int), hipMemcpyDeviceToHost));
+    if (dblocks) HIPCHK(hipMemcpy(dblocks, y->S.diag64+15*(size_t)off, 15*(size_t)h.ic.n*sizeof(double),
+                                  hipMemcpyDeviceToHost));
+    if (odblocks && np_) HIPCHK(hipMemcpy(odblocks, y->S.od64+18*(size_t)k0, 18*(size_t)np_*sizeof(double),
+                                          hipMemcpyDeviceToHost));
+    if (p0 && np_) {
+        HIPCHK(hipMemcpy(p0, y->S.p0+k0, np_*sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < np_; k++) p0[k] -= off;
+    }
+    if (p1 && np_) {
+        HIPCHK(hipMemcpy(p1, y->S.p1+k0, np_*sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < np_; k++) p1[k] -= off;
+    }
     return np_;
 }
 
 extern "C" int lfd_system_get_rhs(lfd_system* y, double* rhs)
 {
     if (!y || !rhs) return fail(LFD_ERR_ARG, "bad argument");
-    int rc = check_scalars(y->sc, y->s);
+    int rc = batch_check(y, 0);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(rhs, y->rhs, 6*(size_t)y->n*sizeof(double), hipMemcpyDeviceToHost));
+    const InstConst& g = y->inst[0].ic;
+    HIPCHK(hipMemcpy(rhs, y->rhs+6*(size_t)g.poff, 6*(size_t)g.n*sizeof(double), hipMemcpyDeviceToHost));
     return LFD_OK;
 }
 
@@ -822,6 +1144,145 @@ extern "C" int lfd_system_get_solve_stats(lfd_system* y, lfd_solve_stats* st, lo
     if (st) { st->iters_inner = h.iters_inner; st->iters_outer = h.iters_outer; st->res_norm = h.res_norm; st->rhs_norm = h.rhs_norm; }
     if (total_inner) *total_inner = h.total_inner;
     return LFD_OK;
+}
+
+// ===================================================================== batch API
+extern "C" int lfd_batch_create(int slot_cap, int cell_cap, int inst_cap, int control, const lfd_solve_opts* o,
+                                lfd_batch** out)
+{
+    if (!out || slot_cap < 2 || cell_cap < 1 || inst_cap < 1 ||
+        (control != LFD_RATE_CONTROL && control != LFD_STRESS_CONTROL))
+        return fail(LFD_ERR_ARG, "lfd_batch_create: bad argument");
+    *out = nullptr;
+    if (o && o->kind != LFD_SOLVER_MIXED) return fail(LFD_ERR_UNSUPPORTED, "lfd_batch_create: batches use the LDS solver");
+    lfd_batch* B = new lfd_batch();
+    int rc = batch_init(B, slot_cap, cell_cap, inst_cap, o, control);
+    if (rc) { delete B; return rc; }
+    *out = B;
+    return LFD_OK;
+}
+
+extern "C" void lfd_batch_destroy(lfd_batch* B) { delete B; }
+
+extern "C" int lfd_batch_footprint(int np, const double* radius, const double L[3], const lfd_params* p,
+                                   int* slots, int* cells, long long* bytes)
+{
+    InstConst ic;
+    std::vector<int> nb, nnb, ctype;
+    const double sd0[3] = {0, 0, 0};
+    int rc = describe_instance(np, radius, L, p, sd0, ic, nb, nnb, ctype);
+    if (rc) return rc;
+    size_t ps = 0, pc = 0;
+    batch_unit_bytes(&ps, &pc);
+    const int sl = round_up(np, LDS_ROWS);
+    if (slots) *slots = sl;
+    if (cells) *cells = ic.nbox;
+    if (bytes) *bytes = (long long)(ps*sl+pc*ic.nbox);
+    return LFD_OK;
+}
+
+extern "C" int lfd_batch_room(lfd_batch* B, int slots, int cells)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    std::vector<std::pair<int, int>> us, uc;
+    bool handle = false;
+    for (int k = 0; k < B->icap; k++) {
+        if (B->inst[k].live) {
+            us.emplace_back(B->inst[k].ic.poff, B->inst[k].slots);
+            uc.emplace_back(B->inst[k].ic.coff, B->inst[k].ic.nbox);
+        } else handle = true;
+    }
+    return handle && first_fit(us, B->M, round_up(slots, LDS_ROWS)) >= 0 && first_fit(uc, B->ccap, cells) >= 0;
+}
+
+extern "C" int lfd_batch_add(lfd_batch* B, int np, const double* pos, const double* radius, const double L[3],
+                             const double shear_disp[3], const lfd_params* p, double strain_end,
+                             double target_stress, int* id)
+{
+    if (!B || !id) return fail(LFD_ERR_ARG, "lfd_batch_add: bad argument");
+    int rc = bat_add(B, np, pos, radius, L, shear_disp, p, strain_end, 1, id);
+    if (rc) return rc;
+    if (B->control == LFD_STRESS_CONTROL) rc = inst_set_scalars(B, *id, &target_stress, nullptr, nullptr, nullptr);
+    return rc;
+}
+
+extern "C" int lfd_batch_remove(lfd_batch* B, int id)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    return bat_remove(B, id);
+}
+
+extern "C" int lfd_batch_step(lfd_batch* B, int nsteps)
+{
+    if (!B || nsteps < 0) return fail(LFD_ERR_ARG, "lfd_batch_step: bad argument");
+    if (B->S.launches.empty()) return nsteps ? fail(LFD_ERR_ARG, "lfd_batch_step: empty batch") : LFD_OK;
+    for (int k = 0; k < nsteps; k++) {
+        int rc = bat_step(B);
+        if (rc) return rc;
+    }
+    return LFD_OK;
+}
+
+extern "C" int lfd_batch_sync(lfd_batch* B)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    return batch_check(B, -1);
+}
+
+extern "C" int lfd_batch_query(lfd_batch* B, int id, double* scalars8, lfd_solve_stats* st, int* status)
+{
+    if (!live_inst(B, id)) return LFD_ERR_ARG;
+    DevScalars h;
+    HIPCHK(hipMemcpyAsync(&h, B->sc+id, sizeof(h), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    if (scalars8) fill_scalars8(h, scalars8);
+    if (st) { st->iters_inner = h.iters_inner; st->iters_outer = h.iters_outer; st->res_norm = h.res_norm; st->rhs_norm = h.rhs_norm; }
+    if (status) {
+        std::string keep = g_err;
+        *status = status_of(h.error);
+        g_err = keep;
+    }
+    return LFD_OK;
+}
+
+extern "C" int lfd_batch_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* scalars8)
+{
+    return inst_get_state(B, id, pos, vel6, na6, scalars8);
+}
+
+extern "C" int lfd_batch_get_pairs(lfd_batch* B, int id, int cap, int* p0, int* p1, int* state, double* geom5,
+                                   double* disp_tan)
+{
+    return inst_get_pairs(B, id, cap, p0, p1, state, geom5, disp_tan);
+}
+
+extern "C" int lfd_batch_num_pairs(lfd_batch* B, int id) { return inst_num_pairs(B, id); }
+
+extern "C" int lfd_batch_info(lfd_batch* B, int* live, int* slot_cap, int* cell_cap, int* inst_cap,
+                              int* solver_launches, long long* bytes)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    int nl = 0;
+    for (auto& h : B->inst) nl += h.live;
+    if (live) *live = nl;
+    if (slot_cap) *slot_cap = B->M;
+    if (cell_cap) *cell_cap = B->ccap;
+    if (inst_cap) *inst_cap = B->icap;
+    if (solver_launches) *solver_launches = (int)B->S.launches.size();
+    if (bytes) {
+        size_t ps = 0, pc = 0;
+        batch_unit_bytes(&ps, &pc);
+        *bytes = (long long)(ps*B->M+pc*B->ccap);
+    }
+    return LFD_OK;
+}
+
+extern "C" int lfd_batch_profile(lfd_batch* B, int enable, double* solve_ms_total, int* solve_launches)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    int rc = collect_profile(B->S, B->s, solve_ms_total, solve_launches);
+    B->S.profiling = enable != 0;
+    return rc;
 }
 
 // ===================================================================== StokesSolver staging
@@ -963,6 +1424,7 @@ extern "C" int lfd_stokes_fill_done(lfd_stokes* st)
                           st->S.minv32, st->cap, st->sc, s);
     int rc = check_scalars(st->sc, s);
     if (rc) return rc;
+    if ((rc = set_jobs(st->S, std::vector<InstSpan>{{0, 0, n}}, s))) return rc;
     st->filled = true;
     return LFD_OK;
 }
