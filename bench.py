@@ -160,6 +160,193 @@ def run_c5(args):
     print(json.dumps(out), flush=True)
 
 
+def sweep_specs(count, seed):
+    """Batched sweep: N log-uniform in [250, 2000], phi uniform in [0.45, 0.58],
+    shear rate log-uniform over 3 decades in repulsive-force units
+    (10^U(-2, 1), i.e. repulsion = 1/rate in hydrodynamic units); instance k
+    is seeded seed + k."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(250), np.log(2000)))))
+        phi = float(rng.uniform(0.45, 0.58))
+        rate_r = float(10 ** rng.uniform(-2, 1))
+        out.append((n, phi, seed + k, 1.0 / rate_r))
+    return out
+
+
+def _gen(spec):
+    from lf_dem_amd import confgen
+    n, phi, seed, _ = spec
+    return confgen.generate(n, phi, seed)
+
+
+def generate_many(specs, workers=8):
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_gen, specs))
+
+
+def run_c3(args):
+    """Batched sweep of 64 independent instances stepped together: one set of
+    kernels per phase and the solver packed into cooperative launches.  The
+    same instances stepped one at a time (each a batch of one) are timed
+    beside it."""
+    from lf_dem_amd import capi
+    if capi.lib().lfd_device_count() < 1:
+        raise SystemExit("no GPU visible")
+    specs = sweep_specs(args.instances, args.seed)
+    t0 = time.time()
+    confs = generate_many(specs)
+    gen_s = time.time() - t0
+    print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
+    opts = capi.default_opts(kind=0, tol=args.tol)
+    params = [capi.default_params(repulsion=s[3]) for s in specs]
+    slots = cells = 0
+    for (pos, rad, L), p in zip(confs, params):
+        sl, ce, _ = capi.footprint(len(rad), rad, L, p)
+        slots += sl
+        cells += ce
+    B = capi.Batch(slots, cells, len(specs), opts=opts)
+    for (pos, rad, L), p in zip(confs, params):
+        B.add(pos, rad, L, params=p)
+    info = B.info()
+    B.step(args.warmup)
+    B.sync()
+    B.profile(True)
+    t0 = time.perf_counter()
+    B.step(args.steps)
+    B.sync()
+    elapsed = time.perf_counter() - t0
+    solve_ms, launches = B.profile(False)
+    lat = []
+    for _ in range(args.latency_steps):
+        t1 = time.perf_counter()
+        B.step(1)
+        B.sync()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat = np.array(lat)
+    del B
+    # the same instances one at a time
+    lone_s = 0.0
+    lone_steps = 2
+    for (pos, rad, L), p in zip(confs, params):
+        S = capi.System(pos, rad, L, params=p, opts=opts)
+        S.first_update()
+        S.step(1)
+        t1 = time.perf_counter()
+        S.step(lone_steps)
+        lone_s += time.perf_counter() - t1
+        del S
+    total_n = sum(s[0] for s in specs)
+    bsps = args.steps / elapsed
+    lone_sweep_sps = lone_steps / lone_s          # sweeps of all instances per second, one at a time
+    out = {
+        "metric": "batched sweep: steps/s of 64 instances stepped together", "value": round(bsps, 3),
+        "unit": "batch steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 / bsps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "64 instances, N log-uniform [250, 2000], phi U(0.45, 0.58), rate 10^U(-2, 1) "
+                               "repulsive units", "instances": len(specs), "total_N": total_n,
+                   "seed": args.seed, "tol": args.tol, "parallelism": "single GPU"},
+        "instance_steps_per_s": round(bsps * len(specs), 1),
+        "particle_steps_per_s": round(bsps * total_n, 0),
+        "one_at_a_time_steps_per_s": round(lone_sweep_sps, 3),
+        "speedup_vs_one_at_a_time": round(bsps / lone_sweep_sps, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
+        "solver": {"launches_per_solve": info["solver_launches"], "solve_launches": launches,
+                   "avg_solve_ms": round(solve_ms / max(launches, 1), 4)},
+        "device_bytes": info["bytes"],
+        "confgen_s": round(gen_s, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def serving_requests(seed, rate, duration, strain):
+    """Poisson arrivals (mean `rate` per second) over `duration` seconds; N
+    log-uniform in [200, 8000], phi uniform in [0.50, 0.58], each running
+    `strain` strain units."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        n = int(round(np.exp(rng.uniform(np.log(200), np.log(8000)))))
+        phi = float(rng.uniform(0.50, 0.58))
+        out.append((t, n, phi, seed * 1000 + len(out), strain))
+    return out
+
+
+def run_c4(args):
+    """Serving: Poisson-arriving simulation requests admitted into one batch
+    by footprint as room frees up, stepped together, retired at their strain
+    (lf_dem_amd.serve).  Reports step latency percentiles beside throughput;
+    configurations are the clients' inputs and are generated before the clock
+    starts."""
+    from lf_dem_amd import capi, serve
+    if capi.lib().lfd_device_count() < 1:
+        raise SystemExit("no GPU visible")
+    from lf_dem_amd import confgen
+    arr = serving_requests(args.seed, args.rate, args.duration, args.strain)
+    # the first requests' configurations are prepared before the clock starts;
+    # later ones (if the loop reaches them) are generated at admission
+    k = min(len(arr), args.serve_pregen)
+    t0 = time.time()
+    confs = generate_many([(n, phi, sd, 0.0) for (_, n, phi, sd, _) in arr[:k]], workers=16)
+    gen_s = time.time() - t0
+    print("%d of %d request configurations generated in %.1f s" % (k, len(arr), gen_s), file=sys.stderr,
+          flush=True)
+    opts = capi.default_opts(kind=0, tol=args.tol)
+    reqs = []
+    for i, a in enumerate(arr):
+        if i < k:
+            pos, rad, L = confs[i]
+            reqs.append(serve.Request(i, pos, rad, L, strain=a[4], arrival=a[0]))
+        else:
+            n1, n2 = confgen.bidisperse_counts(a[1])
+            rad = np.concatenate([np.full(n1, 1.0), np.full(n2, 1.4)])
+            L = np.full(3, confgen.box_size(rad, a[2]))
+            reqs.append(serve.Request(i, None, rad, L, strain=a[4], arrival=a[0],
+                                      loader=lambda a=a: confgen.generate(a[1], a[2], a[3])[0]))
+    slot_cap = args.serve_slots
+    cell_cap = slot_cap // 4
+    srv = serve.Server(slot_cap, cell_cap, 1024, opts=opts, keep_state=False)
+    # warm the kernels on a throwaway request
+    w = serve.Request(-1, *confs[0], strain=1e-3)
+    srv.submit(w)
+    srv.admit(0)
+    while srv.live:
+        srv.step()
+    srv.done.clear()
+    srv.step_ms.clear()
+    srv.particle_steps = srv.instance_steps = 0
+    elapsed = srv.run(reqs, duration=args.duration)
+    summ = srv.summary(elapsed)
+    info = srv.batch.info()
+    out = {
+        "metric": "serving: p50/p99 batch step latency with continuous batching", "value":
+            round(summ["particle_steps_per_s"], 0), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": summ["batch_steps"], "warmup": 1, "ms_per_step": round(summ["step_ms"]["p50"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "Poisson requests %.0f/s for %.0f s, N log-uniform [200, 8000], phi U(0.50, 0.58), "
+                               "%.2f strain each" % (args.rate, args.duration, args.strain),
+                   "requests": len(reqs), "slot_cap": info["slot_cap"], "device_bytes": info["bytes"],
+                   "seed": args.seed, "tol": args.tol, "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(summ["step_ms"]["p50"], 3), "p99": round(summ["step_ms"]["p99"], 3),
+                       "steps": summ["batch_steps"]},
+        "instance_steps_per_s": round(summ["instance_steps_per_s"], 1),
+        "completed": summ["completed"], "failed": summ["failed"], "live_at_end": summ["live_at_end"],
+        "waiting_at_end": summ["waiting_at_end"],
+        "response_s": {k: round(v, 3) for k, v in summ["response_s"].items()},
+        "admission_wait_s": {k: round(v, 3) for k, v in summ["admission_wait_s"].items()},
+        "elapsed_s": round(elapsed, 2), "confgen_s": round(gen_s, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_leg(pos, rad, L, params, opts, args):
     """FP64 direct (oracle) solve at t=0: error of the GPU velocities and the
     host-core time of one assembly + sparse Cholesky factorisation + solve."""
@@ -195,13 +382,24 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c5"), default="c2",
-                    help="c2: N=4000 rate control (headline); c5: N=32000 stress control")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=20.0, help="c4: request arrivals per second")
+    ap.add_argument("--duration", type=float, default=25.0, help="c4: seconds of arrivals and service")
+    ap.add_argument("--strain", type=float, default=0.5, help="c4: strain units per request")
+    ap.add_argument("--serve-slots", type=int, default=131072, help="c4: particle slots of the batch")
+    ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+                    help="c2: N=4000 rate control (headline); c3: batched sweep of 64 instances; "
+                         "c4: served requests; c5: N=32000 stress control")
     args = ap.parse_args()
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)")
     if args.workload == "c5":
         run_c5(args)
+    elif args.workload == "c3":
+        run_c3(args)
+    elif args.workload == "c4":
+        run_c4(args)
     else:
         run(args)
 

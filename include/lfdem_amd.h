@@ -215,6 +215,9 @@ int lfd_batch_step(lfd_batch *b, int nsteps);
 int lfd_batch_sync(lfd_batch *b);
 /* scalars8 as lfd_system_get_state; status: the instance's own lfd_status */
 int lfd_batch_query(lfd_batch *b, int id, double *scalars8, lfd_solve_stats *st, int *status);
+/* Every live instance in one device read: ids, strain and own lfd_status;
+ * returns the count (at most cap). */
+int lfd_batch_status(lfd_batch *b, int cap, int *ids, double *strain, int *status);
 int lfd_batch_get_state(lfd_batch *b, int id, double *pos, double *vel6, double *na6, double *scalars8);
 int lfd_batch_get_pairs(lfd_batch *b, int id, int cap, int *p0, int *p1, int *state, double *geom5,
                         double *disp_tan);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "lfd_batch_step": ([_vp, ctypes.c_int], ctypes.c_int),
     "lfd_batch_sync": ([_vp], ctypes.c_int),
     "lfd_batch_query": ([_vp, ctypes.c_int, _dp, _vp, _ip], ctypes.c_int),
+    "lfd_batch_status": ([_vp, ctypes.c_int, _ip, _dp, _ip], ctypes.c_int),
     "lfd_batch_get_state": ([_vp, ctypes.c_int, _dp, _dp, _dp, _dp], ctypes.c_int),
     "lfd_batch_get_pairs": ([_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
     "lfd_batch_num_pairs": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -462,6 +463,15 @@ class Batch:
         _check(lib().lfd_batch_query(self.h, int(iid), _d(sc), ctypes.byref(st), ctypes.byref(status)))
         return dict(shear_disp=sc[:3].copy(), dt=sc[3], time=sc[4], strain=sc[5], status=status.value,
                     iters_inner=st.iters_inner, iters_outer=st.iters_outer)
+
+    def status(self):
+        """(ids, strains, statuses) of every live instance, one device read."""
+        cap = max(len(self.np), 1)
+        ids = np.zeros(cap, dtype=np.int32)
+        strain = np.zeros(cap)
+        st = np.zeros(cap, dtype=np.int32)
+        m = _check(lib().lfd_batch_status(self.h, cap, _i(ids), _d(strain), _i(st)))
+        return ids[:m], strain[:m], st[:m]
 
     def state(self, iid):
         n = self.np[iid]

@@ -1245,6 +1245,29 @@ extern "C" int lfd_batch_query(lfd_batch* B, int id, double* scalars8, lfd_solve
     return LFD_OK;
 }
 
+extern "C" int lfd_batch_status(lfd_batch* B, int cap, int* ids, double* strain, int* status)
+{
+    if (!B || cap < 0) return fail(LFD_ERR_ARG, "lfd_batch_status: bad argument");
+    std::vector<DevScalars> h(std::max(B->ninst, 1));
+    if (B->ninst) HIPCHK(hipMemcpyAsync(h.data(), B->sc, B->ninst*sizeof(DevScalars), hipMemcpyDeviceToHost, B->s));
+    BatchScalars hb;
+    HIPCHK(hipMemcpyAsync(&hb, B->bs, sizeof(hb), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    if (hb.error) return status_of(hb.error);
+    std::string keep = g_err;
+    int m = 0;
+    for (int k = 0; k < B->ninst; k++) {
+        if (!B->inst[k].live) continue;
+        if (m >= cap) return fail(LFD_ERR_ARG, "lfd_batch_status: cap too small");
+        if (ids) ids[m] = k;
+        if (strain) strain[m] = h[k].strain;
+        if (status) status[m] = status_of(h[k].error);
+        m++;
+    }
+    g_err = keep;
+    return m;
+}
+
 extern "C" int lfd_batch_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* scalars8)
 {
     return inst_get_state(B, id, pos, vel6, na6, scalars8);
