@@ -50,7 +50,7 @@ def run(args):
     pos, rad, L = confgen.generate(args.n, args.phi, args.seed)
     gen_s = time.time() - t0
     params = capi.default_params()
-    opts = capi.default_opts(kind=0, tol=args.tol)
+    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
     S = capi.System(pos, rad, L, params=params, opts=opts)
     S.first_update()
     S.step(args.warmup)
@@ -93,6 +93,7 @@ def run(args):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "N=4000 bidisperse 1.0/1.4 phi=0.56 rate-controlled shear, mixed precision",
                    "N": args.n, "phi": args.phi, "seed": args.seed, "pairs": npairs, "tol": args.tol,
+                   "precond": ["block-jacobi", "aggregate"][args.precond],
                    "parallelism": "single GPU"},
         "plain_gpu_steps_per_s": round(plain_sps, 3),
         "speedup_vs_plain_gpu": round(sps / plain_sps, 2),
@@ -100,7 +101,8 @@ def run(args):
                        "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
         "solver": {"inner_iters_per_solve": round(iters_per_solve, 1), "solve_launches": launches,
                    "avg_solve_ms": round(avg_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_solve_lds", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": ["k_solve_lds", "k_solve_agg"][args.precond],
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None},
         "confgen_s": round(gen_s, 2),
@@ -126,7 +128,7 @@ def run_c5(args):
     gen_s = time.time() - t0
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
-    opts = capi.default_opts(kind=0, tol=args.tol)
+    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
     S = capi.System(pos, rad, L, params=params, opts=opts)
     S.first_update()
     S.compute_stress(False)
@@ -200,7 +202,7 @@ def run_c3(args):
     confs = generate_many(specs)
     gen_s = time.time() - t0
     print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
-    opts = capi.default_opts(kind=0, tol=args.tol)
+    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
     params = [capi.default_params(repulsion=s[3]) for s in specs]
     slots = cells = 0
     for (pos, rad, L), p in zip(confs, params):
@@ -298,7 +300,7 @@ def run_c4(args):
     gen_s = time.time() - t0
     print("%d of %d request configurations generated in %.1f s" % (k, len(arr), gen_s), file=sys.stderr,
           flush=True)
-    opts = capi.default_opts(kind=0, tol=args.tol)
+    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
     reqs = []
     for i, a in enumerate(arr):
         if i < k:
@@ -388,10 +390,14 @@ def main():
     ap.add_argument("--strain", type=float, default=0.5, help="c4: strain units per request")
     ap.add_argument("--serve-slots", type=int, default=131072, help="c4: particle slots of the batch")
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
+    ap.add_argument("--precond", type=int, default=None,
+                    help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0)")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: N=4000 rate control (headline); c3: batched sweep of 64 instances; "
                          "c4: served requests; c5: N=32000 stress control")
     args = ap.parse_args()
+    if args.precond is None:
+        args.precond = 0 if args.workload == "c3" else 1
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)")
     if args.workload == "c5":

@@ -50,6 +50,13 @@ typedef enum lfd_solver_kind {
 	LFD_SOLVER_MIXED_GLOBAL = 2  /* as MIXED, blocks read from HBM each iteration */
 } lfd_solver_kind;
 
+/* Preconditioner of the MIXED solver on the device-resident paths. */
+typedef enum lfd_precond {
+	LFD_PRECOND_BLOCK_JACOBI = 0,  /* inverse of each 6x6 diagonal block */
+	LFD_PRECOND_AGGREGATE = 1      /* inverse of the 48x48 diagonal block of groups of up to 8 strongly
+	                                  coupled particles (instances of up to 8192 particles) */
+} lfd_precond;
+
 typedef struct lfd_solve_opts {
 	double tol;             /* target ||b - R x|| / ||b|| in FP64 */
 	double inner_tol;       /* residual drop per FP32 inner solve */
@@ -57,6 +64,9 @@ typedef struct lfd_solve_opts {
 	int lanes_per_row;      /* SpMV lanes per 6x6 block row: 1, 2, 4, 8 */
 	int groups;             /* workgroups of the persistent solver, 0 = auto */
 	int kind;               /* lfd_solver_kind */
+	int precond;            /* lfd_precond */
+	int regroup_interval;   /* LFD_PRECOND_AGGREGATE: steps between regroupings (the block inverses
+	                           are recomputed at every assembly) */
 } lfd_solve_opts;
 
 typedef struct lfd_solve_stats {

@@ -1,0 +1,835 @@
+// lfd_agg.hip -- aggregate block-Jacobi preconditioner for the resistance
+// matrix and the persistent mixed-precision solver that applies it (gfx950).
+//
+// Why: lubrication couples near-contact pairs with resistances ~ 1/(h + delta)
+// that are up to a thousand times the Stokes drag.  With 6x6 block-Jacobi the
+// slowest modes are rigid motions of such strongly coupled clusters (scaled
+// eigenvalues ~ drag / coupling), so PCG needs ~250 iterations at N = 4000,
+// phi = 0.56.  Inverting the diagonal block of each group of up to 8 strongly
+// coupled particles instead removes most of them (about 0.65x the iterations).
+//
+// Pipeline, per step (grouping) and per assembly (inverses):
+//   k_agg_weight   coupling weight of every pair from its FP64 blocks;
+//   k_agg_group    one workgroup per instance, everything in LDS: three levels
+//                  of parallel matching (each group merges with its mutually
+//                  strongest neighbour group while the size stays <= 2, 4, 8),
+//                  then the groups are packed into waves of <= 8 particles by
+//                  a best-fit-decreasing rule evaluated on the size counts and
+//                  every particle gets its solver slot;
+//   k_agg_perm     slot -> particle map;
+//   k_agg_inverse  one wavefront per wave: FP64 Gauss-Jordan inverse of the
+//                  wave's 48 x 48 diagonal block (all couplings between its
+//                  particles), symmetrised, packed lower triangle in FP32;
+//   k_solve_agg    FP64 iterative refinement around pipelined FP32 PCG with
+//                  the wave inverses as preconditioner; one wavefront per
+//                  wave, 12 waves per workgroup, blocks and inverses staged in
+//                  LDS once per solve.
+#include <hip/hip_runtime.h>
+#include "lfd_internal.hpp"
+#include "lfd_device.hpp"
+
+namespace lfd {
+
+static constexpr int AG_LEVELS = 3, AG_ROUNDS = 3;
+static constexpr int AGB_THREADS = 1024;
+static constexpr int AG_RMAX = 64;                  // run-table entries of the packing
+static constexpr int AG_HMASK = 16383;              // 14-bit local partner index in a matching key
+
+// ------------------------------------------------------------------ coupling adjacency
+// Per particle, its incident pairs as (quantised weight << 14 | partner's local
+// index), lower incidence then upper, at row_ptr[i] + tptr[i] (both prefix
+// arrays are exclusive sums over the particle slots, so the segments tile
+// [0, 2P)).  The weight |B|_F / sqrt(|D_p0|_F |D_p1|_F) over the packed FP64
+// values is the size of the coupling relative to the two particles' own
+// resistances; the 18-bit quantisation only orders the candidates.
+__device__ inline unsigned agg_weight_q(const BsrView& A, int k)
+{
+    const double* o = A.od64+18*(size_t)k;
+    const double* d0 = A.diag64+15*(size_t)A.p0[k];
+    const double* d1 = A.diag64+15*(size_t)A.p1[k];
+    double so = 0, s0 = 0, s1 = 0;
+    for (int m = 0; m < 18; m++) so += o[m]*o[m];
+    for (int m = 0; m < 15; m++) { s0 += d0[m]*d0[m]; s1 += d1[m]*d1[m]; }
+    const double den = sqrt(sqrt(s0*s1));
+    const double w = den > 0 ? sqrt(so)/den : 0.0;
+    return (unsigned)fmin(fmax(w, 0.0)*262144.0, 262143.0);
+}
+
+// An instance is regrouped at its first assembly and then at the predictor
+// assembly of every `interval`-th step of its own, so its preconditioner
+// never depends on the other instances of the batch.
+__device__ inline bool agg_due(const DevScalars* sc, int interval)
+{
+    return sc->group_age < 0 || sc->group_age >= interval;
+}
+
+__global__ void k_agg_age(DevScalars* sc, int ninst)
+{
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k < ninst && sc[k].group_age >= 0) sc[k].group_age++;
+}
+
+__global__ void k_agg_weight(BsrView A, const int* __restrict__ inst_of, const DevScalars* __restrict__ scs,
+                             int interval, unsigned* __restrict__ wq, int pcap)
+{
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= pcap || k >= *A.npairs) return;
+    if (!agg_due(scs+(inst_of ? inst_of[A.p0[k]] : 0), interval)) return;
+    wq[k] = agg_weight_q(A, k);
+}
+
+__global__ void k_agg_adjacency(BsrView A, const unsigned* __restrict__ wq, const int* __restrict__ inst_of,
+                                const InstConst* __restrict__ IC, const DevScalars* __restrict__ scs, int interval,
+                                unsigned* __restrict__ adj, int nslots)
+{
+    const int i = blockIdx.x*blockDim.x+threadIdx.x;
+    if (i >= nslots) return;
+    int poff = 0, inst = 0;
+    if (inst_of) {
+        inst = inst_of[i];
+        if (inst < 0) return;
+        poff = IC[inst].poff;
+    }
+    if (!agg_due(scs+inst, interval)) return;
+    const int l0 = A.tptr[i], l1 = A.tptr[i+1], u0 = A.row_ptr[i], u1 = A.row_ptr[i+1];
+    unsigned* out = adj+(size_t)(u0+l0);
+    for (int e = l0; e < l1; e++) {
+        const int k = A.tidx[e];
+        out[e-l0] = wq[k] << 14 | (unsigned)(A.p0[k]-poff);
+    }
+    out += l1-l0;
+    for (int k = u0; k < u1; k++) out[k-u0] = wq[k] << 14 | (unsigned)(A.p1[k]-poff);
+}
+
+// Inclusive scan over the workgroup of two 64-bit words (four 16-bit counters each).
+__device__ inline void block_scan2(unsigned long long& a, unsigned long long& b, unsigned long long* lds)
+{
+    const int lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE, nw = blockDim.x/WAVE;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        unsigned long long ta = __shfl_up(a, o), tb = __shfl_up(b, o);
+        if (lane >= o) { a += ta; b += tb; }
+    }
+    if (lane == WAVE-1) { lds[2*wid] = a; lds[2*wid+1] = b; }
+    __syncthreads();
+    unsigned long long pa = 0, pb = 0;
+    for (int q = 0; q < wid; q++) { pa += lds[2*q]; pb += lds[2*q+1]; }
+    a += pa; b += pb;
+    (void)nw;
+    __syncthreads();
+}
+
+__device__ inline int cls_count(unsigned long long lo, unsigned long long hi, int s)
+{
+    return s <= 4 ? (int)((lo >> (16*(s-1))) & 0xFFFF) : (int)((hi >> (16*(s-5))) & 0xFFFF);
+}
+
+// ------------------------------------------------------------------ grouping and packing
+// One workgroup per instance.  Local indices l = particle - poff.
+__global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G, const AggJob* __restrict__ jobs,
+                                                          DevScalars* scs, int interval)
+{
+    __shared__ short grp[AGG_NMAX];            // group leader (local index)
+    __shared__ unsigned char gsz[AGG_NMAX];    // group size, at the leader
+    __shared__ unsigned char done[AGG_NMAX];   // merged in this level
+    __shared__ unsigned best[AGG_NMAX];        // strongest eligible neighbour group (matching key); later list heads
+    __shared__ short relab[AGG_NMAX];          // new leader of a merged group
+    __shared__ short nxt[AGG_NMAX];            // member lists
+    __shared__ short gbin[AGG_NMAX];           // wave of a group (at the leader)
+    __shared__ unsigned char goff[AGG_NMAX];   // first position of the group in its wave
+    __shared__ unsigned long long sc_lds[2*AGB_THREADS/WAVE];
+    __shared__ unsigned run_comp[AG_RMAX];     // per run: count of each size class in one wave (nibbles)
+    __shared__ int run_rep[AG_RMAX], run_bin0[AG_RMAX];
+    __shared__ int s_nruns, s_bins, s_fallback;
+    __shared__ int sbase[AGG_NMAX+1];          // adjacency segment of each particle
+    const AggJob J = jobs[blockIdx.x];
+    if (!agg_due(scs+J.inst, interval)) return;
+    const int n = J.n, poff = J.poff;
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int ch = (n+T-1)/T, c0 = min(tid*ch, n), c1 = min(c0+ch, n);   // this thread's particles
+    for (int l = tid; l < n; l += T) { grp[l] = (short)l; gsz[l] = 1; best[l] = 0; relab[l] = -1; }
+    for (int l = tid; l <= n; l += T) sbase[l] = A.row_ptr[poff+l]+A.tptr[poff+l];
+    __syncthreads();
+    for (int lev = 0; lev < AG_LEVELS; lev++) {
+        const int cap = 2 << lev;
+        for (int l = tid; l < n; l += T) done[l] = 0;
+        __syncthreads();
+        for (int rnd = 0; rnd < AG_ROUNDS; rnd++) {
+            // propose: every particle offers its group's strongest eligible
+            // neighbour group (key: weight, then the smaller group index)
+            for (int l = c0; l < c1; l++) {
+                const int g = grp[l];
+                if (done[g]) continue;
+                const int sg = gsz[g], e0 = sbase[l], e1 = sbase[l+1];
+                unsigned kb = 0;
+                for (int e = e0; e < e1; e += 16) {
+                    unsigned ent[16];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) ent[q] = e+q < e1 ? G.adj[e+q] : 0u;
+#pragma unroll
+                    for (int q = 0; q < 16; q++) {
+                        if (!(ent[q] >> 14) || e+q >= e1) continue;
+                        const int h = grp[ent[q] & AG_HMASK];
+                        if (h == g || done[h] || sg+gsz[h] > cap) continue;
+                        kb = max(kb, (ent[q] >> 14) << 14 | (unsigned)(AG_HMASK-h));
+                    }
+                }
+                if (kb) atomicMax(&best[g], kb);
+            }
+            __syncthreads();
+            // accept mutual choices; the smaller leader absorbs the larger
+            for (int l = tid; l < n; l += T) {
+                if (grp[l] != l || !best[l]) continue;
+                const int h = AG_HMASK-(int)(best[l] & AG_HMASK);
+                if (l < h && best[h] && AG_HMASK-(int)(best[h] & AG_HMASK) == l) {
+                    relab[h] = (short)l;
+                    gsz[l] = (unsigned char)(gsz[l]+gsz[h]);
+                    done[l] = 1;
+                    done[h] = 1;
+                }
+            }
+            __syncthreads();
+            for (int l = tid; l < n; l += T) {
+                const int r = relab[grp[l]];
+                if (r >= 0) grp[l] = (short)r;
+            }
+            __syncthreads();
+            for (int l = tid; l < n; l += T) { best[l] = 0; relab[l] = -1; }
+            __syncthreads();
+        }
+    }
+    // ---- counts and ranks of the groups by size, in leader order
+    unsigned long long lo = 0, hi = 0;
+    for (int l = c0; l < c1; l++)
+        if (grp[l] == l) {
+            const int s = gsz[l];
+            if (s <= 4) lo += 1ull << (16*(s-1));
+            else hi += 1ull << (16*(s-5));
+        }
+    unsigned long long plo = lo, phi = hi;
+    block_scan2(plo, phi, sc_lds);
+    if (tid == T-1) { sc_lds[0] = plo; sc_lds[1] = phi; }
+    plo -= lo; phi -= hi;                      // exclusive prefix of this chunk
+    __syncthreads();
+    if (tid == 0) {
+        // best-fit decreasing on the counts: open a wave with the largest
+        // remaining group, fill it with the largest groups that fit; runs of
+        // identical waves are recorded once
+        int cnt[9];
+        for (int s = 1; s <= 8; s++) cnt[s] = cls_count(sc_lds[0], sc_lds[1], s);
+        int nr = 0, bins = 0, ok = 1;
+        for (int s = 8; s >= 1 && ok; s--) {
+            while (cnt[s] > 0) {
+                int take[9], before[9];
+                for (int t = 0; t <= 8; t++) { take[t] = 0; before[t] = cnt[t]; }
+                take[s] = 1;
+                cnt[s]--;
+                int rem = 8-s, limited = 0;
+                for (int t = min(rem, s); t >= 1; t--) {
+                    if (rem < t) continue;
+                    const int want = rem/t, k = min(cnt[t], want);
+                    if (k < want && k > 0) limited = 1;
+                    take[t] += k;
+                    cnt[t] -= k;
+                    rem -= k*t;
+                }
+                int rep = limited ? 1 : 1 << 30;
+                if (!limited)
+                    for (int t = 1; t <= 8; t++) if (take[t]) rep = min(rep, before[t]/take[t]);
+                unsigned comp = 0;
+                for (int t = 1; t <= 8; t++) {
+                    cnt[t] = before[t]-rep*take[t];
+                    comp |= (unsigned)take[t] << (4*(t-1));
+                }
+                if (nr == AG_RMAX) { ok = 0; break; }
+                run_comp[nr] = comp;
+                run_rep[nr] = rep;
+                run_bin0[nr] = bins;
+                nr++;
+                bins += rep;
+            }
+        }
+        s_nruns = nr;
+        s_bins = bins;
+        s_fallback = !ok || bins > J.budget;
+    }
+    __syncthreads();
+    if (!s_fallback) {
+        // wave and position of every group from its size class and rank
+        int run_c[9];
+        for (int s = 1; s <= 8; s++) run_c[s] = 0;
+        for (int l = c0; l < c1; l++) {
+            if (grp[l] != l) continue;
+            gbin[l] = -1;
+            const int s = gsz[l];
+            const int r = cls_count(plo, phi, s)+run_c[s];
+            run_c[s]++;
+            int acc = 0;
+            for (int q = 0; q < s_nruns; q++) {
+                const unsigned comp = run_comp[q];
+                const int ts = (int)((comp >> (4*(s-1))) & 0xF);
+                if (!ts) continue;
+                const int here = ts*run_rep[q];
+                if (r < acc+here) {
+                    const int u = r-acc, bin = run_bin0[q]+u/ts;
+                    int off = 0;
+                    for (int t = 8; t > s; t--) off += t*(int)((comp >> (4*(t-1))) & 0xF);
+                    off += (u%ts)*s;
+                    gbin[l] = (short)bin;
+                    goff[l] = (unsigned char)off;
+                    break;
+                }
+                acc += here;
+            }
+            if (gbin[l] < 0 || gbin[l] >= J.budget || goff[l]+s > AGG_CAP) atomicOr(&s_fallback, 1);
+        }
+    }
+    __syncthreads();
+    const int fallback = s_fallback;
+    // member lists (list heads reuse `best`)
+    int* head = reinterpret_cast<int*>(best);
+    for (int l = tid; l < n; l += T) head[l] = -1;
+    __syncthreads();
+    for (int l = tid; l < n; l += T) nxt[l] = (short)atomicExch(&head[grp[l]], l);
+    __syncthreads();
+    // slots: every used slot and every padding slot of the waves in use is
+    // written exactly once (perm) and every particle gets its slot (iperm)
+    for (int l = tid; l < n; l += T) {
+        if (fallback) {
+            G.iperm[poff+l] = J.soff+l;
+            G.perm[J.soff+l] = poff+l;
+            continue;
+        }
+        if (grp[l] != l) continue;
+        int mem[AGG_CAP], m = 0;
+        for (int v = head[l]; v >= 0 && m < AGG_CAP; v = nxt[v]) mem[m++] = v;
+        for (int a = 1; a < m; a++) {            // ascending particle order
+            const int v = mem[a];
+            int b = a-1;
+            while (b >= 0 && mem[b] > v) { mem[b+1] = mem[b]; b--; }
+            mem[b+1] = v;
+        }
+        const int base = J.soff+8*gbin[l]+goff[l];
+        for (int a = 0; a < m; a++) {
+            G.iperm[poff+mem[a]] = base+a;
+            G.perm[base+a] = poff+mem[a];
+        }
+    }
+    if (fallback) {
+        for (int q = n+tid; q < AGG_CAP*((n+AGG_CAP-1)/AGG_CAP); q += T) G.perm[J.soff+q] = -1;
+    } else {
+        for (int b = tid; b < s_bins; b += T) {
+            int fill = AGG_CAP;
+            for (int q = 0; q < s_nruns; q++)
+                if (b >= run_bin0[q] && b < run_bin0[q]+run_rep[q]) {
+                    fill = 0;
+                    for (int t = 1; t <= 8; t++) fill += t*(int)((run_comp[q] >> (4*(t-1))) & 0xF);
+                    break;
+                }
+            for (int q = fill; q < AGG_CAP; q++) G.perm[J.soff+AGG_CAP*b+q] = -1;
+        }
+    }
+    if (tid == 0) {
+        G.waves[J.inst] = fallback ? (n+AGG_CAP-1)/AGG_CAP : s_bins;
+        scs[J.inst].group_age = 0;
+    }
+}
+
+void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, int njobs, const int* inst_of,
+                      const InstConst* IC, DevScalars* sc, int interval, int nslots, hipStream_t s)
+{
+    if (njobs <= 0) return;
+    k_agg_weight<<<(G.pcap+255)/256, 256, 0, s>>>(A, inst_of, sc, interval, G.wq, G.pcap);
+    k_agg_adjacency<<<(nslots+255)/256, 256, 0, s>>>(A, G.wq, inst_of, IC, sc, interval, G.adj, nslots);
+    k_agg_group<<<njobs, AGB_THREADS, 0, s>>>(A, G, jobs, sc, interval);
+}
+
+void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s)
+{
+    if (ninst > 0) k_agg_age<<<(ninst+63)/64, 64, 0, s>>>(sc, ninst);
+}
+
+// ------------------------------------------------------------------ wave inverses
+__device__ inline double readlane_d(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// One wavefront per solver wave: the 48 x 48 diagonal block of the wave's
+// particles (every coupling between them) is assembled row by row in LDS
+// from the FP64 blocks, then inverted in FP64 by Gauss-Jordan elimination
+// without pivoting (the block is SPD), lane r holding row r in registers and
+// the pivot row broadcast with readlane.  Padding positions are the identity
+// during the elimination and zero in the result, which is written as the
+// packed lower triangle in FP32.  A block that is not positive definite in
+// the elimination falls back to the 6x6 block-Jacobi inverses.
+__global__ __launch_bounds__(WAVE) void k_agg_inverse(BsrView A, AggView G)
+{
+    __shared__ double M[AGG_DIM*AGG_DIM];
+    __shared__ int mem[AGG_CAP];
+    const int w = blockIdx.x;
+    const int inst = G.wave_inst[w];
+    if (inst < 0 || G.wave_local[w] >= G.waves[inst]) return;
+    const int t = threadIdx.x;
+    if (t < AGG_CAP) mem[t] = G.perm[AGG_CAP*w+t];
+    for (int a = t; a < AGG_DIM*AGG_DIM; a += WAVE) M[a] = 0.0;
+    __syncthreads();
+    if (t < AGG_DIM) {
+        const int p = t/6, a = t%6, i = mem[p];
+        double* row = M+t*AGG_DIM;
+        if (i >= 0) {
+            double ea[6] = {0, 0, 0, 0, 0, 0}, y[6] = {0, 0, 0, 0, 0, 0};
+            ea[a] = 1;
+            dmul<double, double>(A.diag64+15*(size_t)i, ea, y);        // row a of the symmetric D
+            for (int b = 0; b < 6; b++) row[6*p+b] = y[b];
+            // pair (j, i), j < i: B at (row i, col j), row a of B
+            for (int e = A.tptr[i]; e < A.tptr[i+1]; e++) {
+                const int k = A.tidx[e];
+                const int q = G.iperm[A.p0[k]]-AGG_CAP*w;
+                if (q < 0 || q >= AGG_CAP) continue;
+                double z[6] = {0, 0, 0, 0, 0, 0};
+                omul_t<double, double>(A.od64+18*(size_t)k, ea, z);
+                for (int b = 0; b < 6; b++) row[6*q+b] = z[b];
+            }
+            // pair (i, j), j > i: B at (row j, col i), row a of B^T = column a of B
+            for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) {
+                const int q = G.iperm[A.p1[k]]-AGG_CAP*w;
+                if (q < 0 || q >= AGG_CAP) continue;
+                double z[6] = {0, 0, 0, 0, 0, 0};
+                omul<double, double>(A.od64+18*(size_t)k, ea, z);
+                for (int b = 0; b < 6; b++) row[6*q+b] = z[b];
+            }
+        } else {
+            row[t] = 1.0;
+        }
+    }
+    __syncthreads();
+    double R[AGG_DIM];
+#pragma unroll
+    for (int c = 0; c < AGG_DIM; c++) R[c] = t < AGG_DIM ? M[t*AGG_DIM+c] : 0.0;
+    bool bad = false;
+#pragma unroll 1
+    for (int k = 0; k < AGG_DIM; k++) {
+        // own element k (register index k is not a compile-time constant)
+        double fo = 0.0;
+#pragma unroll
+        for (int j = 0; j < AGG_DIM; j++) fo = j == k ? R[j] : fo;
+        double piv = readlane_d(fo, k);
+        if (!(piv > 0) || !isfinite(piv)) { bad = true; piv = 1.0; }
+        const double rp = 1.0/piv;
+        const bool me = t == k;
+        const double f = me ? -1.0 : fo;
+        const double keep = me ? 0.0 : 1.0;
+#pragma unroll
+        for (int j = 0; j < AGG_DIM; j++) {
+            const double rkj = readlane_d(R[j], k)*rp;
+            R[j] = j == k ? -f*rp : keep*R[j]-f*rkj;
+        }
+    }
+    if (t >= AGG_DIM) return;
+    float* out = G.inv+(size_t)w*AGG_PACK+t*(t+1)/2;
+    const int pr = t/6, i = mem[pr];
+    if (bad) {
+        const float* mi = A.minv32+36*(size_t)(i >= 0 ? i : 0);
+        for (int c = 0; c <= t; c++) out[c] = (i >= 0 && c/6 == pr) ? mi[6*(t%6)+c%6] : 0.f;
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < AGG_DIM; c++)
+        if (c <= t) out[c] = (i >= 0 && mem[c/6] >= 0) ? (float)R[c] : 0.f;
+}
+
+void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStream_t s)
+{
+    if (nwaves > 0) k_agg_inverse<<<nwaves, WAVE, 0, s>>>(A, G);
+}
+
+// ------------------------------------------------------------------ persistent solver
+static constexpr int AS_WAVES = AGG_WAVES_PER_WG, AS_BLOCK = AS_WAVES*WAVE, AS_LPR = 8;
+static constexpr int AS_ROWS = AS_BLOCK/AS_LPR;     // solver slots per workgroup (12 waves of 8)
+static constexpr int AS_ENTRY_CAP = 1024;           // staged blocks per workgroup (78 KB)
+
+struct AsRow { int e0, ne; };       // staged entries of a slot row (ne < 0: not staged)
+
+// Same structure as the 6x6 block-Jacobi persistent solver (FP64 refinement
+// with the translation coarse correction around pipelined FP32 PCG, classic
+// PCG once a refinement pass fails to reduce the residual), with:
+//  * rows = solver slots: lane c < 6 of the 8-lane group of slot s owns
+//    component c of particle perm[s]; b and x stay in particle order, the
+//    exchanged Krylov vectors are in slot order;
+//  * the preconditioner z = W^-1 r for the wave W of the lane's wavefront:
+//    the 48 residual components go through a per-wave LDS vector and every
+//    lane forms its row of the packed symmetric inverse.
+__global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, const double* __restrict__ b,
+    double* __restrict__ x, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
+{
+    __shared__ int ent_j[AS_ENTRY_CAP];             // partner slot; ~slot for the transposed block
+    __shared__ float ent_b[AS_ENTRY_CAP*18];
+    __shared__ float s_inv[AS_WAVES*AGG_PACK];
+    __shared__ float s_vec[AS_WAVES*AGG_DIM];
+    __shared__ AsRow rows[AS_ROWS];
+    __shared__ double red[3*AS_WAVES+4];
+    __shared__ double s_ei[9];
+    __shared__ int s_fail;
+    const SolveJob J = jobs[blockIdx.x];
+    const int nw = Gv.waves[J.inst];
+    const int G = (nw+AS_WAVES-1)/AS_WAVES, wg = J.lwg;
+    if (wg >= G) return;                            // beyond the waves in use: not part of this solve
+    const int sbase = J.row0+wg*AS_ROWS, send = J.row0+AGG_CAP*nw;
+    DevScalars* sc = scs+J.inst;
+    unsigned* bar = w.barrier+128*(size_t)J.key;
+    unsigned seq = 0;
+    double g0 = 0, g1 = 0, g2 = 0;
+    const int lr = threadIdx.x/AS_LPR, c = threadIdx.x%AS_LPR;
+    const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
+    const int slot = sbase+lr;
+    const int i = slot < send ? Gv.perm[slot] : -1;
+    const bool live = i >= 0;
+    const bool own = live && c < 6;
+    const size_t gi = 6*(size_t)(live ? i : 0)+c;   // particle order (b, x)
+    const size_t gs = 6*(size_t)slot+c;             // slot order (exchanged vectors)
+    const int kk = c < 6 ? 6*(int)(lane/AS_LPR)+c : 0;
+    unsigned gen = ld_acquire(bar+32);
+    if (threadIdx.x == 0) s_fail = 0;
+    // ---- stage the blocks of this workgroup's rows and its waves' inverses
+    if (threadIdx.x < AS_ROWS) {
+        const int ss = sbase+threadIdx.x;
+        const int ii = ss < send ? Gv.perm[ss] : -1;
+        rows[threadIdx.x].ne = ii >= 0 ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
+    }
+    {
+        const float* src = Gv.inv+(size_t)(sbase/AGG_CAP)*AGG_PACK;
+        const int nload = min(AS_WAVES, nw-(sbase-J.row0)/AGG_CAP)*AGG_PACK;
+        for (int a = threadIdx.x; a < AS_WAVES*AGG_PACK; a += AS_BLOCK) s_inv[a] = a < nload ? src[a] : 0.f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int used = 0;
+        for (int t = 0; t < AS_ROWS; t++) {
+            const int deg = rows[t].ne;
+            if (deg > 0 && used+deg <= AS_ENTRY_CAP) { rows[t].e0 = used; used += deg; }
+            else { rows[t].e0 = 0; rows[t].ne = deg > 0 ? -1 : 0; }
+        }
+    }
+    __syncthreads();
+    if (live && rows[lr].ne > 0) {
+        const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+        for (int e = c; e < rows[lr].ne; e += AS_LPR) {
+            int k, js;
+            if (e < nl) { k = A.tidx[l0+e]; js = Gv.iperm[A.p0[k]]; }
+            else { k = u0+(e-nl); js = ~Gv.iperm[A.p1[k]]; }
+            const int s = rows[lr].e0+e;
+            ent_j[s] = js;
+            for (int m = 0; m < 18; m++) ent_b[18*s+m] = A.od32[18*(size_t)k+m];
+        }
+    }
+    // row c of D in registers (lanes 0..5)
+    float Dr[6] = {0, 0, 0, 0, 0, 0};
+    if (own) {
+        for (int m = 0; m < 6; m++) {
+            float e6[6] = {0, 0, 0, 0, 0, 0};
+            e6[m] = 1.f;
+            float y[6] = {0, 0, 0, 0, 0, 0};
+            dmul<float, float>(A.diag32+15*(size_t)i, e6, y);
+            Dr[m] = y[c];
+        }
+    }
+    __syncthreads();
+    const int ne = live ? rows[lr].ne : 0, e0 = live ? rows[lr].e0 : 0;
+
+    // component c of A32 v for this row: neighbours' v from vbuf (slot order),
+    // own v from the group's lanes
+    auto spmv32 = [&](const float* vbuf, float vown) -> float {
+        float y[6] = {0, 0, 0, 0, 0, 0};
+        if (live) {
+            if (ne >= 0) {
+                for (int e = c; e < ne; e += AS_LPR) {
+                    const int jj = ent_j[e0+e];
+                    const float* bl = ent_b+18*(e0+e);
+                    float xj[6];
+                    load6(vbuf+6*(size_t)(jj >= 0 ? jj : ~jj), xj);
+                    if (jj >= 0) omul<float, float>(bl, xj, y);
+                    else omul_t<float, float>(bl, xj, y);
+                }
+            } else {
+                const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+                const int nt = nl+(A.row_ptr[i+1]-u0);
+                for (int e = c; e < nt; e += AS_LPR) {
+                    float xj[6];
+                    if (e < nl) {
+                        const int k = A.tidx[l0+e];
+                        load6(vbuf+6*(size_t)Gv.iperm[A.p0[k]], xj);
+                        omul<float, float>(A.od32+18*(size_t)k, xj, y);
+                    } else {
+                        const int k = u0+(e-nl);
+                        load6(vbuf+6*(size_t)Gv.iperm[A.p1[k]], xj);
+                        omul_t<float, float>(A.od32+18*(size_t)k, xj, y);
+                    }
+                }
+            }
+        }
+        for (int m = 0; m < 6; m++) y[m] = sum8f(y[m]);
+        float vv[6];
+        bcast8_6(vown, c, vv);
+        float yc = 0, mine = 0;
+        for (int m = 0; m < 6; m++) { yc += Dr[m]*vv[m]; if (m == c) mine = y[m]; }
+        return own ? mine+yc : 0.f;
+    };
+    // z = W^-1 r on the lane's wave (LDS accesses of one wavefront execute in order)
+    auto precond = [&](float rv) -> float {
+        float* sv = s_vec+wid*AGG_DIM;
+        if (c < 6) sv[kk] = own ? rv : 0.f;
+        __builtin_amdgcn_wave_barrier();
+        const float* Mw = s_inv+wid*AGG_PACK;
+        float z = 0.f;
+#pragma unroll 8
+        for (int m = 0; m < AGG_DIM; m++) {
+            const int a = m <= kk ? kk*(kk+1)/2+m : m*(m+1)/2+kk;
+            z += Mw[a]*sv[m];
+        }
+        __builtin_amdgcn_wave_barrier();
+        return own ? z : 0.f;
+    };
+
+    double* part = Gv.part+AGG_PART_PER_BLOCK*(size_t)J.key;
+    int buf = 0;
+    auto publish3 = [&](double a, double bs, double cs) {
+        a = wave_sum_d(a); bs = wave_sum_d(bs); cs = wave_sum_d(cs);
+        if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bs; red[3*wid+2] = cs; }
+        drain_vmem();
+        __syncthreads();
+        double* pb = part+(size_t)buf*3*G;
+        seq++;
+        if (wid == 0) {
+            double t0 = lane < AS_WAVES ? red[3*lane] : 0.0;
+            double t1 = lane < AS_WAVES ? red[3*lane+1] : 0.0;
+            double t2 = lane < AS_WAVES ? red[3*lane+2] : 0.0;
+            const unsigned tg = seq & 0xFF;
+            t0 = tag_partial(wave_sum_d(t0), tg);
+            t1 = tag_partial(wave_sum_d(t1), tg);
+            t2 = tag_partial(wave_sum_d(t2), tg);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
+            const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
+            bool ok = false;
+            for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
+                double a0 = 0, a1 = 0, a2 = 0;
+                bool dn = true;
+                for (int q = lane; q < G; q += WAVE) {
+                    const unsigned long long v0 = ld_dev(pq+3*q), v1 = ld_dev(pq+3*q+1), v2 = ld_dev(pq+3*q+2);
+                    dn &= (v0 & 0xFF) == tg && (v1 & 0xFF) == tg && (v2 & 0xFF) == tg;
+                    a0 += untag_partial(v0); a1 += untag_partial(v1); a2 += untag_partial(v2);
+                }
+                if (__all(dn)) { g0 = a0; g1 = a1; g2 = a2; ok = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ok && lane == 0) s_fail = 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+    };
+    auto gather3_start = [&](double& t0, double& t1, double& t2) { t0 = g0; t1 = g1; t2 = g2; };
+    auto gather3_finish = [&](double& t0, double& t1, double& t2) {
+        if (wid == 0) {
+            t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
+            if (lane == 0) { red[3*AS_WAVES] = t0; red[3*AS_WAVES+1] = t1; red[3*AS_WAVES+2] = t2; }
+        }
+        __syncthreads();
+        t0 = red[3*AS_WAVES]; t1 = red[3*AS_WAVES+1]; t2 = red[3*AS_WAVES+2];
+        __syncthreads();
+        buf ^= 1;
+    };
+    auto gsum3 = [&](double& a, double& bs, double& cs) {
+        publish3(a, bs, cs);
+        gather3_start(a, bs, cs);
+        gather3_finish(a, bs, cs);
+    };
+
+    // Galerkin correction onto the three uniform translations, as in the
+    // block-Jacobi solver: (A z_a) rows in registers, E^-1 in LDS
+    double AZ[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        double y[6] = {0, 0, 0, 0, 0, 0};
+        if (live) row_translation<AS_LPR>(A, i, c, a, y);
+        for (int m = 0; m < 6; m++) if (m == c && own) AZ[a] = y[m];
+    }
+    bool use_coarse;
+    {
+        double e00 = c == 0 ? AZ[0] : 0.0, e01 = c == 0 ? AZ[1] : 0.0, e02 = c == 0 ? AZ[2] : 0.0;
+        double e11 = c == 1 ? AZ[1] : 0.0, e12 = c == 1 ? AZ[2] : 0.0, e22 = c == 2 ? AZ[2] : 0.0;
+        gsum3(e00, e11, e22);
+        gsum3(e01, e02, e12);
+        const double c00 = e11*e22-e12*e12, c01 = e02*e12-e01*e22, c02 = e01*e12-e02*e11;
+        const double det = e00*c00+e01*c01+e02*c02;
+        use_coarse = det > 0 && isfinite(det) && e00 > 0;
+        const double id = use_coarse ? 1.0/det : 0.0;
+        if (threadIdx.x == 0) {
+            s_ei[0] = c00*id; s_ei[1] = c01*id; s_ei[2] = c02*id;
+            s_ei[4] = (e00*e22-e02*e02)*id; s_ei[5] = (e01*e02-e00*e12)*id; s_ei[8] = (e00*e11-e01*e01)*id;
+            s_ei[3] = s_ei[1]; s_ei[6] = s_ei[2]; s_ei[7] = s_ei[5];
+        }
+        __syncthreads();
+    }
+
+    double bb = 0, rr64 = 0;
+    int outer, inner_total = 0;
+    bool converged = false, use_classic = false;
+    for (outer = 0; outer <= o.max_outer; outer++) {
+        double yv[6] = {0, 0, 0, 0, 0, 0};
+        if (live) spmv_row<double, double, AS_LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, c, yv);
+        double r64 = 0, bi = 0;
+        if (own) {
+            for (int m = 0; m < 6; m++) if (m == c) r64 = yv[m];
+            bi = b[gi];
+            r64 = bi-r64;
+        }
+        double s_rr = r64*r64, s_bb = bi*bi, zr0 = c == 0 ? r64 : 0.0;
+        gsum3(s_rr, s_bb, zr0);
+        if (s_fail) break;
+        if (outer == 0) bb = s_bb;
+        if (bb == 0) {
+            if (own) x[gi] = 0.0;
+            rr64 = 0;
+            converged = true;
+            break;
+        }
+        if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
+        rr64 = s_rr;
+        converged = sqrt(rr64) <= o.tol*sqrt(bb);
+        if (converged || outer == o.max_outer) break;
+        if (use_coarse) {
+            double zr1 = c == 1 ? r64 : 0.0, zr2 = c == 2 ? r64 : 0.0, z3 = 0;
+            gsum3(zr1, zr2, z3);
+            double cv[3];
+            for (int a = 0; a < 3; a++) cv[a] = s_ei[3*a]*zr0+s_ei[3*a+1]*zr1+s_ei[3*a+2]*zr2;
+            if (own) {
+                if (c < 3) x[gi] += cv[c];
+                r64 -= AZ[0]*cv[0]+AZ[1]*cv[1]+AZ[2]*cv[2];
+            }
+        }
+        const double scale = 1.0/sqrt(rr64);
+        float d = 0.f;
+        int it = 0;
+        if (!use_classic) {
+            float* mbuf[2] = {Gv.zb, Gv.qb};
+            float r = own ? (float)(r64*scale) : 0.f;
+            float u = precond(r);
+            if (own) mbuf[1][gs] = u;
+            double s_r2 = (double)r*r, z0 = 0, z1 = 0;
+            gsum3(s_r2, z0, z1);
+            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            float wv = spmv32(mbuf[1], u);
+            float zv = 0, qv = 0, sv = 0, pv = 0;
+            double gamma_old = 1, alpha_old = 1;
+            int par = 0;
+            for (it = 0; it < o.max_inner; it++) {
+                float mv = precond(wv);
+                if (own) mbuf[par][gs] = mv;
+                publish3((double)r*u, (double)wv*u, (double)r*r);
+                double g, dl, rho;
+                gather3_start(g, dl, rho);
+                float nv = spmv32(mbuf[par], mv);
+                gather3_finish(g, dl, rho);
+                if (s_fail || rho <= stop2) break;
+                double beta, alpha;
+                if (it > 0) {
+                    beta = g/gamma_old;
+                    double den = dl-beta*g/alpha_old;
+                    if (!(den > 0)) break;
+                    alpha = g/den;
+                } else {
+                    if (!(dl > 0)) break;
+                    beta = 0;
+                    alpha = g/dl;
+                }
+                const float bf = (float)beta, af = (float)alpha;
+                zv = nv+bf*zv;
+                qv = mv+bf*qv;
+                sv = wv+bf*sv;
+                pv = u+bf*pv;
+                d += af*pv;
+                r -= af*sv;
+                u -= af*qv;
+                wv -= af*zv;
+                gamma_old = g;
+                alpha_old = alpha;
+                par ^= 1;
+            }
+        } else {
+            float r = own ? (float)(r64*scale) : 0.f;
+            float z = precond(r), p = z;
+            if (own) Gv.zb[gs] = z;
+            double s_rz = (double)r*z, s_r2 = (double)r*r, z0 = 0;
+            gsum3(s_rz, s_r2, z0);
+            double rz = s_rz;
+            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            float q = spmv32(Gv.zb, z);
+            double s_pq = (double)p*q, z1 = 0, z2 = 0;
+            gsum3(s_pq, z1, z2);
+            double pq = s_pq;
+            for (it = 0; it < o.max_inner; it++) {
+                if (s_fail || !(pq > 0)) break;
+                const float alpha = (float)(rz/pq);
+                d += alpha*p;
+                r -= alpha*q;
+                z = precond(r);
+                if (own) Gv.zb[gs] = z;
+                double s_rz2 = (double)r*z, s_rr2 = (double)r*r, z3 = 0;
+                gsum3(s_rz2, s_rr2, z3);
+                if (s_rr2 <= stop2) { it++; break; }
+                const float beta = (float)(s_rz2/rz);
+                rz = s_rz2;
+                float wz = spmv32(Gv.zb, z);
+                p = z+beta*p;
+                q = wz+beta*q;
+                double s_pq2 = (double)p*q, z4 = 0, z5 = 0;
+                gsum3(s_pq2, z4, z5);
+                pq = s_pq2;
+            }
+        }
+        inner_total += it;
+        if (own) x[gi] += (double)d*sqrt(rr64);
+        tree_barrier(bar, G, wg, gen, &s_fail);
+        if (s_fail) break;
+    }
+    if (wg == 0 && threadIdx.x == 0) {
+        sc->res_norm = sqrt(rr64);
+        sc->rhs_norm = sqrt(bb);
+        sc->iters_inner = inner_total;
+        sc->iters_outer = outer;
+        sc->total_inner += inner_total;
+        if (!converged) atomicOr(&sc->error, E_NOT_CONVERGED);
+    }
+    if (threadIdx.x == 0 && s_fail) atomicOr(&sc->error, E_SYNC_TIMEOUT);
+}
+
+int agg_solver_max_groups()
+{
+    static int cached = 0;
+    if (!cached) {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_agg, AS_BLOCK, 0) != hipSuccess || per < 1)
+            per = 1;
+        cached = ncu*per;
+    }
+    return cached;
+}
+
+hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double* x, SolverWork& w,
+                     DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s)
+{
+    if (nwg > agg_solver_max_groups()) return hipErrorCooperativeLaunchTooLarge;
+    BsrView Av = A;
+    AggView Gv = G;
+    SolverWork wv = w;
+    SolveOpts ov = o;
+    // no stale round tags in any instance's partial records
+    launch_fill_double(G.part, 0.0, G.part_len, s);
+    void* args[] = {&Av, &Gv, (void*)&b, (void*)&x, &wv, &sc, &ov, (void*)&jobs};
+    return hipLaunchCooperativeKernel((const void*)k_solve_agg, dim3(nwg), dim3(AS_BLOCK), args, 0, s);
+}
+
+}  // namespace lfd

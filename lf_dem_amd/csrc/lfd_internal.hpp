@@ -32,7 +32,7 @@ struct DevScalars {
     int in_predictor;
     int iters_inner, iters_outer;       // last solve
     int total_inner;                    // accumulated over the instance's life
-    int pad;
+    int group_age;                      // steps since the preconditioner grouping (-1: none yet)
     Flow flow;                          // imposed flow at the current shear rate
     // stress control (System::computeShearRate, System.cpp:2017-2055)
     double target_stress, init_strain_limit, init_rate_limit;
@@ -76,8 +76,47 @@ struct SolveJob {
     int inst;      // instance (index into the DevScalars array)
     int lwg;       // workgroup index within the instance
     int G;         // workgroups of the instance
-    int row0;      // first slot of the instance
-    int row_end;   // one past its last particle
+    int row0;      // first slot of the instance (solver slot for the aggregate solver)
+    int row_end;   // one past its last particle (budget end for the aggregate solver)
+    int key;       // first 128-slot particle block of the instance: its partial records and barrier words
+    int pad[2];
+};
+
+// Aggregate block-Jacobi preconditioner.  Strongly coupled particles are
+// grouped (at most 8 per group) and groups are packed into "waves" of at most
+// 8 particles; the solver's rows are the waves' slots (8 per wave, -1 for
+// padding), so one 64-lane wavefront holds one wave and applies the inverse
+// of the wave's 48 x 48 diagonal block.  Each 128-particle slot block of an
+// instance owns AGG_SLOTS_PER_BLOCK solver slots (AGG_SLOTS_PER_BLOCK / 8
+// waves), enough for any packing of at least 6.4 particles per wave; a
+// grouping that needs more falls back to consecutive particles, 8 per wave.
+static constexpr int AGG_CAP = 8;                  // particles per wave
+static constexpr int AGG_DIM = 6*AGG_CAP;          // rows of a wave block
+static constexpr int AGG_PACK = AGG_DIM*(AGG_DIM+1)/2;
+static constexpr int AGG_SLOTS_PER_BLOCK = 160;
+static constexpr int AGG_WAVES_PER_WG = 12;        // wavefronts (waves) per solver workgroup
+static constexpr int AGG_NMAX = 8192;              // largest instance the grouping kernel takes
+static constexpr int AGG_PART_PER_BLOCK = 12;      // partial-record doubles per 128-slot block
+
+struct AggView {
+    int* perm;             // [solver slots] particle slot, -1 for padding
+    int* iperm;            // [particle slots] solver slot, -1 outside aggregate instances
+    int* waves;            // [instances] waves in use
+    const int* wave_inst;  // [solver waves] owning instance, -1 when free
+    const int* wave_local; // [solver waves] wave index within the owning instance
+    unsigned* adj;         // [2 * pair capacity] per particle: incident pairs as (weight << 14 | partner)
+    unsigned* wq;          // [pair capacity] quantised coupling weight of each pair
+    int pcap;
+    float* inv;            // [solver waves * AGG_PACK] wave block inverse, packed lower triangle (FP32)
+    float* zb;             // [6 * solver slots] exchanged Krylov vectors
+    float* qb;
+    double* part;          // partial records, AGG_PART_PER_BLOCK per 128-slot block
+    size_t part_len;
+};
+
+// One instance of the grouping kernel.
+struct AggJob {
+    int inst, poff, n, soff, budget;   // budget: waves available to the instance
     int pad[3];
 };
 
@@ -188,6 +227,17 @@ static constexpr int LDS_ROWS = 128;
 hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                      const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
 int lds_solver_max_groups();
+// aggregate preconditioner (lfd_agg.hip)
+// Grouping of the instances in jobs that are due (never grouped, or grouped
+// `interval` or more steps ago); inst_of / IC null: one instance at slot 0.
+void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, int njobs, const int* inst_of,
+                      const InstConst* IC, DevScalars* sc, int interval, int nslots, hipStream_t s);
+// one step more since each instance's grouping
+void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s);
+void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStream_t s);
+hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double* x, SolverWork& w,
+                     DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
+int agg_solver_max_groups();
 hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
 void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);

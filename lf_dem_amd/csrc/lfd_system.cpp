@@ -68,6 +68,8 @@ extern "C" void lfd_default_solve_opts(lfd_solve_opts* o)
     o->lanes_per_row = 4;
     o->groups = 0;
     o->kind = LFD_SOLVER_MIXED;
+    o->precond = LFD_PRECOND_AGGREGATE;
+    o->regroup_interval = 4;
 }
 
 extern "C" int lfd_device_count(void)
@@ -141,7 +143,21 @@ struct Solver {
     int jobs_cap = 0;
     std::vector<std::pair<int, int>> launches;
     int single = -1;          // the only instance, or -1 when there are several
-    hipError_t alloc(int n_, int pcap_)
+    // aggregate preconditioner: solver slots, groups and inverses, the
+    // grouping jobs (one per instance that uses it) and the aggregate
+    // solver's job table and launches
+    int precond = LFD_PRECOND_AGGREGATE;
+    int scap = 0;             // solver slots (AGG_SLOTS_PER_BLOCK per 128 particle slots)
+    AggView agg{};
+    int *wave_inst = nullptr, *wave_local = nullptr;
+    std::vector<int> h_wave_inst, h_wave_local;
+    AggJob* agg_jobs = nullptr;
+    int n_agg_jobs = 0;
+    SolveJob* ajobs = nullptr;
+    int ajobs_cap = 0;
+    std::vector<std::pair<int, int>> agg_launches;
+    int regroup_interval = 4; // steps between an instance's groupings (the inverses follow every assembly)
+    hipError_t alloc(int n_, int pcap_, int icap = 0)
     {
         n = n_; pcap = pcap_;
         size_t m = 6*(size_t)n;
@@ -165,6 +181,32 @@ struct Solver {
         A(hipMemset(w.barrier, 0, nbar*sizeof(unsigned)));
         jobs_cap = rb+1;
         A(dalloc(&jobs, jobs_cap));
+        if (pcap > 0) {
+            scap = rb*AGG_SLOTS_PER_BLOCK;
+            const int nwv = scap/AGG_CAP;
+            A(dalloc(&agg.perm, scap)); A(dalloc(&agg.iperm, n)); A(dalloc(&agg.waves, std::max(icap, rb)+1));
+            A(dalloc(&wave_inst, nwv)); A(dalloc(&wave_local, nwv)); A(dalloc(&agg.adj, 2*(size_t)pcap));
+            A(dalloc(&agg.wq, pcap));
+            agg.pcap = pcap;
+            A(dalloc(&agg.inv, (size_t)nwv*AGG_PACK));
+            A(dalloc(&agg.zb, 6*(size_t)scap)); A(dalloc(&agg.qb, 6*(size_t)scap));
+            agg.part_len = (size_t)AGG_PART_PER_BLOCK*rb+64;
+            A(dalloc(&agg.part, agg.part_len));
+            A(dalloc(&agg_jobs, rb+1));
+            ajobs_cap = 2*rb+64;
+            A(dalloc(&ajobs, ajobs_cap));
+            agg.wave_inst = wave_inst;
+            agg.wave_local = wave_local;
+            h_wave_inst.assign(nwv, -1);
+            h_wave_local.assign(nwv, 0);
+            if (e == hipSuccess) {
+                launch_fill_int(agg.waves, 0, std::max(icap, rb)+1, nullptr);
+                launch_fill_int(wave_inst, -1, nwv, nullptr);
+                launch_fill_int(agg.iperm, -1, n, nullptr);
+                launch_fill_int(agg.perm, -1, scap, nullptr);
+                e = hipDeviceSynchronize();
+            }
+        }
 #undef A
         return e;
     }
@@ -172,7 +214,8 @@ struct Solver {
     {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
                         w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials,
-                        w.barrier, jobs};
+                        w.barrier, jobs, agg.perm, agg.iperm, agg.waves, wave_inst, wave_local, agg.adj, agg.wq, agg.inv, agg.zb,
+                        agg.qb, agg.part, agg_jobs, ajobs};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (auto& e : prof) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         prof.clear();
@@ -188,18 +231,51 @@ struct Solver {
 
 struct InstSpan { int inst, row0, n; };
 
-// Job table of the batched solver: ceil(n/128) workgroups per instance, packed
-// in slot order into cooperative launches that fit the device.  An instance
-// too large for one launch can only run alone, on the global-memory form.
+// Job table of the batched solvers.  An instance that can take the aggregate
+// preconditioner gets its budget of aggregate-solver workgroups (those beyond
+// the waves in use exit at once); the others get ceil(n/128) workgroups of the
+// 6x6 block-Jacobi solver.  Each kind is packed in slot order into cooperative
+// launches that fit the device.  An instance too large for one launch can
+// only run alone, on the global-memory form.
 static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s)
 {
     S.launches.clear();
+    S.agg_launches.clear();
+    S.n_agg_jobs = 0;
     S.single = spans.size() == 1 ? spans[0].inst : -1;
     if (spans.empty()) return LFD_OK;
     const int maxg = std::min(lds_solver_max_groups(), S.w.max_groups);
-    std::vector<SolveJob> jobs;
-    int first = 0;
+    const int amaxg = S.scap > 0 ? agg_solver_max_groups() : 0;
+    std::vector<SolveJob> jobs, ajobs;
+    std::vector<AggJob> gjobs;
+    std::fill(S.h_wave_inst.begin(), S.h_wave_inst.end(), -1);
+    int first = 0, afirst = 0;
     for (const InstSpan& sp : spans) {
+        const int m = (sp.n+LDS_ROWS-1)/LDS_ROWS;
+        const int bw = m*AGG_SLOTS_PER_BLOCK/AGG_CAP;
+        const int bg = (bw+AGG_WAVES_PER_WG-1)/AGG_WAVES_PER_WG;
+        if (S.scap > 0 && S.precond == LFD_PRECOND_AGGREGATE && S.kind == LFD_SOLVER_MIXED && sp.n <= AGG_NMAX &&
+            bg <= amaxg) {
+            const int soff = sp.row0/LDS_ROWS*AGG_SLOTS_PER_BLOCK;
+            if ((int)ajobs.size()-afirst+bg > amaxg) {
+                S.agg_launches.emplace_back(afirst, (int)ajobs.size()-afirst);
+                afirst = (int)ajobs.size();
+            }
+            for (int g = 0; g < bg; g++) {
+                SolveJob j{};
+                j.inst = sp.inst; j.lwg = g; j.G = bg; j.row0 = soff; j.row_end = soff+AGG_CAP*bw;
+                j.key = sp.row0/LDS_ROWS;
+                ajobs.push_back(j);
+            }
+            AggJob a{};
+            a.inst = sp.inst; a.poff = sp.row0; a.n = sp.n; a.soff = soff; a.budget = bw;
+            gjobs.push_back(a);
+            for (int v = 0; v < bw; v++) {
+                S.h_wave_inst[soff/AGG_CAP+v] = sp.inst;
+                S.h_wave_local[soff/AGG_CAP+v] = v;
+            }
+            continue;
+        }
         const int G = (sp.n+LDS_ROWS-1)/LDS_ROWS;
         if (G > maxg) {
             if (spans.size() == 1) return LFD_OK;
@@ -212,12 +288,27 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
         for (int g = 0; g < G; g++) {
             SolveJob j{};
             j.inst = sp.inst; j.lwg = g; j.G = G; j.row0 = sp.row0; j.row_end = sp.row0+sp.n;
+            j.key = sp.row0/LDS_ROWS;
             jobs.push_back(j);
         }
     }
-    S.launches.emplace_back(first, (int)jobs.size()-first);
-    if ((int)jobs.size() > S.jobs_cap) return fail(LFD_ERR_CAPACITY, "solver job table capacity exceeded");
-    HIPCHK(hipMemcpyAsync(S.jobs, jobs.data(), jobs.size()*sizeof(SolveJob), hipMemcpyHostToDevice, s));
+    if ((int)jobs.size() > first) S.launches.emplace_back(first, (int)jobs.size()-first);
+    if ((int)ajobs.size() > afirst) S.agg_launches.emplace_back(afirst, (int)ajobs.size()-afirst);
+    if ((int)jobs.size() > S.jobs_cap || (int)ajobs.size() > S.ajobs_cap)
+        return fail(LFD_ERR_CAPACITY, "solver job table capacity exceeded");
+    if (!jobs.empty())
+        HIPCHK(hipMemcpyAsync(S.jobs, jobs.data(), jobs.size()*sizeof(SolveJob), hipMemcpyHostToDevice, s));
+    if (!ajobs.empty()) {
+        HIPCHK(hipMemcpyAsync(S.ajobs, ajobs.data(), ajobs.size()*sizeof(SolveJob), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(S.agg_jobs, gjobs.data(), gjobs.size()*sizeof(AggJob), hipMemcpyHostToDevice, s));
+    }
+    if (S.scap > 0) {
+        HIPCHK(hipMemcpyAsync(S.wave_inst, S.h_wave_inst.data(), S.h_wave_inst.size()*sizeof(int),
+                              hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(S.wave_local, S.h_wave_local.data(), S.h_wave_local.size()*sizeof(int),
+                              hipMemcpyHostToDevice, s));
+    }
+    S.n_agg_jobs = (int)gjobs.size();
     HIPCHK(hipStreamSynchronize(s));
     return LFD_OK;
 }
@@ -247,7 +338,8 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
                      hipStream_t s)
 {
     BsrView A = S.view(npairs_dev);
-    if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL || S.launches.empty()) {
+    if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL ||
+        (S.launches.empty() && S.agg_launches.empty())) {
         if (S.single < 0) return fail(LFD_ERR_UNSUPPORTED, "this solver form takes a single instance");
         sc += S.single;
     }
@@ -270,7 +362,9 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, s));
     }
-    if (S.kind == LFD_SOLVER_MIXED && !S.launches.empty()) {
+    if (S.kind == LFD_SOLVER_MIXED && !(S.launches.empty() && S.agg_launches.empty())) {
+        for (auto& L : S.agg_launches)
+            HIPCHK(solve_agg(A, S.agg, b, x, S.w, sc, S.ajobs+L.first, L.second, S.opts, s));
         for (auto& L : S.launches) HIPCHK(solve_lds(A, b, x, S.w, sc, S.jobs+L.first, L.second, S.opts, s));
     } else {
         HIPCHK(solve_mixed(A, b, x, S.w, sc, S.opts, s));
@@ -378,6 +472,8 @@ static int batch_init(lfd_batch* B, int slot_cap, int cell_cap, int inst_cap, co
     if (!o) o = &od;
     B->S.opts = to_opts(o);
     B->S.kind = o->kind;
+    B->S.precond = o->precond;
+    B->S.regroup_interval = std::max(1, o->regroup_interval);
     B->control = control;
     B->M = round_up(std::max(slot_cap, 1), LDS_ROWS);
     B->ccap = std::max(cell_cap, 1);
@@ -403,7 +499,7 @@ static int batch_init(lfd_batch* B, int slot_cap, int cell_cap, int inst_cap, co
     A(dalloc(&B->ca.nb, 64*(size_t)ccap)); A(dalloc(&B->ca.nnb, ccap)); A(dalloc(&B->ca.nstatic, ccap));
     A(dalloc(&B->ca.ctype, ccap)); A(dalloc(&B->ca.cell_inst, ccap));
     A(dalloc(&B->d_ic, B->icap)); A(dalloc(&B->sc, B->icap)); A(dalloc(&B->bs, 1));
-    A(B->S.alloc(M, pcap));
+    A(B->S.alloc(M, pcap, B->icap));
 #undef A
     if (e != hipSuccess) return fail(LFD_ERR_HIP, std::string("batch allocation: ")+hipGetErrorString(e));
     B->ca.ncell = ccap;
@@ -604,6 +700,14 @@ static void bat_assemble(lfd_batch* B, int nparts = 0)
     launch_assemble(B->pairs[B->cur], B->pos, B->M, B->inst_of, B->d_ic, B->sc, B->bs, B->S.row_ptr, B->S.tptr,
                     B->S.tidx, B->S.diag64, B->S.diag32, B->S.od64, B->S.od32, B->S.minv64, B->S.minv32, B->rhs,
                     B->pdb, B->prhs, B->pcap, B->rhs_parts, nparts, B->s);
+    // aggregate preconditioner: the groups of the instances that are due, then
+    // the wave inverses of this matrix
+    if (B->S.n_agg_jobs > 0) {
+        const BsrView A = B->S.view(&B->bs->npairs);
+        launch_agg_group(A, B->S.agg, B->S.agg_jobs, B->S.n_agg_jobs, B->inst_of, B->d_ic, B->sc,
+                         B->S.regroup_interval, B->M, B->s);
+        launch_agg_inverse(A, B->S.agg, B->S.scap/AGG_CAP, B->s);
+    }
 }
 
 // calcStressPerParticle + calcStress (mode 1) or computeShearRate (mode 0)
@@ -652,6 +756,7 @@ static int bat_step(lfd_batch* B, int calc_stress = 0, int adapt_first = 1)
     // the corrector's
     int rc;
     if (adapt_first) launch_adapt_dt(B->sc, B->d_ic, B->ninst, 0, B->s);
+    launch_agg_age(B->sc, B->ninst, B->s);
     if ((rc = bat_velocities(B, 1))) return rc;
     launch_adapt_dt(B->sc, B->d_ic, B->ninst, 1, B->s);
     launch_boxing(B->sc, B->d_ic, B->ninst, B->s);
@@ -711,6 +816,7 @@ static int bat_add(lfd_batch* B, int np, const double* pos, const double* radius
     hs.dt = 1e-4;
     for (int k = 0; k < 3; k++) hs.shear_disp[k] = sd[k];
     make_flow(ic.C, ic.P.shear_rate, hs.flow);
+    hs.group_age = -1;
     hs.init_strain_limit = 0;       // System.cpp:90-91
     hs.init_rate_limit = 999;
     hs.strain_end = strain_end;
@@ -942,7 +1048,9 @@ extern "C" int lfd_system_set_opts(lfd_system* y, const lfd_solve_opts* o)
     if (!y || !o) return fail(LFD_ERR_ARG, "lfd_system_set_opts: bad argument");
     y->S.opts = to_opts(o);
     y->S.kind = o->kind;
-    return LFD_OK;
+    y->S.precond = o->precond;
+    y->S.regroup_interval = std::max(1, o->regroup_interval);
+    return set_jobs(y->S, live_spans(y), y->s);
 }
 
 extern "C" int lfd_system_first_update(lfd_system* y)
@@ -1215,7 +1323,9 @@ extern "C" int lfd_batch_remove(lfd_batch* B, int id)
 extern "C" int lfd_batch_step(lfd_batch* B, int nsteps)
 {
     if (!B || nsteps < 0) return fail(LFD_ERR_ARG, "lfd_batch_step: bad argument");
-    if (B->S.launches.empty()) return nsteps ? fail(LFD_ERR_ARG, "lfd_batch_step: empty batch") : LFD_OK;
+    bool any = false;
+    for (int k = 0; k < B->ninst; k++) any = any || B->inst[k].live;
+    if (!any) return nsteps ? fail(LFD_ERR_ARG, "lfd_batch_step: empty batch") : LFD_OK;
     for (int k = 0; k < nsteps; k++) {
         int rc = bat_step(B);
         if (rc) return rc;
@@ -1291,7 +1401,7 @@ extern "C" int lfd_batch_info(lfd_batch* B, int* live, int* slot_cap, int* cell_
     if (slot_cap) *slot_cap = B->M;
     if (cell_cap) *cell_cap = B->ccap;
     if (inst_cap) *inst_cap = B->icap;
-    if (solver_launches) *solver_launches = (int)B->S.launches.size();
+    if (solver_launches) *solver_launches = (int)(B->S.launches.size()+B->S.agg_launches.size());
     if (bytes) {
         size_t ps = 0, pc = 0;
         batch_unit_bytes(&ps, &pc);
@@ -1369,6 +1479,7 @@ extern "C" int lfd_stokes_set_opts(lfd_stokes* st, const lfd_solve_opts* o)
     if (!st || !o) return fail(LFD_ERR_ARG, "bad argument");
     st->S.opts = to_opts(o);
     st->S.kind = o->kind;
+    st->S.precond = o->precond;
     return LFD_OK;
 }
 
@@ -1416,6 +1527,7 @@ extern "C" int lfd_stokes_fill_done(lfd_stokes* st)
         Solver fresh;
         fresh.opts = st->S.opts;
         fresh.kind = st->S.kind;
+        fresh.precond = st->S.precond;
         st->S.release();
         st->S = fresh;
         HIPCHK(st->S.alloc(n, ncap));
@@ -1442,12 +1554,18 @@ extern "C" int lfd_stokes_fill_done(lfd_stokes* st)
     DevScalars h;
     std::memset(&h, 0, sizeof(h));
     h.npairs = P;
+    h.group_age = -1;
     HIPCHK(hipMemcpyAsync(st->sc, &h, sizeof(h), hipMemcpyHostToDevice, s));
     launch_prepare_matrix(n, st->d_npairs, st->S.diag64, st->S.diag32, st->S.od64, st->S.od32, st->S.minv64,
                           st->S.minv32, st->cap, st->sc, s);
     int rc = check_scalars(st->sc, s);
     if (rc) return rc;
     if ((rc = set_jobs(st->S, std::vector<InstSpan>{{0, 0, n}}, s))) return rc;
+    if (st->S.n_agg_jobs > 0) {
+        const BsrView A = st->S.view(st->d_npairs);
+        launch_agg_group(A, st->S.agg, st->S.agg_jobs, st->S.n_agg_jobs, nullptr, nullptr, st->sc, 1, n, s);
+        launch_agg_inverse(A, st->S.agg, st->S.scap/AGG_CAP, s);
+    }
     st->filled = true;
     return LFD_OK;
 }
