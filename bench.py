@@ -29,17 +29,37 @@ METRIC = "steps/s at N=4000 Ï†=0.56 vs plain GPU; p50/p99 step latency; rel.errâ
 HBM_PEAK_GBS = 8000.0
 
 
-def solver_bytes_per_iter(n, p):
-    """Algorithmic bytes of one inner FP32 PCG iteration (SpMV of packed
-    6x6 blocks in both directions + block-Jacobi apply + vector updates)."""
-    spmv = 2 * p * (72 + 4) + n * (60 + 8) + 6 * 4 * n       # blocks+index both ways, diag, x gather
-    vec = n * 6 * 4 * (4 + 3) + n * 144 + n * 6 * 4 * 4      # phase A r/w + M^-1, phase B p/q r/w
-    return spmv + vec
+def solver_bytes_per_iter(n, p, waves=0):
+    """Algorithmic bytes of one inner FP32 PCG iteration: SpMV with LF_DEM's
+    packed 6x6 blocks applied in both directions (18 floats + index per pair
+    block, 15 floats + index per diagonal block, neighbour 6-vectors), the
+    preconditioner (6x6 FP32 inverses per particle, or the packed 48x48 FP32
+    inverse of each 8-particle wave), and the pipelined-PCG vector updates."""
+    spmv = 2 * p * (72 + 4) + n * (60 + 8) + 6 * 4 * n
+    precond = (AGG_PACK * 4 * waves if waves else 144 * n) + 2 * 6 * 4 * n
+    vec = n * 6 * 4 * (4 + 3) + n * 6 * 4 * 4
+    return spmv + precond + vec
 
 
 def solver_bytes_per_outer(n, p):
     """FP64 residual r = b - R x: packed FP64 blocks both ways, diag, x, b, r."""
     return 2 * p * (144 + 4) + n * (120 + 8) + n * 6 * 8 * 3
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of the solver kernel from the committed rocprofv3
+    PMC summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by
+    the same counter on a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_pmc_traffic.txt" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+AGG_PACK = 48 * 49 // 2
 
 
 def run(args):
@@ -54,6 +74,7 @@ def run(args):
     S = capi.System(pos, rad, L, params=params, opts=opts)
     S.first_update()
     S.step(args.warmup)
+    waves = S.precond_waves()
     st0 = S.solve_stats()
     S.profile(True)
     t0 = time.perf_counter()
@@ -69,23 +90,28 @@ def run(args):
         t1 = time.perf_counter()
         S.step(1)
         lat.append((time.perf_counter() - t1) * 1e3)
-    lat = np.array(lat)
+    lat = np.array(lat) if lat else np.zeros(1)
     inner = st1["total_inner"] - st0["total_inner"]
     iters_per_solve = inner / max(launches, 1)
     # plain single-instance GPU form on the same configuration
-    P = capi.System(pos, rad, L, params=params, opts=capi.default_opts(kind=1, tol=args.tol, max_inner=20000))
-    P.first_update()
-    P.step(1)
-    t0 = time.perf_counter()
-    P.step(args.plain_steps)
-    plain_sps = args.plain_steps / (time.perf_counter() - t0)
-    del P
+    plain_sps = None
+    if args.plain_steps > 0:
+        P = capi.System(pos, rad, L, params=params, opts=capi.default_opts(kind=1, tol=args.tol, max_inner=20000))
+        P.first_update()
+        P.step(1)
+        t0 = time.perf_counter()
+        P.step(args.plain_steps)
+        plain_sps = args.plain_steps / (time.perf_counter() - t0)
+        del P
     # roofline of the dominant kernel (the persistent solver launch)
     avg_ms = solve_ms / max(launches, 1)
     outer_per_solve = float(max(st1["iters_outer"], 1))     # refinement passes of the last solve
-    alg_bytes = iters_per_solve * solver_bytes_per_iter(args.n, npairs) + \
+    alg_bytes = iters_per_solve * solver_bytes_per_iter(args.n, npairs, waves) + \
         outer_per_solve * solver_bytes_per_outer(args.n, npairs)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    kernel = ["k_solve_lds", "k_solve_agg"][args.precond]
+    default_workload = (args.n, args.phi, args.seed) == (4000, 0.56, 1)
+    traffic = measured_traffic(kernel, "n4000_phi056") if default_workload else None
     out = {
         "metric": METRIC, "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 / sps, 3),
@@ -95,16 +121,16 @@ def run(args):
                    "N": args.n, "phi": args.phi, "seed": args.seed, "pairs": npairs, "tol": args.tol,
                    "precond": ["block-jacobi", "aggregate"][args.precond],
                    "parallelism": "single GPU"},
-        "plain_gpu_steps_per_s": round(plain_sps, 3),
-        "speedup_vs_plain_gpu": round(sps / plain_sps, 2),
+        "plain_gpu_steps_per_s": round(plain_sps, 3) if plain_sps else None,
+        "speedup_vs_plain_gpu": round(sps / plain_sps, 2) if plain_sps else None,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
         "solver": {"inner_iters_per_solve": round(iters_per_solve, 1), "solve_launches": launches,
-                   "avg_solve_ms": round(avg_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": ["k_solve_lds", "k_solve_agg"][args.precond],
-                     "achieved": round(achieved, 1),
+                   "avg_solve_ms": round(avg_ms, 4), "precond_waves": waves},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+                     "traffic": round(traffic) if traffic else None,
+                     "algorithmic_bytes_per_launch": round(alg_bytes)},
         "confgen_s": round(gen_s, 2),
     }
     if not args.no_cpu:
@@ -349,6 +375,24 @@ def run_c4(args):
     print(json.dumps(out), flush=True)
 
 
+def run_copy(args):
+    """Counter calibration: device-to-device copies of 256 MiB (64 Mi float32)
+    whose HBM traffic is known, for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    x = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        y = x.clone()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / 3
+    del y
+    print(json.dumps({"metric": "device copy bandwidth (counter calibration)", "value": round(2 * x.numel() * 4 / el / 1e9, 1),
+                      "unit": "GB/s", "bytes_read": x.numel() * 4, "bytes_written": x.numel() * 4, "copies": 3}),
+          flush=True)
+
+
 def cpu_leg(pos, rad, L, params, opts, args):
     """FP64 direct (oracle) solve at t=0: error of the GPU velocities and the
     host-core time of one assembly + sparse Cholesky factorisation + solve."""
@@ -392,15 +436,17 @@ def main():
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
     ap.add_argument("--precond", type=int, default=None,
                     help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0)")
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "copy"), default="c2",
                     help="c2: N=4000 rate control (headline); c3: batched sweep of 64 instances; "
-                         "c4: served requests; c5: N=32000 stress control")
+                         "c4: served requests; c5: N=32000 stress control; copy: 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.precond is None:
         args.precond = 0 if args.workload == "c3" else 1
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)")
-    if args.workload == "c5":
+    if args.workload == "copy":
+        run_copy(args)
+    elif args.workload == "c5":
         run_c5(args)
     elif args.workload == "c3":
         run_c3(args)

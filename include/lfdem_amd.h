@@ -148,6 +148,10 @@ int lfd_system_get_cells(lfd_system *s, int *dims4, int *labels);
 int lfd_system_get_blocks(lfd_system *s, int cap, double *dblocks, double *odblocks, int *p0, int *p1);
 int lfd_system_get_rhs(lfd_system *s, double *rhs);
 int lfd_system_get_solve_stats(lfd_system *s, lfd_solve_stats *st, long long *total_inner);
+/* Waves (groups of up to 8 particles whose 48x48 diagonal block is inverted)
+ * of the aggregate preconditioner; 0 under 6x6 block-Jacobi.  No counterpart
+ * in the reference (its factorisation is CHOLMOD's). */
+int lfd_system_precond_waves(lfd_system *s, int *waves);
 int lfd_system_set_opts(lfd_system *s, const lfd_solve_opts *o);
 /* Solver-launch timing with HIP events on the library's stream: returns the
  * summed duration (ms) and count of the launches recorded since the previous
@@ -229,6 +233,8 @@ int lfd_batch_query(lfd_batch *b, int id, double *scalars8, lfd_solve_stats *st,
  * returns the count (at most cap). */
 int lfd_batch_status(lfd_batch *b, int cap, int *ids, double *strain, int *status);
 int lfd_batch_get_state(lfd_batch *b, int id, double *pos, double *vel6, double *na6, double *scalars8);
+/* As lfd_system_precond_waves, for instance id of a batch. */
+int lfd_batch_precond_waves(lfd_batch *b, int id, int *waves);
 int lfd_batch_get_pairs(lfd_batch *b, int id, int cap, int *p0, int *p1, int *state, double *geom5,
                         double *disp_tan);
 int lfd_batch_num_pairs(lfd_batch *b, int id);

@@ -62,6 +62,8 @@ SIGNATURES = {
     "lfd_system_get_blocks": ([_vp, ctypes.c_int, _dp, _dp, _ip, _ip], ctypes.c_int),
     "lfd_system_get_rhs": ([_vp, _dp], ctypes.c_int),
     "lfd_system_get_solve_stats": ([_vp, _vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "lfd_system_precond_waves": ([_vp, _ip], ctypes.c_int),
+    "lfd_batch_precond_waves": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
     "lfd_system_set_opts": ([_vp, _vp], ctypes.c_int),
     "lfd_system_profile": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
     "lfd_system_set_control": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
@@ -386,6 +388,12 @@ class System:
         v = np.zeros((self.np, 6))
         _check(lib().lfd_system_get_parts(self.h, self.PARTS.index(name), _d(v)))
         return v
+
+    def precond_waves(self):
+        """Waves of the aggregate preconditioner (0 under 6x6 block-Jacobi)."""
+        w = ctypes.c_int(0)
+        _check(lib().lfd_system_precond_waves(self.h, ctypes.byref(w)))
+        return w.value
 
     def solve_stats(self):
         st = SolveStats()

@@ -27,6 +27,30 @@ __device__ inline void atomic_max_nonneg(unsigned long long* addr, double v)
 
 __device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+// Max of non-negative doubles into sc[inst].*field over a whole wavefront (all
+// 64 lanes must reach this call; lanes with inst < 0 or v < 0 contribute
+// nothing).  When every contributing lane has the same instance the wave
+// reduces first and issues one atomic, otherwise each lane issues its own.
+template <unsigned long long DevScalars::*field>
+__device__ inline void wave_max_into(DevScalars* sc, int inst, double v)
+{
+    const bool has = inst >= 0 && v >= 0;
+    unsigned long long b = has ? (unsigned long long)__double_as_longlong(v) : 0ull;
+    const unsigned long long m = __ballot(has);
+    if (!m) return;
+    const int lead = __builtin_ctzll(m);
+    const int i0 = __shfl(inst, lead);
+    if (__all(!has || inst == i0)) {
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const unsigned long long t = __shfl_xor(b, o);
+            b = t > b ? t : b;
+        }
+        if ((int)lane_id() == lead) atomicMax(&(sc[i0].*field), b);
+    } else if (has) {
+        atomicMax(&(sc[inst].*field), b);
+    }
+}
+
 // ------------------------------------------------------------------ fills
 __global__ void k_fill_int(int* p, int v, size_t n)
 {
@@ -489,23 +513,28 @@ __global__ __launch_bounds__(256) void k_update_pairs(PairRec* pairs, const doub
 {
     int k = blockIdx.x*blockDim.x+threadIdx.x;
     int np = bs->npairs;
-    if (k >= np || k >= pcap) return;
-    const int p0 = pairs[k].p0;
-    const int inst = inst_of[p0];
-    if (only == -2 || (only >= 0 && inst != only)) { keep[k] = inst >= 0 ? 1 : 0; return; }
-    PairRec q = pairs[k];
-    const Consts& C = IC[inst].C;
-    DevScalars* si = sc+inst;
-    double4 x0 = pos[q.p0], x1 = pos[q.p1];
-    double a[3] = {x0.x, x0.y, x0.z}, b[3] = {x1.x, x1.y, x1.z};
-    double sd[3] = {si->shear_disp[0], si->shear_disp[1], si->shear_disp[2]};
-    double dt = dt_override >= 0 ? dt_override : si->dt;
-    const double vd[3] = {si->flow.vel_diff[0], si->flow.vel_diff[1], si->flow.vel_diff[2]};
-    int drop = pair_update(q, a, b, x0.w, x1.w, vel6+6*(size_t)q.p0, vel6+6*(size_t)q.p1, C, vd, sd, dt,
-                           in_predictor);
-    if (C.friction && (q.flags & F_CONTACT)) atomic_max_nonneg(&si->max_slide_bits, q.rsv_sq);
-    pairs[k] = q;
-    keep[k] = drop ? 0 : 1;
+    if (blockIdx.x*blockDim.x >= min(np, pcap)) return;      // whole workgroup beyond the pairs
+    const bool valid = k < np && k < pcap;
+    const int inst = valid ? inst_of[pairs[k].p0] : -1;
+    const bool work = valid && inst >= 0 && only != -2 && (only < 0 || inst == only);
+    double slide = -1.0;                       // squared sliding velocity of a frictional contact
+    if (valid && !work) keep[k] = inst >= 0 ? 1 : 0;
+    if (work) {
+        PairRec q = pairs[k];
+        const Consts& C = IC[inst].C;
+        DevScalars* si = sc+inst;
+        double4 x0 = pos[q.p0], x1 = pos[q.p1];
+        double a[3] = {x0.x, x0.y, x0.z}, b[3] = {x1.x, x1.y, x1.z};
+        double sd[3] = {si->shear_disp[0], si->shear_disp[1], si->shear_disp[2]};
+        double dt = dt_override >= 0 ? dt_override : si->dt;
+        const double vd[3] = {si->flow.vel_diff[0], si->flow.vel_diff[1], si->flow.vel_diff[2]};
+        int drop = pair_update(q, a, b, x0.w, x1.w, vel6+6*(size_t)q.p0, vel6+6*(size_t)q.p1, C, vd, sd, dt,
+                               in_predictor);
+        pairs[k] = q;
+        keep[k] = drop ? 0 : 1;
+        if (C.friction && (q.flags & F_CONTACT)) slide = q.rsv_sq;
+    }
+    wave_max_into<&DevScalars::max_slide_bits>(sc, work ? inst : -1, slide);
 }
 // per-instance resets: bit 0 max sliding velocity, bit 1 max velocity
 __global__ void k_reset_inst(DevScalars* sc, int ninst, int what, int only)
@@ -1322,20 +1351,23 @@ __global__ void k_finish_velocity(int n, const double4* pos, const double* na6, 
 {
     // System::computeMaxNAVelocity + adjustVelocityPeriodicBoundary (System.cpp:1887-1902, 2369-2392)
     int i = blockIdx.x*blockDim.x+threadIdx.x;
-    if (i >= n) return;
-    const int inst = inst_of[i];
-    if (inst < 0) return;
-    DevScalars* sc = scs+inst;
-    double4 x4 = pos[i];
-    double x[3] = {x4.x, x4.y, x4.z}, u[3];
-    const Flow& F = sc->flow;
-    uinf(F, x, u);
-    const double* v = na6+6*(size_t)i;
-    if (in_predictor) atomic_max_nonneg(&sc->max_vel_bits, v[0]*v[0]+v[1]*v[1]+v[2]*v[2]);
-    for (int k = 0; k < 3; k++) {
-        vel6[6*(size_t)i+k] = v[k]+u[k];
-        vel6[6*(size_t)i+3+k] = v[3+k]+F.omega[k];
+    if (blockIdx.x*blockDim.x >= n) return;
+    const int inst = i < n ? inst_of[i] : -1;
+    double vmax = -1.0;
+    if (inst >= 0) {
+        DevScalars* sc = scs+inst;
+        double4 x4 = pos[i];
+        double x[3] = {x4.x, x4.y, x4.z}, u[3];
+        const Flow& F = sc->flow;
+        uinf(F, x, u);
+        const double* v = na6+6*(size_t)i;
+        vmax = v[0]*v[0]+v[1]*v[1]+v[2]*v[2];
+        for (int k = 0; k < 3; k++) {
+            vel6[6*(size_t)i+k] = v[k]+u[k];
+            vel6[6*(size_t)i+3+k] = v[3+k]+F.omega[k];
+        }
     }
+    if (in_predictor) wave_max_into<&DevScalars::max_vel_bits>(scs, inst, vmax);
 }
 void launch_finish_velocity(int n, const double4* pos, const double* na6, double* vel6, const int* inst_of,
                             DevScalars* sc, int ninst, int in_predictor, hipStream_t s)

@@ -1254,6 +1254,30 @@ extern "C" int lfd_system_get_solve_stats(lfd_system* y, lfd_solve_stats* st, lo
     return LFD_OK;
 }
 
+// Waves (8-particle groups) of an instance's aggregate preconditioner; 0 when
+// the instance is solved with the 6x6 block-Jacobi preconditioner.
+static int inst_precond_waves(lfd_batch* B, int id, int* waves)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h || !waves) return fail(LFD_ERR_ARG, "precond_waves: bad argument");
+    *waves = 0;
+    for (int k = 0; k < B->S.n_agg_jobs; k++) {
+        AggJob j;
+        HIPCHK(hipMemcpyAsync(&j, B->S.agg_jobs+k, sizeof(j), hipMemcpyDeviceToHost, B->s));
+        HIPCHK(hipStreamSynchronize(B->s));
+        if (j.inst != id) continue;
+        HIPCHK(hipMemcpyAsync(waves, B->S.agg.waves+id, sizeof(int), hipMemcpyDeviceToHost, B->s));
+        HIPCHK(hipStreamSynchronize(B->s));
+    }
+    return LFD_OK;
+}
+
+extern "C" int lfd_system_precond_waves(lfd_system* y, int* waves)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_precond_waves(y, 0, waves);
+}
+
 // ===================================================================== batch API
 extern "C" int lfd_batch_create(int slot_cap, int cell_cap, int inst_cap, int control, const lfd_solve_opts* o,
                                 lfd_batch** out)
@@ -1376,6 +1400,12 @@ extern "C" int lfd_batch_status(lfd_batch* B, int cap, int* ids, double* strain,
     }
     g_err = keep;
     return m;
+}
+
+extern "C" int lfd_batch_precond_waves(lfd_batch* B, int id, int* waves)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    return inst_precond_waves(B, id, waves);
 }
 
 extern "C" int lfd_batch_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* scalars8)
