@@ -17,9 +17,10 @@
 //                  a best-fit-decreasing rule evaluated on the size counts and
 //                  every particle gets its solver slot;
 //   k_agg_perm     slot -> particle map;
-//   k_agg_inverse  one wavefront per wave: FP64 Gauss-Jordan inverse of the
-//                  wave's 48 x 48 diagonal block (all couplings between its
-//                  particles), symmetrised, packed lower triangle in FP32;
+//   k_agg_inverse  one 256-thread workgroup per wave: FP64 Gauss-Jordan
+//                  inverse of the wave's 48 x 48 diagonal block (all
+//                  couplings between its particles), packed lower triangle
+//                  in FP32;
 //   k_solve_agg    FP64 iterative refinement around pipelined FP32 PCG with
 //                  the wave inverses as preconditioner; one wavefront per
 //                  wave, 12 waves per workgroup, blocks and inverses staged in
@@ -349,100 +350,122 @@ void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s)
 }
 
 // ------------------------------------------------------------------ wave inverses
-__device__ inline double readlane_d(double v, int lane)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
+static constexpr int AI_THREADS = 256;                        // four wavefronts per wave block
+static constexpr int AI_PER = AGG_DIM*AGG_DIM/AI_THREADS;     // 9 entries per thread
+static_assert(AGG_DIM*AGG_DIM == AI_PER*AI_THREADS, "48x48 entries spread evenly over the threads");
 
-// One wavefront per solver wave: the 48 x 48 diagonal block of the wave's
-// particles (every coupling between them) is assembled row by row in LDS
-// from the FP64 blocks, then inverted in FP64 by Gauss-Jordan elimination
-// without pivoting (the block is SPD), lane r holding row r in registers and
-// the pivot row broadcast with readlane.  Padding positions are the identity
-// during the elimination and zero in the result, which is written as the
-// packed lower triangle in FP32.  A block that is not positive definite in
-// the elimination falls back to the 6x6 block-Jacobi inverses.
-__global__ __launch_bounds__(WAVE) void k_agg_inverse(BsrView A, AggView G)
+// One workgroup of 256 threads per solver wave.  The 48 x 48 diagonal block
+// of the wave's particles (every coupling between them) is assembled in LDS
+// from the FP64 blocks: thread (p, e) places the e-th pair of member p, 48
+// threads the members' own blocks.  It is then inverted in FP64 by
+// Gauss-Jordan elimination without pivoting (the block is SPD): thread t
+// holds the 9 entries t, t + 256, ... in registers and each step reads the
+// pivot row and column from LDS (double-buffered, one barrier per step).
+// Padding positions are the identity during the elimination and zero in the
+// result, written as the packed lower triangle in FP32.  A block that is not
+// positive definite in the elimination falls back to the particles' 6x6
+// block-Jacobi inverses.
+__global__ __launch_bounds__(AI_THREADS) void k_agg_inverse(BsrView A, AggView G)
 {
     __shared__ double M[AGG_DIM*AGG_DIM];
+    __shared__ double rowb[2][AGG_DIM], colb[2][AGG_DIM];
     __shared__ int mem[AGG_CAP];
     const int w = blockIdx.x;
     const int inst = G.wave_inst[w];
     if (inst < 0 || G.wave_local[w] >= G.waves[inst]) return;
     const int t = threadIdx.x;
     if (t < AGG_CAP) mem[t] = G.perm[AGG_CAP*w+t];
-    for (int a = t; a < AGG_DIM*AGG_DIM; a += WAVE) M[a] = 0.0;
+    for (int a = t; a < AGG_DIM*AGG_DIM; a += AI_THREADS) M[a] = 0.0;
     __syncthreads();
+    // members' own blocks (padding: identity), column b of member p by thread 6p + b
     if (t < AGG_DIM) {
-        const int p = t/6, a = t%6, i = mem[p];
-        double* row = M+t*AGG_DIM;
+        const int p = t/6, b = t%6, i = mem[p];
         if (i >= 0) {
-            double ea[6] = {0, 0, 0, 0, 0, 0}, y[6] = {0, 0, 0, 0, 0, 0};
-            ea[a] = 1;
-            dmul<double, double>(A.diag64+15*(size_t)i, ea, y);        // row a of the symmetric D
-            for (int b = 0; b < 6; b++) row[6*p+b] = y[b];
-            // pair (j, i), j < i: B at (row i, col j), row a of B
-            for (int e = A.tptr[i]; e < A.tptr[i+1]; e++) {
-                const int k = A.tidx[e];
-                const int q = G.iperm[A.p0[k]]-AGG_CAP*w;
-                if (q < 0 || q >= AGG_CAP) continue;
-                double z[6] = {0, 0, 0, 0, 0, 0};
-                omul_t<double, double>(A.od64+18*(size_t)k, ea, z);
-                for (int b = 0; b < 6; b++) row[6*q+b] = z[b];
-            }
-            // pair (i, j), j > i: B at (row j, col i), row a of B^T = column a of B
-            for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) {
-                const int q = G.iperm[A.p1[k]]-AGG_CAP*w;
-                if (q < 0 || q >= AGG_CAP) continue;
-                double z[6] = {0, 0, 0, 0, 0, 0};
-                omul<double, double>(A.od64+18*(size_t)k, ea, z);
-                for (int b = 0; b < 6; b++) row[6*q+b] = z[b];
-            }
+            double eb[6] = {0, 0, 0, 0, 0, 0}, y[6] = {0, 0, 0, 0, 0, 0};
+            eb[b] = 1;
+            dmul<double, double>(A.diag64+15*(size_t)i, eb, y);
+            for (int a = 0; a < 6; a++) M[(6*p+a)*AGG_DIM+6*p+b] = y[a];
         } else {
-            row[t] = 1.0;
+            M[t*AGG_DIM+t] = 1.0;
+        }
+    }
+    // couplings inside the wave, as seen from member p: block (p, q)
+    {
+        const int p = t/32, e0 = t%32, i = mem[p];
+        if (i >= 0) {
+            const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i], nt = nl+(A.row_ptr[i+1]-u0);
+            for (int e = e0; e < nt; e += 32) {
+                const bool lower = e < nl;              // pair (j, i), j < i: B at (row i, col j)
+                const int k = lower ? A.tidx[l0+e] : u0+(e-nl);
+                const int q = G.iperm[lower ? A.p0[k] : A.p1[k]]-AGG_CAP*w;
+                if (q < 0 || q >= AGG_CAP) continue;
+                const double* B = A.od64+18*(size_t)k;
+                for (int b = 0; b < 6; b++) {
+                    double eb[6] = {0, 0, 0, 0, 0, 0}, y[6] = {0, 0, 0, 0, 0, 0};
+                    eb[b] = 1;
+                    if (lower) omul<double, double>(B, eb, y);        // column b of B
+                    else omul_t<double, double>(B, eb, y);            // column b of B^T
+                    for (int a = 0; a < 6; a++) M[(6*p+a)*AGG_DIM+6*q+b] = y[a];
+                }
+            }
         }
     }
     __syncthreads();
-    double R[AGG_DIM];
+    double R[AI_PER];
+    int rr[AI_PER], cc[AI_PER];
 #pragma unroll
-    for (int c = 0; c < AGG_DIM; c++) R[c] = t < AGG_DIM ? M[t*AGG_DIM+c] : 0.0;
+    for (int m = 0; m < AI_PER; m++) {
+        const int e = t+AI_THREADS*m;
+        rr[m] = e/AGG_DIM;
+        cc[m] = e%AGG_DIM;
+        R[m] = M[e];
+    }
+    // Step k, with p = a_kk:  a_kk <- 1/p,  a_kj <- a_kj/p,  a_ik <- -a_ik/p,
+    // a_ij <- a_ij - a_ik a_kj / p.  The owners of row k and column k publish
+    // them with a_kk replaced by 1 in the row and by -1 in the column and zero
+    // their own entries, so that every entry is  a_ij - col_i (row_j / p).
+    __shared__ double pivb[2];
     bool bad = false;
-#pragma unroll 1
     for (int k = 0; k < AGG_DIM; k++) {
-        // own element k (register index k is not a compile-time constant)
-        double fo = 0.0;
+        const int bb = k & 1;
 #pragma unroll
-        for (int j = 0; j < AGG_DIM; j++) fo = j == k ? R[j] : fo;
-        double piv = readlane_d(fo, k);
+        for (int m = 0; m < AI_PER; m++) {
+            const int r = rr[m], c = cc[m];
+            if (r == k) rowb[bb][c] = c == k ? 1.0 : R[m];
+            if (c == k) colb[bb][r] = r == k ? -1.0 : R[m];
+            if (r == k && c == k) pivb[bb] = R[m];
+            if (r == k || c == k) R[m] = 0.0;
+        }
+        __syncthreads();
+        double piv = pivb[bb];
         if (!(piv > 0) || !isfinite(piv)) { bad = true; piv = 1.0; }
         const double rp = 1.0/piv;
-        const bool me = t == k;
-        const double f = me ? -1.0 : fo;
-        const double keep = me ? 0.0 : 1.0;
 #pragma unroll
-        for (int j = 0; j < AGG_DIM; j++) {
-            const double rkj = readlane_d(R[j], k)*rp;
-            R[j] = j == k ? -f*rp : keep*R[j]-f*rkj;
-        }
+        for (int m = 0; m < AI_PER; m++) R[m] -= colb[bb][rr[m]]*(rowb[bb][cc[m]]*rp);
     }
-    if (t >= AGG_DIM) return;
-    float* out = G.inv+(size_t)w*AGG_PACK+t*(t+1)/2;
-    const int pr = t/6, i = mem[pr];
+    float* out = G.inv+(size_t)w*AGG_PACK;
     if (bad) {
-        const float* mi = A.minv32+36*(size_t)(i >= 0 ? i : 0);
-        for (int c = 0; c <= t; c++) out[c] = (i >= 0 && c/6 == pr) ? mi[6*(t%6)+c%6] : 0.f;
+        // 6x6 block-Jacobi inverses on the diagonal, zero elsewhere
+#pragma unroll
+        for (int m = 0; m < AI_PER; m++) {
+            const int r = rr[m], c = cc[m];
+            if (c > r) continue;
+            const int i = mem[r/6];
+            out[r*(r+1)/2+c] = (i >= 0 && c/6 == r/6) ? A.minv32[36*(size_t)i+6*(r%6)+c%6] : 0.f;
+        }
         return;
     }
 #pragma unroll
-    for (int c = 0; c < AGG_DIM; c++)
-        if (c <= t) out[c] = (i >= 0 && mem[c/6] >= 0) ? (float)R[c] : 0.f;
+    for (int m = 0; m < AI_PER; m++) {
+        const int r = rr[m], c = cc[m];
+        if (c > r) continue;
+        out[r*(r+1)/2+c] = (mem[r/6] >= 0 && mem[c/6] >= 0) ? (float)R[m] : 0.f;
+    }
 }
 
 void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStream_t s)
 {
-    if (nwaves > 0) k_agg_inverse<<<nwaves, WAVE, 0, s>>>(A, G);
+    if (nwaves > 0) k_agg_inverse<<<nwaves, AI_THREADS, 0, s>>>(A, G);
 }
 
 // ------------------------------------------------------------------ persistent solver
