@@ -1,0 +1,165 @@
+"""Configuration files and parameter files of LF_DEM (CPU).
+
+Fixtures under tests/golden/lfdem_check/ are LF_DEM's own regression inputs
+(check/confs/D3N200VF0.45Bidi1.4_0.5Cubic_1.{dat,bin}, check/repulsive.txt,
+check/repulsive_stress.txt), kept as data.  The strongest check: the
+reference's stress-controlled check (`LF_DEM -s 5r confs/...dat
+repulsive_stress.txt`, check/Makefile) read through these modules reproduces
+the first row of the reference's own output
+(check/reference/data_..._repulsive_stress_stress5r.dat) in the oracle.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from lf_dem_amd import formats, params
+from oracle import oracle as O
+
+CHECK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lfdem_check")
+STEM = os.path.join(CHECK, "D3N200VF0.45Bidi1.4_0.5Cubic_1")
+
+
+def test_text_configuration_matches_fixture(golden):
+    c = formats.read(STEM + ".dat")
+    assert c.format == formats.TXT_BASE_OLD and c.np == 200
+    assert np.array_equal(c.pos, golden["dat_pos"]) and np.array_equal(c.radius, golden["dat_radius"])
+    assert np.array_equal(c.L, golden["dat_L"]) and c.vf == 0.45
+    assert np.array_equal(c.shear_disp, [0, 0, 0])      # the header's `disp` field is not read (no dispx)
+
+
+def test_binary_configuration_matches_fixture(golden):
+    c = formats.read(STEM + ".bin")
+    assert c.format == formats.BIN_BASE_NEW and c.np == 200 and c.contacts == []
+    assert np.array_equal(c.pos, golden["bin_pos"]) and np.array_equal(c.radius, golden["bin_radius"])
+    assert np.array_equal(c.L, golden["bin_L"]) and np.array_equal(c.shear_disp, golden["bin_disp"])
+
+
+def test_binary_round_trip_with_contacts(tmp_path):
+    c = formats.read(STEM + ".bin")
+    c.contacts = [(3, 17, np.array([1e-3, -2e-4, 5e-5]), np.zeros(3)), (40, 41, np.array([0.0, 1e-6, 0]), np.ones(3))]
+    p = str(tmp_path / "c.bin")
+    formats.write_bin(p, c)
+    d = formats.read_bin(p)
+    assert np.array_equal(d.pos, c.pos) and np.array_equal(d.radius, c.radius)
+    assert np.array_equal(d.L, c.L) and np.array_equal(d.shear_disp, c.shear_disp) and d.vf == c.vf
+    assert len(d.contacts) == 2
+    for a, b in zip(c.contacts, d.contacts):
+        assert a[:2] == b[:2] and np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    assert open(p, "rb").read()[:12] == struct.pack("<iii", -1, 2, 200)
+
+
+def test_binary_v1_with_16bit_contact_indices(tmp_path):
+    """v1 files have no format header and may store p0, p1 as unsigned short
+    (told apart by p0 > np, Configuration.h:60-110)."""
+    n = 3
+    pos = np.array([[0, 0, 0, 1], [2.1, 0, 0, 1], [5, 5, 5, 1.4]], dtype="<f8")
+    body = struct.pack("<i", n) + struct.pack("<6d", 0.1, 10, 10, 10, 0.5, 0) + pos.tobytes()
+    body += struct.pack("<I", 1) + struct.pack("<HH6d", 0, 1, 1e-3, 0, 0, 0, 0, 0)
+    p = str(tmp_path / "v1.bin")
+    open(p, "wb").write(body)
+    c = formats.read_bin(p)
+    assert c.format == formats.BIN_BASE_OLD and c.np == 3
+    assert np.array_equal(c.shear_disp, [0.5, 0, 0])
+    assert c.contacts[0][:2] == (0, 1) and c.contacts[0][2][0] == 1e-3
+
+
+def test_text_round_trip(tmp_path):
+    c = formats.read(STEM + ".dat")
+    p = str(tmp_path / "c.dat")
+    formats.write_txt(p, c)
+    d = formats.read_txt(p)
+    assert np.allclose(d.pos, c.pos, rtol=1e-14, atol=0) and np.array_equal(d.radius, c.radius)
+    hdr = open(p).readline()
+    assert hdr == "# np1 np2 vf lx ly lz vf1 vf2 disp\n"
+
+
+def test_unsupported_layouts_are_rejected(tmp_path):
+    p = str(tmp_path / "fv.bin")
+    open(p, "wb").write(struct.pack("<iiii", -1, 3, 1, 0) + b"\0" * 100)
+    with pytest.raises(formats.FormatError):
+        formats.read_bin(p)
+    q = str(tmp_path / "cc.dat")
+    open(q, "w").write("# np lx ly lz vf radius_in\n# 1 1 1 1 0.1 2\n0 0 0 1\n")
+    with pytest.raises(formats.FormatError):
+        formats.read_txt(q)
+
+
+# ---------------------------------------------------------------- parameter files
+def test_stress_check_parameters_resolve_to_repulsive_units():
+    S = params.load(os.path.join(CHECK, "repulsive_stress.txt"), "stress", "5r", legacy_keys=True)
+    assert S.control == "stress" and S.internal_unit == "repulsion"
+    assert S.target_stress == pytest.approx(5 / (6 * np.pi), rel=1e-15)
+    p = S.params
+    assert (p["kn"], p["kt"], p["tau_n"], p["tau_t"], p["repulsion"]) == (5000, 2500, 2e-3, 0, 1)
+    assert (p["sd_coeff"], p["repulsive_length"], p["lub_reduce"], p["lub_max_gap"]) == (1e-8, 0.02, 1e-3, 0.5)
+    assert (p["mu_static"], p["mu_dynamic"], p["friction_model"], p["lubrication_model"]) == (1, 1, 1, 2)
+    assert S.strain_end == 0.01 and S.output_data == (1e-3, "strain")
+
+
+def test_rate_check_parameters_resolve_to_hydro_units():
+    S = params.load(os.path.join(CHECK, "repulsive.txt"), "rate", "0.2r", legacy_keys=True)
+    p = S.params
+    assert S.internal_unit == "hydro" and p["shear_rate"] == 1
+    assert p["repulsion"] == pytest.approx(5.0, rel=1e-15)        # F_R / F_H = 1 / 0.2
+    assert (p["kn"], p["kt"], p["tau_n"], p["tau_t"], p["sd_coeff"]) == (1e4, 5e3, 1e-3, 0, 1e-12)
+    assert S.fixed_dt and p["dt_min"] == p["dt_max"] == 5e-5
+
+
+def test_check_files_use_a_renamed_keyword():
+    """Both check files set `repulsiveforce_length`, which LF_DEM's current
+    parser rejects (SimulationInit.cpp:866-869); so does this one unless the
+    legacy name is allowed."""
+    with pytest.raises(params.ParameterError, match="repulsiveforce_length"):
+        params.load(os.path.join(CHECK, "repulsive.txt"), "rate", "0.2r")
+
+
+def _write(tmp_path, text):
+    p = str(tmp_path / "p.txt")
+    open(p, "w").write(text)
+    return p
+
+
+def test_unit_chains_and_times(tmp_path):
+    p = _write(tmp_path, "/* header */ kn = 5000r; kt = 0.5kn; contact_relaxation_time = 2e-3r;\n"
+                         "contact_relaxation_time_tan = 0r; time_end = 3h; /* trailing, never read */")
+    S = params.load(p, "rate", "0.2r")
+    assert S.params["kn"] == pytest.approx(25000, rel=1e-15)      # 5000 F_R, F_R = 5 F_H
+    assert S.params["kt"] == pytest.approx(12500, rel=1e-15)
+    assert S.params["tau_n"] == pytest.approx(4e-4, rel=1e-15)    # times scale as 1 / force ratio
+    assert S.strain_end == 3 and S.time_end is None
+    assert S.params["repulsive_length"] == 0.05                   # default
+
+
+def test_parser_errors(tmp_path):
+    with pytest.raises(params.ParameterError, match="suffix"):
+        params.load(_write(tmp_path, "kn = 5000;"), "rate", "1h")
+    with pytest.raises(params.ParameterError, match="//"):
+        params.load(_write(tmp_path, "kn = 5000h; // no\nkt = 1h;"), "rate", "1h")
+    with pytest.raises(params.ParameterError):
+        params.load(_write(tmp_path, "kn = 100b;"), "rate", "1h")       # no brownian scale to resolve
+    with pytest.raises(params.ParameterError, match="redefinition"):
+        params.load(_write(tmp_path, "repulsion = 2h;"), "rate", "0.2r")
+    with pytest.raises(params.UnsupportedParameter):
+        params.load(_write(tmp_path, "brownian = 1h;"), "rate", "1h")
+    with pytest.raises(params.UnsupportedParameter):
+        params.load(_write(tmp_path, "integration_method = 0;"), "rate", "1h")
+
+
+def test_reference_stress_check_reproduces_its_golden_row():
+    """`LF_DEM -s 5r confs/D3N200...dat repulsive_stress.txt` through the
+    readers: the oracle's first data row equals the reference's
+    (check/reference/data_..._stress5r.dat) to the printed digits."""
+    c = formats.read(STEM + ".dat")
+    S = params.load(os.path.join(CHECK, "repulsive_stress.txt"), "stress", "5r", legacy_keys=True)
+    R = O.System(c.pos, c.radius, c.L, shear_disp=c.shear_disp, params=O.default_params(**S.params))
+    R.set_control("stress", S.target_stress)
+    R.first_update()
+    assert R.npairs == 745
+    R.set_in_predictor(True)
+    R.compute_velocities_components()
+    R.calc_stress()
+    st = R.stress()
+    assert st["shear_rate"] == pytest.approx(1.08058, abs=5e-6)
+    assert 6 * np.pi * st["viscosity"] == pytest.approx(4.62714, abs=5e-6)
