@@ -71,6 +71,11 @@ def run(args):
     gen_s = time.time() - t0
     params = capi.default_params()
     opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
+    tuned = None
+    if args.autotune:
+        from lf_dem_amd import autotune
+        opts, tuned = autotune.tune_system(pos, rad, L, params=params, base=opts, cache_path=None)
+        args.precond = opts.precond
     S = capi.System(pos, rad, L, params=params, opts=opts)
     S.first_update()
     S.step(args.warmup)
@@ -119,7 +124,8 @@ def run(args):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "N=4000 bidisperse 1.0/1.4 phi=0.56 rate-controlled shear, mixed precision",
                    "N": args.n, "phi": args.phi, "seed": args.seed, "pairs": npairs, "tol": args.tol,
-                   "precond": ["block-jacobi", "aggregate"][args.precond],
+                   "precond": ["block-jacobi", "aggregate"][opts.precond], "regroup_interval": opts.regroup_interval,
+                   "inner_tol": opts.inner_tol,
                    "parallelism": "single GPU"},
         "plain_gpu_steps_per_s": round(plain_sps, 3) if plain_sps else None,
         "speedup_vs_plain_gpu": round(sps / plain_sps, 2) if plain_sps else None,
@@ -133,6 +139,8 @@ def run(args):
                      "algorithmic_bytes_per_launch": round(alg_bytes)},
         "confgen_s": round(gen_s, 2),
     }
+    if tuned:
+        out["autotune"] = tuned
     if not args.no_cpu:
         out["rel_err"], out["cpu_baseline"] = cpu_leg(pos, rad, L, params, opts, args)
     print(json.dumps(out), flush=True)
@@ -230,6 +238,10 @@ def run_c3(args):
     print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
     params = [capi.default_params(repulsion=s[3]) for s in specs]
+    tuned = None
+    if args.autotune:
+        from lf_dem_amd import autotune
+        opts, tuned = autotune.tune_batch(confs, params, base=opts, cache_path=None)
     slots = cells = 0
     for (pos, rad, L), p in zip(confs, params):
         sl, ce, _ = capi.footprint(len(rad), rad, L, p)
@@ -288,6 +300,9 @@ def run_c3(args):
         "device_bytes": info["bytes"],
         "confgen_s": round(gen_s, 1),
     }
+    out["config"]["precond"] = ["block-jacobi", "aggregate"][opts.precond]
+    if tuned:
+        out["autotune"] = tuned
     print(json.dumps(out), flush=True)
 
 
@@ -435,7 +450,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=131072, help="c4: particle slots of the batch")
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
     ap.add_argument("--precond", type=int, default=None,
-                    help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0)")
+                    help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0); the autotuner overrides it")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="c2/c3: skip the solver-option autotuner (lf_dem_amd.autotune) before the timed run")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "copy"), default="c2",
                     help="c2: N=4000 rate control (headline); c3: batched sweep of 64 instances; "
                          "c4: served requests; c5: N=32000 stress control; copy: 256 MiB device copy (counter calibration)")

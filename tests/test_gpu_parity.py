@@ -170,3 +170,29 @@ def test_multi_step_trajectory(golden, oracle, capi):
         d -= np.round(d / L) * L
         assert np.abs(d).max() <= 1e-8
         assert _pairset(S.pairs()) == _pairset(R.pairs())
+
+
+def test_reference_rate_check_inputs_on_gpu(oracle, capi):
+    """LF_DEM's rate-controlled check (`-r 0.2r confs/D3N200...bin
+    repulsive.txt`, check/Makefile) read through lf_dem_amd.formats and
+    lf_dem_amd.params: the GPU path follows the oracle over 5 steps (pair set
+    bit-exact, positions within 1e-8)."""
+    import os
+    from lf_dem_amd import formats, params
+    check = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lfdem_check")
+    c = formats.read(os.path.join(check, "D3N200VF0.45Bidi1.4_0.5Cubic_1.bin"))
+    P = params.load(os.path.join(check, "repulsive.txt"), "rate", "0.2r", legacy_keys=True)
+    S = capi.System(c.pos, c.radius, c.L, c.shear_disp, params=capi.default_params(**P.params))
+    R = oracle.System(c.pos, c.radius, c.L, c.shear_disp, params=oracle.default_params(**P.params))
+    S.first_update()
+    R.first_update()
+    for _ in range(5):
+        S.step(1)
+        R.step()
+    gp, rp = S.pairs(), R.pairs()
+    assert np.array_equal(gp["p0"], rp["p0"]) and np.array_equal(gp["p1"], rp["p1"])
+    g, o = S.state(), R.state()
+    assert g["dt"] == pytest.approx(5e-5, rel=1e-12) and o["dt"] == pytest.approx(5e-5, rel=1e-12)
+    d = g["pos"] - o["pos"]
+    d -= np.round(d / c.L) * c.L
+    assert np.abs(d).max() <= 1e-8
