@@ -35,14 +35,18 @@ static constexpr int AG_LEVELS = 3, AG_ROUNDS = 3;
 static constexpr int AGB_THREADS = 1024;
 static constexpr int AG_RMAX = 64;                  // run-table entries of the packing
 static constexpr int AG_HMASK = 16383;              // 14-bit local partner index in a matching key
+static constexpr int AG_K = AGG_ADJ_K;
+static constexpr int AG_CH = AGG_NMAX/AGB_THREADS;  // particles of one grouping thread at most
+static_assert(AG_K % 4 == 0, "adjacency rows are stored as uint4");
 
 // ------------------------------------------------------------------ coupling adjacency
-// Per particle, its incident pairs as (quantised weight << 14 | partner's local
-// index), lower incidence then upper, at row_ptr[i] + tptr[i] (both prefix
-// arrays are exclusive sums over the particle slots, so the segments tile
-// [0, 2P)).  The weight |B|_F / sqrt(|D_p0|_F |D_p1|_F) over the packed FP64
-// values is the size of the coupling relative to the two particles' own
-// resistances; the 18-bit quantisation only orders the candidates.
+// Per particle, its AG_K strongest couplings as (quantised weight << 14 |
+// partner's local index), in descending order of that key, zero-padded, at
+// adj[AG_K * slot].  The weight |B|_F / sqrt(|D_p0|_F |D_p1|_F) over the
+// packed FP64 values is the size of the coupling relative to the two
+// particles' own resistances; the 18-bit quantisation only orders the
+// candidates.  A group of at most 8 particles merges along its members'
+// strongest links, so the weaker ones are never candidates that matter.
 __device__ inline unsigned agg_weight_q(const BsrView& A, int k)
 {
     const double* o = A.od64+18*(size_t)k;
@@ -92,14 +96,27 @@ __global__ void k_agg_adjacency(BsrView A, const unsigned* __restrict__ wq, cons
         poff = IC[inst].poff;
     }
     if (!agg_due(scs+inst, interval)) return;
-    const int l0 = A.tptr[i], l1 = A.tptr[i+1], u0 = A.row_ptr[i], u1 = A.row_ptr[i+1];
-    unsigned* out = adj+(size_t)(u0+l0);
-    for (int e = l0; e < l1; e++) {
+    unsigned top[AG_K];
+#pragma unroll
+    for (int j = 0; j < AG_K; j++) top[j] = 0;
+    auto offer = [&](unsigned w, int partner) {          // insertion into the descending list
+        if (!w) return;
+        unsigned a = w << 14 | (unsigned)(partner-poff);
+#pragma unroll
+        for (int j = 0; j < AG_K; j++) {
+            const unsigned t = top[j];
+            top[j] = max(a, t);
+            a = min(a, t);
+        }
+    };
+    for (int e = A.tptr[i]; e < A.tptr[i+1]; e++) {
         const int k = A.tidx[e];
-        out[e-l0] = wq[k] << 14 | (unsigned)(A.p0[k]-poff);
+        offer(wq[k], A.p0[k]);
     }
-    out += l1-l0;
-    for (int k = u0; k < u1; k++) out[k-u0] = wq[k] << 14 | (unsigned)(A.p1[k]-poff);
+    for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) offer(wq[k], A.p1[k]);
+    uint4* out = reinterpret_cast<uint4*>(adj+(size_t)AG_K*i);
+#pragma unroll
+    for (int j = 0; j < AG_K/4; j++) out[j] = make_uint4(top[4*j], top[4*j+1], top[4*j+2], top[4*j+3]);
 }
 
 // Inclusive scan over the workgroup of two 64-bit words (four 16-bit counters each).
@@ -141,40 +158,54 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
     __shared__ unsigned run_comp[AG_RMAX];     // per run: count of each size class in one wave (nibbles)
     __shared__ int run_rep[AG_RMAX], run_bin0[AG_RMAX];
     __shared__ int s_nruns, s_bins, s_fallback;
-    __shared__ int sbase[AGG_NMAX+1];          // adjacency segment of each particle
     const AggJob J = jobs[blockIdx.x];
     if (!agg_due(scs+J.inst, interval)) return;
     const int n = J.n, poff = J.poff;
-    const int tid = threadIdx.x, T = blockDim.x;
-    const int ch = (n+T-1)/T, c0 = min(tid*ch, n), c1 = min(c0+ch, n);   // this thread's particles
+    const int tid = threadIdx.x, T = AGB_THREADS;
+    const int ch = (n+T-1)/T, c0 = min(tid*ch, n), c1 = min(c0+ch, n);   // contiguous chunk for the ranks
     for (int l = tid; l < n; l += T) { grp[l] = (short)l; gsz[l] = 1; best[l] = 0; relab[l] = -1; }
-    for (int l = tid; l <= n; l += T) sbase[l] = A.row_ptr[poff+l]+A.tptr[poff+l];
+    // the couplings of particles tid, tid + T, ... stay in registers
+    unsigned ent[AG_CH][AG_K];
+#pragma unroll
+    for (int k = 0; k < AG_CH; k++) {
+        const int l = tid+k*T;
+        const uint4* row = reinterpret_cast<const uint4*>(G.adj+(size_t)AG_K*(poff+l));
+#pragma unroll
+        for (int j = 0; j < AG_K/4; j++) {
+            const uint4 v = l < n ? row[j] : make_uint4(0, 0, 0, 0);
+            ent[k][4*j] = v.x; ent[k][4*j+1] = v.y; ent[k][4*j+2] = v.z; ent[k][4*j+3] = v.w;
+        }
+    }
     __syncthreads();
     for (int lev = 0; lev < AG_LEVELS; lev++) {
         const int cap = 2 << lev;
         for (int l = tid; l < n; l += T) done[l] = 0;
         __syncthreads();
         for (int rnd = 0; rnd < AG_ROUNDS; rnd++) {
-            // propose: every particle offers its group's strongest eligible
-            // neighbour group (key: weight, then the smaller group index)
-            for (int l = c0; l < c1; l++) {
+            // propose: every particle offers its group the strongest eligible
+            // neighbour group among its couplings (the first eligible one in
+            // key order); the group keeps the maximum of (weight, smaller
+            // group index)
+#pragma unroll
+            for (int k = 0; k < AG_CH; k++)
+#pragma unroll
+                for (int j = 0; j < AG_K; j++) asm volatile("" : "+v"(ent[k][j]));   // keep derived values out of registers
+#pragma unroll
+            for (int k = 0; k < AG_CH; k++) {
+                const int l = tid+k*T;
+                if (l >= n) continue;
                 const int g = grp[l];
                 if (done[g]) continue;
-                const int sg = gsz[g], e0 = sbase[l], e1 = sbase[l+1];
-                unsigned kb = 0;
-                for (int e = e0; e < e1; e += 16) {
-                    unsigned ent[16];
+                const int sg = gsz[g];
 #pragma unroll
-                    for (int q = 0; q < 16; q++) ent[q] = e+q < e1 ? G.adj[e+q] : 0u;
-#pragma unroll
-                    for (int q = 0; q < 16; q++) {
-                        if (!(ent[q] >> 14) || e+q >= e1) continue;
-                        const int h = grp[ent[q] & AG_HMASK];
-                        if (h == g || done[h] || sg+gsz[h] > cap) continue;
-                        kb = max(kb, (ent[q] >> 14) << 14 | (unsigned)(AG_HMASK-h));
-                    }
+                for (int j = 0; j < AG_K; j++) {
+                    const unsigned a = ent[k][j];
+                    if (!a) break;
+                    const int h = grp[a & AG_HMASK];
+                    if (h == g || done[h] || sg+gsz[h] > cap) continue;
+                    atomicMax(&best[g], (a >> 14) << 14 | (unsigned)(AG_HMASK-h));
+                    break;
                 }
-                if (kb) atomicMax(&best[g], kb);
             }
             __syncthreads();
             // accept mutual choices; the smaller leader absorbs the larger

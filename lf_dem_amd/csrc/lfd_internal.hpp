@@ -95,6 +95,7 @@ static constexpr int AGG_DIM = 6*AGG_CAP;          // rows of a wave block
 static constexpr int AGG_PACK = AGG_DIM*(AGG_DIM+1)/2;
 static constexpr int AGG_SLOTS_PER_BLOCK = 160;
 static constexpr int AGG_WAVES_PER_WG = 12;        // wavefronts (waves) per solver workgroup
+static constexpr int AGG_ADJ_K = 8;               // strongest couplings kept per particle for the grouping
 static constexpr int AGG_NMAX = 8192;              // largest instance the grouping kernel takes
 static constexpr int AGG_PART_PER_BLOCK = 12;      // partial-record doubles per 128-slot block
 
@@ -104,7 +105,7 @@ struct AggView {
     int* waves;            // [instances] waves in use
     const int* wave_inst;  // [solver waves] owning instance, -1 when free
     const int* wave_local; // [solver waves] wave index within the owning instance
-    unsigned* adj;         // [2 * pair capacity] per particle: incident pairs as (weight << 14 | partner)
+    unsigned* adj;         // [AGG_ADJ_K * slots] per particle: strongest couplings as (weight << 14 | partner)
     unsigned* wq;          // [pair capacity] quantised coupling weight of each pair
     int pcap;
     float* inv;            // [solver waves * AGG_PACK] wave block inverse, packed lower triangle (FP32)

@@ -185,7 +185,7 @@ struct Solver {
             scap = rb*AGG_SLOTS_PER_BLOCK;
             const int nwv = scap/AGG_CAP;
             A(dalloc(&agg.perm, scap)); A(dalloc(&agg.iperm, n)); A(dalloc(&agg.waves, std::max(icap, rb)+1));
-            A(dalloc(&wave_inst, nwv)); A(dalloc(&wave_local, nwv)); A(dalloc(&agg.adj, 2*(size_t)pcap));
+            A(dalloc(&wave_inst, nwv)); A(dalloc(&wave_local, nwv)); A(dalloc(&agg.adj, AGG_ADJ_K*(size_t)n));
             A(dalloc(&agg.wq, pcap));
             agg.pcap = pcap;
             A(dalloc(&agg.inv, (size_t)nwv*AGG_PACK));
