@@ -381,25 +381,25 @@ void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s)
 }
 
 // ------------------------------------------------------------------ wave inverses
-static constexpr int AI_THREADS = 256;                        // four wavefronts per wave block
-static constexpr int AI_PER = AGG_DIM*AGG_DIM/AI_THREADS;     // 9 entries per thread
-static_assert(AGG_DIM*AGG_DIM == AI_PER*AI_THREADS, "48x48 entries spread evenly over the threads");
+static constexpr int AI_THREADS = WAVE;                       // one wavefront per wave block
+static constexpr int AI_TILES = AGG_DIM/6;                    // 8 x 8 tiles of 6 x 6
+static_assert(AI_TILES*AI_TILES == AI_THREADS, "one 6 x 6 tile per lane");
 
-// One workgroup of 256 threads per solver wave.  The 48 x 48 diagonal block
-// of the wave's particles (every coupling between them) is assembled in LDS
-// from the FP64 blocks: thread (p, e) places the e-th pair of member p, 48
-// threads the members' own blocks.  It is then inverted in FP64 by
-// Gauss-Jordan elimination without pivoting (the block is SPD): thread t
-// holds the 9 entries t, t + 256, ... in registers and each step reads the
-// pivot row and column from LDS (double-buffered, one barrier per step).
-// Padding positions are the identity during the elimination and zero in the
-// result, written as the packed lower triangle in FP32.  A block that is not
-// positive definite in the elimination falls back to the particles' 6x6
-// block-Jacobi inverses.
+// One wavefront per solver wave.  The 48 x 48 diagonal block of the wave's
+// particles (every coupling between them) is assembled in LDS from the FP64
+// blocks: lanes 6p + b place column b of member p's own block, the 8 lanes of
+// member p its pairs.  It is then inverted in FP64 by Gauss-Jordan
+// elimination without pivoting (the block is SPD): lane t holds the 6 x 6
+// tile (t / 8, t % 8) -- the coupling of members t / 8 and t % 8 -- in
+// registers, and each step reads the pivot row and column from LDS.  Padding
+// positions are the identity during the elimination and zero in the result,
+// written as the packed lower triangle in FP32.  A block that is not positive
+// definite in the elimination falls back to the particles' 6x6 block-Jacobi
+// inverses.
 __global__ __launch_bounds__(AI_THREADS) void k_agg_inverse(BsrView A, AggView G)
 {
     __shared__ double M[AGG_DIM*AGG_DIM];
-    __shared__ double rowb[2][AGG_DIM], colb[2][AGG_DIM];
+    __shared__ double rowb[AGG_DIM], colb[AGG_DIM], pivb;
     __shared__ int mem[AGG_CAP];
     const int w = blockIdx.x;
     const int inst = G.wave_inst[w];
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(AI_THREADS) void k_agg_inverse(BsrView A, AggView G
     if (t < AGG_CAP) mem[t] = G.perm[AGG_CAP*w+t];
     for (int a = t; a < AGG_DIM*AGG_DIM; a += AI_THREADS) M[a] = 0.0;
     __syncthreads();
-    // members' own blocks (padding: identity), column b of member p by thread 6p + b
+    // members' own blocks (padding: identity), column b of member p by lane 6p + b
     if (t < AGG_DIM) {
         const int p = t/6, b = t%6, i = mem[p];
         if (i >= 0) {
@@ -422,10 +422,10 @@ __global__ __launch_bounds__(AI_THREADS) void k_agg_inverse(BsrView A, AggView G
     }
     // couplings inside the wave, as seen from member p: block (p, q)
     {
-        const int p = t/32, e0 = t%32, i = mem[p];
+        const int p = t/AI_TILES, e0 = t%AI_TILES, i = mem[p];
         if (i >= 0) {
             const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i], nt = nl+(A.row_ptr[i+1]-u0);
-            for (int e = e0; e < nt; e += 32) {
+            for (int e = e0; e < nt; e += AI_TILES) {
                 const bool lower = e < nl;              // pair (j, i), j < i: B at (row i, col j)
                 const int k = lower ? A.tidx[l0+e] : u0+(e-nl);
                 const int q = G.iperm[lower ? A.p0[k] : A.p1[k]]-AGG_CAP*w;
@@ -442,56 +442,74 @@ __global__ __launch_bounds__(AI_THREADS) void k_agg_inverse(BsrView A, AggView G
         }
     }
     __syncthreads();
-    double R[AI_PER];
-    int rr[AI_PER], cc[AI_PER];
+    const int ti = t/AI_TILES, tj = t%AI_TILES, r0 = 6*ti, c0 = 6*tj;
+    double R[6][6];
 #pragma unroll
-    for (int m = 0; m < AI_PER; m++) {
-        const int e = t+AI_THREADS*m;
-        rr[m] = e/AGG_DIM;
-        cc[m] = e%AGG_DIM;
-        R[m] = M[e];
-    }
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int b = 0; b < 6; b++) R[a][b] = M[(r0+a)*AGG_DIM+c0+b];
     // Step k, with p = a_kk:  a_kk <- 1/p,  a_kj <- a_kj/p,  a_ik <- -a_ik/p,
     // a_ij <- a_ij - a_ik a_kj / p.  The owners of row k and column k publish
     // them with a_kk replaced by 1 in the row and by -1 in the column and zero
     // their own entries, so that every entry is  a_ij - col_i (row_j / p).
-    __shared__ double pivb[2];
     bool bad = false;
-    for (int k = 0; k < AGG_DIM; k++) {
-        const int bb = k & 1;
+    for (int kt = 0; kt < AI_TILES; kt++) {
+        const bool rown = ti == kt, cown = tj == kt;
 #pragma unroll
-        for (int m = 0; m < AI_PER; m++) {
-            const int r = rr[m], c = cc[m];
-            if (r == k) rowb[bb][c] = c == k ? 1.0 : R[m];
-            if (c == k) colb[bb][r] = r == k ? -1.0 : R[m];
-            if (r == k && c == k) pivb[bb] = R[m];
-            if (r == k || c == k) R[m] = 0.0;
+        for (int kk = 0; kk < 6; kk++) {
+            if (rown) {
+#pragma unroll
+                for (int b = 0; b < 6; b++) {
+                    rowb[c0+b] = (cown && b == kk) ? 1.0 : R[kk][b];
+                    R[kk][b] = (cown && b == kk) ? R[kk][b] : 0.0;
+                }
+            }
+            if (cown) {
+#pragma unroll
+                for (int a = 0; a < 6; a++) colb[r0+a] = (rown && a == kk) ? -1.0 : R[a][kk];
+                if (rown) pivb = R[kk][kk];
+#pragma unroll
+                for (int a = 0; a < 6; a++) R[a][kk] = 0.0;
+            }
+            __syncthreads();
+            double piv = pivb;
+            if (!(piv > 0) || !isfinite(piv)) { bad = true; piv = 1.0; }
+            const double rp = 1.0/piv;
+            double cl[6], rw[6];
+#pragma unroll
+            for (int a = 0; a < 6; a++) cl[a] = colb[r0+a];
+#pragma unroll
+            for (int b = 0; b < 6; b++) rw[b] = rowb[c0+b]*rp;
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 6; a++)
+#pragma unroll
+                for (int b = 0; b < 6; b++) R[a][b] = fma(-cl[a], rw[b], R[a][b]);
         }
-        __syncthreads();
-        double piv = pivb[bb];
-        if (!(piv > 0) || !isfinite(piv)) { bad = true; piv = 1.0; }
-        const double rp = 1.0/piv;
-#pragma unroll
-        for (int m = 0; m < AI_PER; m++) R[m] -= colb[bb][rr[m]]*(rowb[bb][cc[m]]*rp);
     }
     float* out = G.inv+(size_t)w*AGG_PACK;
+    const bool live_r = mem[ti] >= 0, live_c = mem[tj] >= 0;
     if (bad) {
         // 6x6 block-Jacobi inverses on the diagonal, zero elsewhere
+        const int i = mem[ti];
 #pragma unroll
-        for (int m = 0; m < AI_PER; m++) {
-            const int r = rr[m], c = cc[m];
-            if (c > r) continue;
-            const int i = mem[r/6];
-            out[r*(r+1)/2+c] = (i >= 0 && c/6 == r/6) ? A.minv32[36*(size_t)i+6*(r%6)+c%6] : 0.f;
-        }
+        for (int a = 0; a < 6; a++)
+#pragma unroll
+            for (int b = 0; b < 6; b++) {
+                const int r = r0+a, c = c0+b;
+                if (c > r) continue;
+                out[r*(r+1)/2+c] = (i >= 0 && ti == tj) ? A.minv32[36*(size_t)i+6*a+b] : 0.f;
+            }
         return;
     }
 #pragma unroll
-    for (int m = 0; m < AI_PER; m++) {
-        const int r = rr[m], c = cc[m];
-        if (c > r) continue;
-        out[r*(r+1)/2+c] = (mem[r/6] >= 0 && mem[c/6] >= 0) ? (float)R[m] : 0.f;
-    }
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int b = 0; b < 6; b++) {
+            const int r = r0+a, c = c0+b;
+            if (c > r) continue;
+            out[r*(r+1)/2+c] = (live_r && live_c) ? (float)R[a][b] : 0.f;
+        }
 }
 
 void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStream_t s)
@@ -519,11 +537,14 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
     double* __restrict__ x, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
 {
     __shared__ int ent_j[AS_ENTRY_CAP];             // partner slot; ~slot for the transposed block
+    __shared__ int ent_k[AS_ENTRY_CAP];             // pair index (FP64 block)
     __shared__ float ent_b[AS_ENTRY_CAP*18];
     __shared__ float s_inv[AS_WAVES*AGG_PACK];
     __shared__ float s_vec[AS_WAVES*AGG_DIM];
     __shared__ AsRow rows[AS_ROWS];
     __shared__ double red[3*AS_WAVES+4];
+    __shared__ double red8[8*AS_WAVES+8];
+    __shared__ double s_tv[4];
     __shared__ double s_ei[9];
     __shared__ int s_fail;
     const SolveJob J = jobs[blockIdx.x];
@@ -575,6 +596,7 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
             else { k = u0+(e-nl); js = ~Gv.iperm[A.p1[k]]; }
             const int s = rows[lr].e0+e;
             ent_j[s] = js;
+            ent_k[s] = k;
             for (int m = 0; m < 18; m++) ent_b[18*s+m] = A.od32[18*(size_t)k+m];
         }
     }
@@ -700,6 +722,60 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
         gather3_finish(a, bs, cs);
     };
 
+    // Sums of eight values over every workgroup of the instance, for the
+    // refinement passes: own double-buffered records after the pipelined
+    // ones, own round tags (a record is rewritten every second call).
+    unsigned seq8 = 0;
+    int buf8 = 0;
+    auto gsum8 = [&](double* v) __attribute__((always_inline)) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const double t = wave_sum_d(v[m]);
+            if (lane == 0) red8[8*wid+m] = t;
+        }
+        drain_vmem();
+        __syncthreads();
+        seq8++;
+        if (wid == 0) {
+            // lane L works on value L % 8 throughout: its workgroup total, then
+            // the records q = L / 8, L / 8 + 8, ... of the instance
+            double* pb = part+6*(size_t)G+(size_t)buf8*8*G;
+            const unsigned tg = seq8 & 0xFF;
+            const int m = lane & 7;
+            double t = 0;
+            for (int q = 0; q < AS_WAVES; q++) t += red8[8*q+m];
+            t = tag_partial(t, tg);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane < 8) st_dev(pb+8*wg+m, t);
+            const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
+            bool ok = false;
+            for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
+                double acc = 0;
+                bool dn = true;
+                for (int e = lane; e < 8*G; e += WAVE) {
+                    const unsigned long long u = ld_dev(pq+e);
+                    dn &= (u & 0xFF) == tg;
+                    acc += untag_partial(u);
+                }
+                if (__all(dn)) {
+                    acc += __shfl_xor(acc, 8);
+                    acc += __shfl_xor(acc, 16);
+                    acc += __shfl_xor(acc, 32);
+                    if (lane < 8) red8[8*AS_WAVES+m] = acc;
+                    ok = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ok && lane == 0) s_fail = 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = red8[8*AS_WAVES+m];
+        buf8 ^= 1;
+    };
+
     // Galerkin correction onto the three uniform translations, as in the
     // block-Jacobi solver: (A z_a) rows in registers, E^-1 in LDS
     double AZ[3] = {0, 0, 0};
@@ -710,10 +786,10 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
     }
     bool use_coarse;
     {
-        double e00 = c == 0 ? AZ[0] : 0.0, e01 = c == 0 ? AZ[1] : 0.0, e02 = c == 0 ? AZ[2] : 0.0;
-        double e11 = c == 1 ? AZ[1] : 0.0, e12 = c == 1 ? AZ[2] : 0.0, e22 = c == 2 ? AZ[2] : 0.0;
-        gsum3(e00, e11, e22);
-        gsum3(e01, e02, e12);
+        double ev[8] = {c == 0 ? AZ[0] : 0.0, c == 1 ? AZ[1] : 0.0, c == 2 ? AZ[2] : 0.0,
+                        c == 0 ? AZ[1] : 0.0, c == 0 ? AZ[2] : 0.0, c == 1 ? AZ[2] : 0.0, 0.0, 0.0};
+        gsum8(ev);
+        const double e00 = ev[0], e11 = ev[1], e22 = ev[2], e01 = ev[3], e02 = ev[4], e12 = ev[5];
         const double c00 = e11*e22-e12*e12, c01 = e02*e12-e01*e22, c02 = e01*e12-e02*e11;
         const double det = e00*c00+e01*c01+e02*c02;
         use_coarse = det > 0 && isfinite(det) && e00 > 0;
@@ -726,24 +802,96 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
         __syncthreads();
     }
 
-    double bb = 0, rr64 = 0;
-    int outer, inner_total = 0;
-    bool converged = false, use_classic = false;
-    for (outer = 0; outer <= o.max_outer; outer++) {
-        double yv[6] = {0, 0, 0, 0, 0, 0};
-        if (live) spmv_row<double, double, AS_LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, c, yv);
-        double r64 = 0, bi = 0;
-        if (own) {
-            for (int m = 0; m < 6; m++) if (m == c) r64 = yv[m];
-            bi = b[gi];
-            r64 = bi-r64;
+    // Refinement passes.  x stays at its initial value x0 until the end; the
+    // correction D = x - x0 is accumulated in FP64: D_0 = 0 and
+    // D_k = (D_{k-1} + t_{k-1}) + s_{k-1} d_{k-1}, with t the translation
+    // shift of the coarse correction and s d the scaled FP32 inner solution of
+    // pass k-1.  The residual of pass k is b - A x0 - A D_k.  Every
+    // workgroup forms D_k of its neighbours from the previous pass's D and the
+    // exchanged d (both in slot order), so the end of a pass needs no barrier:
+    // d is exchanged by the inner solver's own reductions.
+    // A64 D_k for this row (component c)
+    auto spmv64_d = [&](const double* Dp, bool first, const double* tv, double sv) __attribute__((always_inline)) -> double {
+        double y[6] = {0, 0, 0, 0, 0, 0};
+        auto dval = [&](int js, double* xj) __attribute__((always_inline)) {
+            for (int m = 0; m < 6; m++) {
+                double v = first ? 0.0 : Dp[6*(size_t)js+m];
+                if (m < 3) v += tv[m];
+                xj[m] = v+sv*(double)Gv.db[6*(size_t)js+m];
+            }
+        };
+        if (live) {
+            if (ne >= 0) {
+                for (int e = c; e < ne; e += AS_LPR) {
+                    const int jj = ent_j[e0+e];
+                    double xj[6];
+                    dval(jj >= 0 ? jj : ~jj, xj);
+                    if (jj >= 0) omul<double, double>(A.od64+18*(size_t)ent_k[e0+e], xj, y);
+                    else omul_t<double, double>(A.od64+18*(size_t)ent_k[e0+e], xj, y);
+                }
+            } else {
+                const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+                const int nt = nl+(A.row_ptr[i+1]-u0);
+                for (int e = c; e < nt; e += AS_LPR) {
+                    double xj[6];
+                    if (e < nl) {
+                        const int k = A.tidx[l0+e];
+                        dval(Gv.iperm[A.p0[k]], xj);
+                        omul<double, double>(A.od64+18*(size_t)k, xj, y);
+                    } else {
+                        const int k = u0+(e-nl);
+                        dval(Gv.iperm[A.p1[k]], xj);
+                        omul_t<double, double>(A.od64+18*(size_t)k, xj, y);
+                    }
+                }
+            }
+            if (c == 0) {
+                double xi[6];
+                dval(slot, xi);
+                dmul<double, double>(A.diag64+15*(size_t)i, xi, y);
+            }
         }
-        double s_rr = r64*r64, s_bb = bi*bi, zr0 = c == 0 ? r64 : 0.0;
-        gsum3(s_rr, s_bb, zr0);
+        for (int o2 = 1; o2 < AS_LPR; o2 <<= 1)
+            for (int m = 0; m < 6; m++) y[m] += __shfl_xor(y[m], o2, AS_LPR);
+        double mine = 0;
+        for (int m = 0; m < 6; m++) if (m == c) mine = y[m];
+        return mine;
+    };
+
+    double bb = 0, rr64 = 0, Dfin = 0;
+    int outer, inner_total = 0;
+    bool converged = false, use_classic = false, zero_rhs = false;
+    // per lane in global memory (slot order): b - A x0 in r0buf, D in dsum,
+    // the last inner solution in db; the last shift and scale in s_tv
+    for (outer = 0; outer <= o.max_outer; outer++) {
+        double r64 = 0, bi = 0, Dv = 0;
+        if (outer == 0) {
+            double yv[6] = {0, 0, 0, 0, 0, 0};
+            if (live) spmv_row<double, double, AS_LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, c, yv);
+            if (own) {
+                for (int m = 0; m < 6; m++) if (m == c) r64 = yv[m];
+                bi = b[gi];
+                r64 = bi-r64;
+                Gv.r0buf[gs] = r64;
+            }
+        } else {
+            const double tv[3] = {s_tv[0], s_tv[1], s_tv[2]}, sv = s_tv[3];
+            const double y = spmv64_d((outer & 1) ? Gv.dsum[0] : Gv.dsum[1], outer == 1, tv, sv);
+            if (own) {
+                const double Dold = outer == 1 ? 0.0 : ((outer & 1) ? Gv.dsum[0] : Gv.dsum[1])[gs];
+                Dv = (Dold+(c == 0 ? tv[0] : c == 1 ? tv[1] : c == 2 ? tv[2] : 0.0))+sv*(double)Gv.db[gs];
+                ((outer & 1) ? Gv.dsum[1] : Gv.dsum[0])[gs] = Dv;
+                r64 = Gv.r0buf[gs]-y;
+            }
+        }
+        double rv[8] = {r64*r64, bi*bi, c == 0 ? r64 : 0.0, c == 1 ? r64 : 0.0, c == 2 ? r64 : 0.0, 0.0, 0.0, 0.0};
+        if (!own) rv[2] = rv[3] = rv[4] = 0.0;
+        gsum8(rv);
         if (s_fail) break;
-        if (outer == 0) bb = s_bb;
+        const double s_rr = rv[0];
+        if (outer == 0) bb = rv[1];
         if (bb == 0) {
-            if (own) x[gi] = 0.0;
+            zero_rhs = true;
             rr64 = 0;
             converged = true;
             break;
@@ -751,19 +899,18 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
         if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
         rr64 = s_rr;
         converged = sqrt(rr64) <= o.tol*sqrt(bb);
-        if (converged || outer == o.max_outer) break;
-        if (use_coarse) {
-            double zr1 = c == 1 ? r64 : 0.0, zr2 = c == 2 ? r64 : 0.0, z3 = 0;
-            gsum3(zr1, zr2, z3);
-            double cv[3];
-            for (int a = 0; a < 3; a++) cv[a] = s_ei[3*a]*zr0+s_ei[3*a+1]*zr1+s_ei[3*a+2]*zr2;
-            if (own) {
-                if (c < 3) x[gi] += cv[c];
-                r64 -= AZ[0]*cv[0]+AZ[1]*cv[1]+AZ[2]*cv[2];
-            }
+        if (converged || outer == o.max_outer) {
+            Dfin = Dv;
+            break;
         }
-        const double scale = 1.0/sqrt(rr64);
+        double tv[3] = {0, 0, 0};
+        if (use_coarse) {
+            for (int a = 0; a < 3; a++) tv[a] = s_ei[3*a]*rv[2]+s_ei[3*a+1]*rv[3]+s_ei[3*a+2]*rv[4];
+            if (own) r64 -= AZ[0]*tv[0]+AZ[1]*tv[1]+AZ[2]*tv[2];
+        }
+        if (threadIdx.x == 0) { s_tv[0] = tv[0]; s_tv[1] = tv[1]; s_tv[2] = tv[2]; s_tv[3] = sqrt(rr64); }
         float d = 0.f;
+        const double scale = 1.0/sqrt(rr64);
         int it = 0;
         if (!use_classic) {
             float* mbuf[2] = {Gv.zb, Gv.qb};
@@ -774,12 +921,12 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
             gsum3(s_r2, z0, z1);
             const double stop2 = o.inner_tol*o.inner_tol*s_r2;
             float wv = spmv32(mbuf[1], u);
-            float zv = 0, qv = 0, sv = 0, pv = 0;
+            float zv = 0, qv = 0, sv2 = 0, pv = 0;
             double gamma_old = 1, alpha_old = 1;
             int par = 0;
             for (it = 0; it < o.max_inner; it++) {
                 float mv = precond(wv);
-                if (own) mbuf[par][gs] = mv;
+                if (own) { mbuf[par][gs] = mv; Gv.db[gs] = d; }
                 publish3((double)r*u, (double)wv*u, (double)r*r);
                 double g, dl, rho;
                 gather3_start(g, dl, rho);
@@ -800,15 +947,19 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
                 const float bf = (float)beta, af = (float)alpha;
                 zv = nv+bf*zv;
                 qv = mv+bf*qv;
-                sv = wv+bf*sv;
+                sv2 = wv+bf*sv2;
                 pv = u+bf*pv;
                 d += af*pv;
-                r -= af*sv;
+                r -= af*sv2;
                 u -= af*qv;
                 wv -= af*zv;
                 gamma_old = g;
                 alpha_old = alpha;
                 par ^= 1;
+            }
+            if (it == o.max_inner) {                  // d changed after its last exchange
+                if (own) Gv.db[gs] = d;
+                tree_barrier(bar, G, wg, gen, &s_fail);
             }
         } else {
             float r = own ? (float)(r64*scale) : 0.f;
@@ -841,12 +992,13 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
                 gsum3(s_pq2, z4, z5);
                 pq = s_pq2;
             }
+            if (own) Gv.db[gs] = d;
+            tree_barrier(bar, G, wg, gen, &s_fail);
         }
         inner_total += it;
-        if (own) x[gi] += (double)d*sqrt(rr64);
-        tree_barrier(bar, G, wg, gen, &s_fail);
         if (s_fail) break;
     }
+    if (own && !s_fail) x[gi] = zero_rhs ? 0.0 : x[gi]+Dfin;
     if (wg == 0 && threadIdx.x == 0) {
         sc->res_norm = sqrt(rr64);
         sc->rhs_norm = sqrt(bb);

@@ -97,7 +97,7 @@ static constexpr int AGG_SLOTS_PER_BLOCK = 160;
 static constexpr int AGG_WAVES_PER_WG = 12;        // wavefronts (waves) per solver workgroup
 static constexpr int AGG_ADJ_K = 8;               // strongest couplings kept per particle for the grouping
 static constexpr int AGG_NMAX = 8192;              // largest instance the grouping kernel takes
-static constexpr int AGG_PART_PER_BLOCK = 12;      // partial-record doubles per 128-slot block
+static constexpr int AGG_PART_PER_BLOCK = 64;      // partial-record doubles per 128-slot block
 
 struct AggView {
     int* perm;             // [solver slots] particle slot, -1 for padding
@@ -111,6 +111,9 @@ struct AggView {
     float* inv;            // [solver waves * AGG_PACK] wave block inverse, packed lower triangle (FP32)
     float* zb;             // [6 * solver slots] exchanged Krylov vectors
     float* qb;
+    float* db;             // [6 * solver slots] inner solution of the current refinement pass
+    double* dsum[2];       // [6 * solver slots] accumulated correction, alternating by pass
+    double* r0buf;         // [6 * solver slots] b - A x0
     double* part;          // partial records, AGG_PART_PER_BLOCK per 128-slot block
     size_t part_len;
 };

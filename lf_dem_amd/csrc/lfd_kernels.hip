@@ -150,25 +150,54 @@ __global__ void k_scan_down(const int* in, int* out, const int* count_dev, int c
     if (blockIdx.x == 0 && threadIdx.x == 0) out[count] = sums[gridDim.x];
 }
 // Whole scan in one workgroup for arrays up to SCAN1_MAX entries: tiles of
-// SCAN1_T*SCAN_I entries with a running carry.  out2 (optional) receives a
+// SCAN1_T*SCAN1_I entries with a running carry.  out2 (optional) receives a
 // second copy of out[0..count) (a cursor array); it may alias in.
-static constexpr int SCAN1_T = 1024, SCAN1_MAX = 1 << 17;
+static constexpr int SCAN1_T = 1024, SCAN1_I = 32, SCAN1_MAX = 1 << 17;
 __global__ __launch_bounds__(SCAN1_T) void k_scan_single(const int* in, int* out, int* out2, const int* count_dev,
                                                          int cap)
 {
     __shared__ int lds[SCAN1_T/WAVE+1];
     const int count = count_dev ? min(*count_dev, cap) : cap;
+    // 16-byte accesses when every array allows them (a thread's run of
+    // SCAN1_I entries is one 128-byte line)
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out) |
+                       reinterpret_cast<uintptr_t>(out2)) & 15) == 0;
     int carry = 0;
-    for (int base = 0; base < count; base += SCAN1_T*SCAN_I) {
-        const int b0 = base+threadIdx.x*SCAN_I;
-        int v[SCAN_I], acc = 0;
-        for (int k = 0; k < SCAN_I; k++) { int i = b0+k; v[k] = i < count ? in[i] : 0; acc += v[k]; }
+    for (int base = 0; base < count; base += SCAN1_T*SCAN1_I) {
+        const int b0 = base+threadIdx.x*SCAN1_I;
+        const bool full = vec && b0+SCAN1_I <= count;
+        int v[SCAN1_I], acc = 0;
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < SCAN1_I/4; k++) {
+                const int4 t = reinterpret_cast<const int4*>(in+b0)[k];
+                v[4*k] = t.x; v[4*k+1] = t.y; v[4*k+2] = t.z; v[4*k+3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < SCAN1_I; k++) v[k] = b0+k < count ? in[b0+k] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < SCAN1_I; k++) acc += v[k];
         int total;
         int e = block_exclusive_scan(acc, lds, total)+carry;
-        for (int k = 0; k < SCAN_I; k++) {
-            int i = b0+k;
-            if (i < count) { out[i] = e; if (out2) out2[i] = e; }
-            e += v[k];
+#pragma unroll
+        for (int k = 0; k < SCAN1_I; k++) {             // v becomes the exclusive prefix
+            const int t = v[k];
+            v[k] = e;
+            e += t;
+        }
+        if (full) {
+#pragma unroll
+            for (int k = 0; k < SCAN1_I/4; k++) {
+                const int4 t = make_int4(v[4*k], v[4*k+1], v[4*k+2], v[4*k+3]);
+                reinterpret_cast<int4*>(out+b0)[k] = t;
+                if (out2) reinterpret_cast<int4*>(out2+b0)[k] = t;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < SCAN1_I; k++)
+                if (b0+k < count) { out[b0+k] = v[k]; if (out2) out2[b0+k] = v[k]; }
         }
         carry += total;
     }

@@ -190,6 +190,9 @@ struct Solver {
             agg.pcap = pcap;
             A(dalloc(&agg.inv, (size_t)nwv*AGG_PACK));
             A(dalloc(&agg.zb, 6*(size_t)scap)); A(dalloc(&agg.qb, 6*(size_t)scap));
+            A(dalloc(&agg.db, 6*(size_t)scap));
+            A(dalloc(&agg.dsum[0], 6*(size_t)scap)); A(dalloc(&agg.dsum[1], 6*(size_t)scap));
+            A(dalloc(&agg.r0buf, 6*(size_t)scap));
             agg.part_len = (size_t)AGG_PART_PER_BLOCK*rb+64;
             A(dalloc(&agg.part, agg.part_len));
             A(dalloc(&agg_jobs, rb+1));
@@ -215,7 +218,7 @@ struct Solver {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
                         w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials,
                         w.barrier, jobs, agg.perm, agg.iperm, agg.waves, wave_inst, wave_local, agg.adj, agg.wq, agg.inv, agg.zb,
-                        agg.qb, agg.part, agg_jobs, ajobs};
+                        agg.qb, agg.db, agg.dsum[0], agg.dsum[1], agg.r0buf, agg.part, agg_jobs, ajobs};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (auto& e : prof) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         prof.clear();
