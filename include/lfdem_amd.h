@@ -57,6 +57,20 @@ typedef enum lfd_precond {
 	                                  coupled particles (instances of up to 8192 particles) */
 } lfd_precond;
 
+/* How the persistent solver kernels are launched.  They synchronise their
+ * workgroups with bounded spin waits, so every workgroup of a launch must be
+ * resident at once.  A cooperative launch guarantees it and costs about 20 us
+ * of idle device time per solve; a plain launch relies on the device having
+ * room, which holds when no other solve runs on the device at the same time
+ * (otherwise the solve ends with LFD_ERR_SYNC_TIMEOUT, never hangs).  The
+ * solver that streams the matrix from HBM (LFD_SOLVER_MIXED_GLOBAL) is always
+ * launched cooperatively. */
+typedef enum lfd_launch {
+	LFD_LAUNCH_AUTO = 0,           /* plain when the launch needs at most half of the device, else cooperative */
+	LFD_LAUNCH_COOPERATIVE = 1,
+	LFD_LAUNCH_PLAIN = 2
+} lfd_launch;
+
 typedef struct lfd_solve_opts {
 	double tol;             /* target ||b - R x|| / ||b|| in FP64 */
 	double inner_tol;       /* residual drop per FP32 inner solve */
@@ -67,6 +81,7 @@ typedef struct lfd_solve_opts {
 	int precond;            /* lfd_precond */
 	int regroup_interval;   /* LFD_PRECOND_AGGREGATE: steps between regroupings (the block inverses
 	                           are recomputed at every assembly) */
+	int launch;             /* lfd_launch */
 } lfd_solve_opts;
 
 typedef struct lfd_solve_stats {

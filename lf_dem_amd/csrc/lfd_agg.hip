@@ -1035,7 +1035,7 @@ hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double
     // no stale round tags in any instance's partial records
     launch_fill_double(G.part, 0.0, G.part_len, s);
     void* args[] = {&Av, &Gv, (void*)&b, (void*)&x, &wv, &sc, &ov, (void*)&jobs};
-    return hipLaunchCooperativeKernel((const void*)k_solve_agg, dim3(nwg), dim3(AS_BLOCK), args, 0, s);
+    return launch_persistent((const void*)k_solve_agg, nwg, AS_BLOCK, args, o, agg_solver_max_groups(), s);
 }
 
 }  // namespace lfd

@@ -163,7 +163,19 @@ struct SolveOpts {
     int lanes_per_row;     // 1, 2, 4 or 8 (SpMV lanes per block row)
     int block;             // workgroup size
     int groups;            // workgroups of the persistent kernel (0 = auto)
+    int launch;            // lfd_launch
 };
+
+// Launch of a persistent solver kernel of G workgroups when at most `cap`
+// fit on the device at once (cooperative, or plain when that is asked for or
+// the launch needs at most half of the device).
+inline hipError_t launch_persistent(const void* fn, int G, int block, void** args, const SolveOpts& o, int cap,
+                                    hipStream_t s)
+{
+    const bool plain = o.launch == 2 || (o.launch == 0 && 2*G <= cap);
+    if (plain) return hipLaunchKernel(fn, dim3(G), dim3(block), args, 0, s);
+    return hipLaunchCooperativeKernel(fn, dim3(G), dim3(block), args, 0, s);
+}
 
 // ---------------- launchers (lfd_kernels.hip) ----------------
 void launch_fill_int(int* p, int v, size_t n, hipStream_t s);

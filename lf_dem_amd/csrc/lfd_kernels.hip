@@ -849,6 +849,7 @@ __global__ __launch_bounds__(128) void k_assemble_pairs(const PairRec* __restric
 // then pairs (i, j)); then one lane per particle inverts the block and
 // combines the RHS from the workgroup's LDS copy.
 static constexpr int GD_LANES = 8, GD_BLOCK = 256, GD_PER = GD_BLOCK/GD_LANES, GD_VALS = 15+RHS_TERMS;
+static constexpr int GD_UNROLL = 4;
 
 __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __restrict__ pdb,
     const double* __restrict__ prhs, const double4* __restrict__ pos, int n, const int* __restrict__ inst_of,
@@ -875,24 +876,43 @@ __global__ __launch_bounds__(GD_BLOCK) void k_gather_diag_rhs(const double* __re
             if (q == 9 || q == 12 || q == 14) v = TW;
             acc[u] = v;
         }
+        // pairs in batches of GD_UNROLL: the loads of a batch are issued
+        // together, the sums keep the order of the pairs
         const int t0 = tptr[i], t1 = tptr[i+1];
-        for (int t = t0; t < t1; t++) {                      // pairs (j, i), j < i
-            const size_t k = tidx[t];
+        for (int t = t0; t < t1; t += GD_UNROLL) {           // pairs (j, i), j < i
+            double v[GD_UNROLL][5];
             #pragma unroll
-            for (int u = 0; u < 5; u++) {
-                const int q = c+GD_LANES*u;
-                if (q < 15) acc[u] += pdb[30*k+15+q];
-                else if (q < GD_VALS) acc[u] += prhs[42*k+RHS_TERMS+(q-15)];
+            for (int j = 0; j < GD_UNROLL; j++) {
+                const bool in = t+j < t1;
+                const size_t k = in ? tidx[t+j] : 0;
+                #pragma unroll
+                for (int u = 0; u < 5; u++) {
+                    const int q = c+GD_LANES*u;
+                    v[j][u] = !in ? 0.0 : q < 15 ? pdb[30*k+15+q] : q < GD_VALS ? prhs[42*k+RHS_TERMS+(q-15)] : 0.0;
+                }
             }
+            #pragma unroll
+            for (int j = 0; j < GD_UNROLL; j++)
+                #pragma unroll
+                for (int u = 0; u < 5; u++) if (t+j < t1) acc[u] += v[j][u];
         }
         const int k0 = row_ptr[i], k1 = row_ptr[i+1];
-        for (int k = k0; k < k1; k++) {                      // pairs (i, j), j > i
+        for (int kb = k0; kb < k1; kb += GD_UNROLL) {        // pairs (i, j), j > i
+            double v[GD_UNROLL][5];
             #pragma unroll
-            for (int u = 0; u < 5; u++) {
-                const int q = c+GD_LANES*u;
-                if (q < 15) acc[u] += pdb[30*(size_t)k+q];
-                else if (q < GD_VALS) acc[u] += prhs[42*(size_t)k+(q-15)];
+            for (int j = 0; j < GD_UNROLL; j++) {
+                const bool in = kb+j < k1;
+                const size_t k = in ? (size_t)(kb+j) : 0;
+                #pragma unroll
+                for (int u = 0; u < 5; u++) {
+                    const int q = c+GD_LANES*u;
+                    v[j][u] = !in ? 0.0 : q < 15 ? pdb[30*k+q] : q < GD_VALS ? prhs[42*k+(q-15)] : 0.0;
+                }
             }
+            #pragma unroll
+            for (int j = 0; j < GD_UNROLL; j++)
+                #pragma unroll
+                for (int u = 0; u < 5; u++) if (kb+j < k1) acc[u] += v[j][u];
         }
         #pragma unroll
         for (int u = 0; u < 5; u++) {
@@ -2141,7 +2161,7 @@ hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w
     // no stale round tags in any instance's partial records
     launch_fill_double(w.partials, 0.0, 6*(size_t)((A.n+LS_ROWS-1)/LS_ROWS), s);
     void* args[] = {&Av, (void*)&b, (void*)&x, &wv, &sc, &ov, (void*)&jobs};
-    return hipLaunchCooperativeKernel((const void*)k_solve_lds, dim3(nwg), dim3(LS_BLOCK), args, 0, s);
+    return launch_persistent((const void*)k_solve_lds, nwg, LS_BLOCK, args, o, lds_solver_max_groups(), s);
 }
 
 int persistent_groups(const SolveOpts& o)
@@ -2189,6 +2209,8 @@ hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork&
     case 8: fn = (const void*)k_solve_mixed<8>; break;
     default: fn = (const void*)k_solve_mixed<4>; break;
     }
+    // cooperative always: the kernel's cg grid barrier needs the workspace
+    // that only a cooperative launch provides
     return hipLaunchCooperativeKernel(fn, dim3(G), dim3(PS_BLOCK), args, 0, s);
 }
 

@@ -70,6 +70,7 @@ extern "C" void lfd_default_solve_opts(lfd_solve_opts* o)
     o->kind = LFD_SOLVER_MIXED;
     o->precond = LFD_PRECOND_AGGREGATE;
     o->regroup_interval = 4;
+    o->launch = LFD_LAUNCH_AUTO;
 }
 
 extern "C" int lfd_device_count(void)
@@ -95,7 +96,7 @@ static SolveOpts to_opts(const lfd_solve_opts* o)
 {
     SolveOpts so;
     so.tol = o->tol; so.inner_tol = o->inner_tol; so.max_outer = o->max_outer; so.max_inner = o->max_inner;
-    so.lanes_per_row = o->lanes_per_row; so.block = 256; so.groups = o->groups;
+    so.lanes_per_row = o->lanes_per_row; so.block = 256; so.groups = o->groups; so.launch = o->launch;
     return so;
 }
 

@@ -57,3 +57,25 @@ def test_oracle_not_imported_by_product():
             if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in src.lower(), f
+
+
+def _header_struct_fields(name):
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, flags=re.S).group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = decl.split(None, 1)[1]
+        fields += [n.strip().lstrip("*").split("[")[0] for n in names.split(",")]
+    return fields
+
+
+def test_solve_opts_layout_matches_header():
+    """ctypes mirror of lfd_solve_opts: same fields in the same order."""
+    from lf_dem_amd import capi
+    assert [n for n, _ in capi.SolveOpts._fields_] == _header_struct_fields("lfd_solve_opts")
+    o = capi.default_opts()
+    assert o.launch == 0 and o.precond == 1 and o.regroup_interval >= 1

@@ -120,6 +120,23 @@ def test_velocities_vs_direct(case, kind, precond, golden, oracle, capi):
     assert gs["max_velocity"] == pytest.approx(rs["max_velocity"], rel=1e-8)
 
 
+@pytest.mark.parametrize("kind,precond", [(0, 1), (0, 0)])
+def test_plain_and_cooperative_launch_agree(kind, precond, capi):
+    """The persistent solvers give bit-identical steps whether launched
+    cooperatively or plainly (the launch only decides residency, not the
+    arithmetic)."""
+    pos, rad, L, sd = _c1()
+    out = []
+    for launch in (1, 2):
+        print("solver kind", kind, "preconditioner", precond, "launch", launch, flush=True)
+        S = capi.System(pos, rad, L, sd, opts=capi.default_opts(kind=kind, precond=precond, launch=launch))
+        S.first_update()
+        S.step(3)
+        out.append(S.state())
+    for key in ("pos", "vel", "ang_vel", "na_vel"):
+        assert np.array_equal(out[0][key], out[1][key]), key
+
+
 def test_stokes_dropin_matches_direct(golden, oracle, capi):
     """Blocks handed over through the StokesSolver column protocol."""
     pos, rad, L, sd = _c1()
