@@ -540,7 +540,6 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
     __shared__ int ent_k[AS_ENTRY_CAP];             // pair index (FP64 block)
     __shared__ float ent_b[AS_ENTRY_CAP*18];
     __shared__ float s_inv[AS_WAVES*AGG_PACK];
-    __shared__ float s_vec[AS_WAVES*AGG_DIM];
     __shared__ AsRow rows[AS_ROWS];
     __shared__ double red[3*AS_WAVES+4];
     __shared__ double red8[8*AS_WAVES+8];
@@ -652,19 +651,22 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
         for (int m = 0; m < 6; m++) { yc += Dr[m]*vv[m]; if (m == c) mine = y[m]; }
         return own ? mine+yc : 0.f;
     };
-    // z = W^-1 r on the lane's wave (LDS accesses of one wavefront execute in order)
+    // z = W^-1 r on the lane's wave: component m of r is read from its lane
+    // (8 (m / 6) + m % 6) into a scalar register, so LDS serves only the
+    // inverse (one read per product)
     auto precond = [&](float rv) -> float {
-        float* sv = s_vec+wid*AGG_DIM;
-        if (c < 6) sv[kk] = own ? rv : 0.f;
-        __builtin_amdgcn_wave_barrier();
+        const int x = __float_as_int(own ? rv : 0.f);
         const float* Mw = s_inv+wid*AGG_PACK;
+        int kq = kk;
+        asm volatile("" : "+v"(kq));       // the addresses are recomputed per call, not held in registers
+        const int kb = kq*(kq+1)/2;
         float z = 0.f;
-#pragma unroll 8
+#pragma unroll
         for (int m = 0; m < AGG_DIM; m++) {
-            const int a = m <= kk ? kk*(kk+1)/2+m : m*(m+1)/2+kk;
-            z += Mw[a]*sv[m];
+            const float sm = __int_as_float(__builtin_amdgcn_readlane(x, 8*(m/6)+m%6));
+            const int a = m <= kq ? kb+m : m*(m+1)/2+kq;
+            z += Mw[a]*sm;
         }
-        __builtin_amdgcn_wave_barrier();
         return own ? z : 0.f;
     };
 
