@@ -554,7 +554,6 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
     DevScalars* sc = scs+J.inst;
     unsigned* bar = w.barrier+128*(size_t)J.key;
     unsigned seq = 0;
-    double g0 = 0, g1 = 0, g2 = 0;
     const int lr = threadIdx.x/AS_LPR, c = threadIdx.x%AS_LPR;
     const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
     const int slot = sbase+lr;
@@ -699,7 +698,12 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
                     dn &= (v0 & 0xFF) == tg && (v1 & 0xFF) == tg && (v2 & 0xFF) == tg;
                     a0 += untag_partial(v0); a1 += untag_partial(v1); a2 += untag_partial(v2);
                 }
-                if (__all(dn)) { g0 = a0; g1 = a1; g2 = a2; ok = true; break; }
+                if (__all(dn)) {
+                    a0 = wave_sum_d(a0); a1 = wave_sum_d(a1); a2 = wave_sum_d(a2);
+                    if (lane == 0) { red[3*AS_WAVES] = a0; red[3*AS_WAVES+1] = a1; red[3*AS_WAVES+2] = a2; }
+                    ok = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
             if (!ok && lane == 0) s_fail = 1;
@@ -707,17 +711,12 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
         }
         __syncthreads();
     };
-    auto gather3_start = [&](double& t0, double& t1, double& t2) { t0 = g0; t1 = g1; t2 = g2; };
-    auto gather3_finish = [&](double& t0, double& t1, double& t2) {
-        if (wid == 0) {
-            t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
-            if (lane == 0) { red[3*AS_WAVES] = t0; red[3*AS_WAVES+1] = t1; red[3*AS_WAVES+2] = t2; }
-        }
-        __syncthreads();
+    // the instance totals, in place after publish3 (the slots are rewritten
+    // only after the next call's first barrier)
+    auto gather3_start = [&](double& t0, double& t1, double& t2) {
         t0 = red[3*AS_WAVES]; t1 = red[3*AS_WAVES+1]; t2 = red[3*AS_WAVES+2];
-        __syncthreads();
-        buf ^= 1;
     };
+    auto gather3_finish = [&](double&, double&, double&) { buf ^= 1; };
     auto gsum3 = [&](double& a, double& bs, double& cs) {
         publish3(a, bs, cs);
         gather3_start(a, bs, cs);
