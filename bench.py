@@ -88,12 +88,23 @@ def run(args):
     solve_ms, launches = S.profile(False)
     st1 = S.solve_stats()
     npairs = S.npairs
-    sps = args.steps / elapsed
+    eager_sps = args.steps / elapsed
+    # the same steps replayed from captured step graphs (lfd_system_step_graph),
+    # the library's mode for a fixed configuration: one capture per pair-buffer
+    # parity before the timed region
+    graph_sps = None
+    if args.graph:
+        S.step(2, graph=True)
+        if S.graphs() > 0:
+            t0 = time.perf_counter()
+            S.step(args.steps, graph=True)
+            graph_sps = args.steps / (time.perf_counter() - t0)
+    sps = graph_sps if graph_sps else eager_sps
     # per-step latency (each step synchronised)
     lat = []
     for _ in range(args.latency_steps):
         t1 = time.perf_counter()
-        S.step(1)
+        S.step(1, graph=graph_sps is not None)
         lat.append((time.perf_counter() - t1) * 1e3)
     lat = np.array(lat) if lat else np.zeros(1)
     inner = st1["total_inner"] - st0["total_inner"]
@@ -127,6 +138,8 @@ def run(args):
                    "precond": ["block-jacobi", "aggregate"][opts.precond], "regroup_interval": opts.regroup_interval,
                    "inner_tol": opts.inner_tol,
                    "parallelism": "single GPU"},
+        "stepping": "captured step graphs" if graph_sps else "eager",
+        "eager_steps_per_s": round(eager_sps, 3),
         "plain_gpu_steps_per_s": round(plain_sps, 3) if plain_sps else None,
         "speedup_vs_plain_gpu": round(sps / plain_sps, 2) if plain_sps else None,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -451,6 +464,8 @@ def main():
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
     ap.add_argument("--precond", type=int, default=None,
                     help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0); the autotuner overrides it")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="c2: time eager steps only (no captured step graphs)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="c2/c3: skip the solver-option autotuner (lf_dem_amd.autotune) before the timed run")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "copy"), default="c2",

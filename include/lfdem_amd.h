@@ -149,6 +149,15 @@ int lfd_system_first_update(lfd_system *s);
 /* n predictor-corrector steps (System.cpp:1241-1256, 888-983); strain_end < 0
  * disables the end-of-interval dt clamp.  Enqueued; errors are checked at the end. */
 int lfd_system_step(lfd_system *s, int nsteps, double strain_end);
+/* As lfd_system_step, each step replayed from a captured HIP graph (captured
+ * on first use, once per parity of the pair buffers, and again when the
+ * solver jobs, options or control change).  Steps run eagerly instead when
+ * the solver form needs host round trips (LFD_SOLVER_PLAIN), a cooperative
+ * launch, or while profiling.  LF_DEM has no counterpart: this is the
+ * capture of the step loop of System.cpp:1241-1256. */
+int lfd_system_step_graph(lfd_system *s, int nsteps, double strain_end);
+/* Number of step graphs held (0 to 2, one per pair-buffer parity). */
+int lfd_system_graphs(const lfd_system *s);
 int lfd_system_sync(lfd_system *s);
 /* individual phases (tests) */
 int lfd_system_check_new_interactions(lfd_system *s);
@@ -241,6 +250,9 @@ int lfd_batch_add(lfd_batch *b, int np, const double *pos, const double *radius,
 int lfd_batch_remove(lfd_batch *b, int id);
 /* nsteps predictor-corrector steps of every live instance (enqueued). */
 int lfd_batch_step(lfd_batch *b, int nsteps);
+/* As lfd_batch_step through captured step graphs (see lfd_system_step_graph). */
+int lfd_batch_step_graph(lfd_batch *b, int nsteps);
+int lfd_batch_graphs(const lfd_batch *b);
 int lfd_batch_sync(lfd_batch *b);
 /* scalars8 as lfd_system_get_state; status: the instance's own lfd_status */
 int lfd_batch_query(lfd_batch *b, int id, double *scalars8, lfd_solve_stats *st, int *status);

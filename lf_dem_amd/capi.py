@@ -50,6 +50,8 @@ SIGNATURES = {
     "lfd_system_destroy": ([_vp], None),
     "lfd_system_first_update": ([_vp], ctypes.c_int),
     "lfd_system_step": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "lfd_system_step_graph": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "lfd_system_graphs": ([_vp], ctypes.c_int),
     "lfd_system_sync": ([_vp], ctypes.c_int),
     "lfd_system_check_new_interactions": ([_vp], ctypes.c_int),
     "lfd_system_update_interactions": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
@@ -83,6 +85,8 @@ SIGNATURES = {
                       ctypes.c_int),
     "lfd_batch_remove": ([_vp, ctypes.c_int], ctypes.c_int),
     "lfd_batch_step": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_batch_step_graph": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lfd_batch_graphs": ([_vp], ctypes.c_int),
     "lfd_batch_sync": ([_vp], ctypes.c_int),
     "lfd_batch_query": ([_vp, ctypes.c_int, _dp, _vp, _ip], ctypes.c_int),
     "lfd_batch_status": ([_vp, ctypes.c_int, _ip, _dp, _ip], ctypes.c_int),
@@ -276,8 +280,14 @@ class System:
     def first_update(self):
         _check(lib().lfd_system_first_update(self.h))
 
-    def step(self, nsteps=1, strain_end=-1.0):
-        _check(lib().lfd_system_step(self.h, int(nsteps), float(strain_end)))
+    def step(self, nsteps=1, strain_end=-1.0, graph=False):
+        """nsteps predictor-corrector steps; graph=True replays a captured step."""
+        fn = lib().lfd_system_step_graph if graph else lib().lfd_system_step
+        _check(fn(self.h, int(nsteps), float(strain_end)))
+
+    def graphs(self):
+        """Step graphs held (0-2)."""
+        return lib().lfd_system_graphs(self.h)
 
     def sync(self):
         _check(lib().lfd_system_sync(self.h))
@@ -459,8 +469,12 @@ class Batch:
         _check(lib().lfd_batch_remove(self.h, int(iid)))
         self.np.pop(iid, None)
 
-    def step(self, nsteps=1):
-        _check(lib().lfd_batch_step(self.h, int(nsteps)))
+    def step(self, nsteps=1, graph=False):
+        fn = lib().lfd_batch_step_graph if graph else lib().lfd_batch_step
+        _check(fn(self.h, int(nsteps)))
+
+    def graphs(self):
+        return lib().lfd_batch_graphs(self.h)
 
     def sync(self):
         _check(lib().lfd_batch_sync(self.h))

@@ -158,6 +158,7 @@ struct Solver {
     int ajobs_cap = 0;
     std::vector<std::pair<int, int>> agg_launches;
     int regroup_interval = 4; // steps between an instance's groupings (the inverses follow every assembly)
+    long long gen = 0;        // bumped whenever the solver's launch sequence changes
     hipError_t alloc(int n_, int pcap_, int icap = 0)
     {
         n = n_; pcap = pcap_;
@@ -243,6 +244,7 @@ struct InstSpan { int inst, row0, n; };
 // only run alone, on the global-memory form.
 static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s)
 {
+    S.gen++;
     S.launches.clear();
     S.agg_launches.clear();
     S.n_agg_jobs = 0;
@@ -436,6 +438,12 @@ struct lfd_batch {
     BatchScalars* bs = nullptr;
     Solver S;
     std::vector<InstHost> inst;
+    // captured single steps, one per parity of the pair buffers at the start
+    // of the step, with the key they were captured under and the parity flip
+    // a step makes
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    long long gkey[2] = {-1, -1};
+    int gflip[2] = {0, 0};
     ~lfd_batch();
 };
 
@@ -450,6 +458,7 @@ lfd_batch::~lfd_batch()
                     scan_tmp, probe, pdb, prhs, rhs_parts, vel_parts, pst, ps, st_partial, ca.nb, ca.nnb,
                     ca.nstatic, ca.ctype, ca.cell_inst, d_ic, sc, bs};
     for (void* p : ptrs) if (p) (void)hipFree(p);
+    for (auto& g : gexec) if (g) (void)hipGraphExecDestroy(g);
     S.release();
     if (s) (void)hipStreamDestroy(s);
 }
@@ -775,6 +784,77 @@ static int bat_step(lfd_batch* B, int calc_stress = 0, int adapt_first = 1)
     return LFD_OK;
 }
 
+// ------------------------------------------------------------------ captured steps
+// A step is a fixed sequence of kernel launches on the batch's stream with no
+// host round trip, so it can be captured once into a HIP graph and replayed.
+// The capture depends on the pair-buffer parity at the start of the step
+// (host-side ping-pong), the solver's job table and options (Solver::gen) and
+// the control; it is redone when any of them changes.  Not captured: the plain
+// FP64 solver (host-side scalars), profiling (events), and solver launches
+// that must be cooperative.
+static long long graph_key(const lfd_batch* B)
+{
+    return 2*B->S.gen+(B->control == LFD_STRESS_CONTROL ? 1 : 0);
+}
+
+static bool capturable(const lfd_batch* B)
+{
+    const Solver& S = B->S;
+    if (S.kind != LFD_SOLVER_MIXED || S.profiling || (S.launches.empty() && S.agg_launches.empty())) return false;
+    auto plain = [&](int nwg, int cap) { return S.opts.launch == LFD_LAUNCH_PLAIN ||
+                                                (S.opts.launch == LFD_LAUNCH_AUTO && 2*nwg <= cap); };
+    for (auto& L : S.agg_launches) if (!plain(L.second, agg_solver_max_groups())) return false;
+    for (auto& L : S.launches) if (!plain(L.second, lds_solver_max_groups())) return false;
+    return true;
+}
+
+static int bat_capture(lfd_batch* B, int parity)
+{
+    const int cur0 = B->cur;
+    B->cur = parity;
+    HIPCHK(hipStreamBeginCapture(B->s, hipStreamCaptureModeThreadLocal));
+    const int rc = bat_step(B);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->s, &g);
+    const int flip = B->cur^parity;
+    B->cur = cur0;
+    if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return rc ? rc : fail(LFD_ERR_HIP, "step capture failed");
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(LFD_ERR_HIP, "step graph instantiation failed");
+    if (B->gexec[parity]) (void)hipGraphExecDestroy(B->gexec[parity]);
+    B->gexec[parity] = x;
+    B->gkey[parity] = graph_key(B);
+    B->gflip[parity] = flip;
+    return LFD_OK;
+}
+
+static int bat_step_graph(lfd_batch* B, int nsteps)
+{
+    if (!capturable(B)) {
+        for (int k = 0; k < nsteps; k++) {
+            const int rc = bat_step(B);
+            if (rc) return rc;
+        }
+        return LFD_OK;
+    }
+    for (int k = 0; k < nsteps; k++) {
+        const int p = B->cur;
+        if (!B->gexec[p] || B->gkey[p] != graph_key(B)) {
+            const int rc = bat_capture(B, p);
+            if (rc) return rc;
+        }
+        HIPCHK(hipGraphLaunch(B->gexec[p], B->s));
+        B->cur ^= B->gflip[p];
+    }
+    return LFD_OK;
+}
+
 // Admission of one instance: slots, cells and a handle are taken first fit;
 // the particles are binned and the instance's pairs found (System::setupSystem,
 // System.cpp:468-611), touching no other instance's pairs.  first_update also
@@ -1076,6 +1156,20 @@ extern "C" int lfd_system_step(lfd_system* y, int nsteps, double strain_end)
     return batch_check(y, 0);
 }
 
+extern "C" int lfd_system_step_graph(lfd_system* y, int nsteps, double strain_end)
+{
+    if (!y || nsteps < 0) return fail(LFD_ERR_ARG, "lfd_system_step_graph: bad argument");
+    launch_set_strain_end(y->sc, y->ninst, strain_end, y->s);
+    const int rc = bat_step_graph(y, nsteps);
+    if (rc) return rc;
+    return batch_check(y, 0);
+}
+
+extern "C" int lfd_system_graphs(const lfd_system* y)
+{
+    return y ? (y->gexec[0] != nullptr)+(y->gexec[1] != nullptr) : 0;
+}
+
 extern "C" int lfd_system_set_control(lfd_system* y, int control, double target_stress)
 {
     if (!y || (control != LFD_RATE_CONTROL && control != LFD_STRESS_CONTROL))
@@ -1359,6 +1453,20 @@ extern "C" int lfd_batch_step(lfd_batch* B, int nsteps)
         if (rc) return rc;
     }
     return LFD_OK;
+}
+
+extern "C" int lfd_batch_step_graph(lfd_batch* B, int nsteps)
+{
+    if (!B || nsteps < 0) return fail(LFD_ERR_ARG, "lfd_batch_step_graph: bad argument");
+    bool any = false;
+    for (int k = 0; k < B->ninst; k++) any = any || B->inst[k].live;
+    if (!any) return nsteps ? fail(LFD_ERR_ARG, "lfd_batch_step_graph: empty batch") : LFD_OK;
+    return bat_step_graph(B, nsteps);
+}
+
+extern "C" int lfd_batch_graphs(const lfd_batch* B)
+{
+    return B ? (B->gexec[0] != nullptr)+(B->gexec[1] != nullptr) : 0;
 }
 
 extern "C" int lfd_batch_sync(lfd_batch* B)
