@@ -176,12 +176,20 @@ def run_c5(args):
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
     opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
+
+    def to_stress_control(S):
+        S.compute_stress(False)
+        target = S.stress()["total"][2]*2*0.5   # S:E_hat with E_hat = (0, 0, 1/2, 0, 0, 0)
+        S.set_control("stress", target)
+        return target
+    tuned = None
+    if args.autotune:
+        from lf_dem_amd import autotune
+        opts, tuned = autotune.tune_system(pos, rad, L, params=params, base=opts, cache_path=None, warmup=2,
+                                           steps=4, control="stress", prepare=to_stress_control)
     S = capi.System(pos, rad, L, params=params, opts=opts)
     S.first_update()
-    S.compute_stress(False)
-    st = S.stress()
-    target = st["total"][2]*2*0.5          # S:E_hat with E_hat = (0, 0, 1/2, 0, 0, 0)
-    S.set_control("stress", target)
+    target = to_stress_control(S)
     S.step(args.warmup)
     st0 = S.solve_stats()
     S.profile(True)
@@ -201,11 +209,15 @@ def run_c5(args):
         "config": {"workload": "N=32000 bidisperse 1.0/1.4 phi=0.58 stress-controlled shear", "N": n, "phi": phi,
                    "seed": args.seed, "pairs": S.npairs, "target_stress": target, "tol": args.tol,
                    "parallelism": "single GPU"},
+        "solver_form": {0: "mixed, LDS-staged", 1: "plain FP64", 2: "mixed, HBM-streamed"}[opts.kind],
+        "precond_in_use": "aggregate" if S.precond_waves() > 0 else "6x6 block-Jacobi",
         "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],
         "solver": {"solve_launches": launches, "avg_solve_ms": round(solve_ms/max(launches, 1), 4),
                    "inner_iters_per_solve": round((st1["total_inner"]-st0["total_inner"])/max(launches, 1), 1)},
         "confgen_s": round(gen_s, 1),
     }
+    if tuned:
+        out["autotune"] = tuned
     print(json.dumps(out), flush=True)
 
 
@@ -467,7 +479,7 @@ def main():
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="c2: time eager steps only (no captured step graphs)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="c2/c3: skip the solver-option autotuner (lf_dem_amd.autotune) before the timed run")
+                    help="c2/c3/c5: skip the solver-option autotuner (lf_dem_amd.autotune) before the timed run")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "copy"), default="c2",
                     help="c2: N=4000 rate control (headline); c3: batched sweep of 64 instances; "
                          "c4: served requests; c5: N=32000 stress control; copy: 256 MiB device copy (counter calibration)")

@@ -38,6 +38,11 @@ SYSTEM_CANDIDATES = [
     ("aggregate, regroup 2", dict(precond=1, regroup_interval=2)),
     ("aggregate, regroup 4, inner 3e-3", dict(precond=1, regroup_interval=4, inner_tol=3e-3)),
     ("block-jacobi", dict(precond=0)),
+    # blocks streamed from HBM every iteration (no LDS staging), with the
+    # SpMV lanes per 6x6 block row as the tuned tile parameter
+    ("hbm blocks, 2 lanes per row", dict(kind=2, precond=0, lanes_per_row=2)),
+    ("hbm blocks, 4 lanes per row", dict(kind=2, precond=0, lanes_per_row=4)),
+    ("hbm blocks, 8 lanes per row", dict(kind=2, precond=0, lanes_per_row=8)),
 ]
 # (label, SolveOpts overrides) tried for a batch
 BATCH_CANDIDATES = [
@@ -84,9 +89,11 @@ def _opts(base, over):
     return o
 
 
-def _time_system(pos, rad, L, params, opts, warmup, steps):
+def _time_system(pos, rad, L, params, opts, warmup, steps, prepare=None):
     S = capi.System(pos, rad, L, params=params, opts=opts)
     S.first_update()
+    if prepare is not None:
+        prepare(S)
     S.step(warmup)
     S.sync()
     t0 = time.perf_counter()
@@ -139,12 +146,15 @@ def _tune(sig, candidates, timer, base, cache_path, use_cache):
 
 
 def tune_system(pos, rad, L, params=None, base=None, warmup=3, steps=10, candidates=None,
-                cache_path=DEFAULT_CACHE, use_cache=True, control="rate"):
-    """Fastest SolveOpts for one instance of this configuration: (opts, report)."""
+                cache_path=DEFAULT_CACHE, use_cache=True, control="rate", prepare=None):
+    """Fastest SolveOpts for one instance of this configuration: (opts, report).
+    prepare(system), when given, runs after the first update of every
+    candidate's system (e.g. to switch it to stress control)."""
     params = params if params is not None else capi.default_params()
     sig = _signature("system", [len(rad)], [_phi(rad, L)], control)
     return _tune(sig, candidates or SYSTEM_CANDIDATES,
-                 lambda o: _time_system(pos, rad, L, params, o, warmup, steps), base, cache_path, use_cache)
+                 lambda o: _time_system(pos, rad, L, params, o, warmup, steps, prepare), base, cache_path,
+                 use_cache)
 
 
 def tune_batch(confs, params, base=None, warmup=2, steps=6, candidates=None, cache_path=DEFAULT_CACHE,
