@@ -8,23 +8,23 @@
 // phi = 0.56.  Inverting the diagonal block of each group of up to 8 strongly
 // coupled particles instead removes most of them (about 0.65x the iterations).
 //
-// Pipeline, per step (grouping) and per assembly (inverses):
-//   k_agg_weight   coupling weight of every pair from its FP64 blocks;
-//   k_agg_group    one workgroup per instance, everything in LDS: three levels
-//                  of parallel matching (each group merges with its mutually
-//                  strongest neighbour group while the size stays <= 2, 4, 8),
-//                  then the groups are packed into waves of <= 8 particles by
-//                  a best-fit-decreasing rule evaluated on the size counts and
-//                  every particle gets its solver slot;
-//   k_agg_perm     slot -> particle map;
-//   k_agg_inverse  one 256-thread workgroup per wave: FP64 Gauss-Jordan
-//                  inverse of the wave's 48 x 48 diagonal block (all
-//                  couplings between its particles), packed lower triangle
-//                  in FP32;
-//   k_solve_agg    FP64 iterative refinement around pipelined FP32 PCG with
-//                  the wave inverses as preconditioner; one wavefront per
-//                  wave, 12 waves per workgroup, blocks and inverses staged in
-//                  LDS once per solve.
+// Pipeline, per regrouping (every few steps of an instance) and per assembly:
+//   k_agg_weight     coupling weight of every pair from its FP64 blocks;
+//   k_agg_adjacency  each particle's 8 strongest couplings, sorted;
+//   k_agg_group      one workgroup per instance, everything in LDS: three
+//                    levels of parallel matching (each group merges with its
+//                    mutually strongest neighbour group while the size stays
+//                    <= 2, 4, 8), then the groups are packed into waves of
+//                    <= 8 particles by a best-fit-decreasing rule evaluated on
+//                    the size counts, and every particle gets its solver slot
+//                    (slot -> particle and particle -> slot maps);
+//   k_agg_inverse    one wavefront per wave: FP64 Gauss-Jordan inverse of the
+//                    wave's 48 x 48 diagonal block (all couplings between its
+//                    particles), packed lower triangle in FP32;
+//   k_solve_agg      FP64 iterative refinement around pipelined FP32 PCG with
+//                    the wave inverses as preconditioner; one wavefront per
+//                    wave, 12 waves per workgroup, blocks and inverses staged
+//                    in LDS once per solve.
 #include <hip/hip_runtime.h>
 #include "lfd_internal.hpp"
 #include "lfd_device.hpp"
