@@ -167,6 +167,19 @@ int lfd_system_compute_velocities(lfd_system *s, int in_predictor);
 /* state access */
 int lfd_system_num_pairs(lfd_system *s);
 int lfd_system_get_pairs(lfd_system *s, int cap, int *p0, int *p1, int *state, double *geom5, double *disp_tan);
+/* System::getContacts (System.cpp:306-318): p0, p1, disp_tan[3], disp_rolling[3]
+ * of every active contact (disp_rolling is zero: rolling friction is not
+ * modelled); returns the count (only the count when p0 is NULL). */
+int lfd_system_get_contacts(lfd_system *s, int cap, int *p0, int *p1, double *disp_tan, double *disp_rolling);
+/* System::setContacts (System.cpp:285-304), after lfd_system_first_update as
+ * setupConfiguration does after checkNewInteraction: the listed contacts'
+ * tangential displacements are restored on their interactions (each must
+ * exist, else LFD_ERR_ARG; nonzero disp_rolling gives LFD_ERR_UNSUPPORTED),
+ * then the pairs are updated with dt = 0. */
+int lfd_system_set_contacts(lfd_system *s, int n, const int *p0, const int *p1, const double *disp_tan,
+                            const double *disp_rolling);
+/* The clock of a checkpoint (States.h:13-17): time and cumulated strain. */
+int lfd_system_set_clock(lfd_system *s, double time, double strain);
 int lfd_system_get_state(lfd_system *s, double *pos, double *vel6, double *na6, double *scalars8);
 int lfd_system_get_cells(lfd_system *s, int *dims4, int *labels);
 int lfd_system_get_blocks(lfd_system *s, int cap, double *dblocks, double *odblocks, int *p0, int *p1);

@@ -59,6 +59,9 @@ SIGNATURES = {
     "lfd_system_compute_velocities": ([_vp, ctypes.c_int], ctypes.c_int),
     "lfd_system_num_pairs": ([_vp], ctypes.c_int),
     "lfd_system_get_pairs": ([_vp, ctypes.c_int, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
+    "lfd_system_get_contacts": ([_vp, ctypes.c_int, _ip, _ip, _dp, _dp], ctypes.c_int),
+    "lfd_system_set_contacts": ([_vp, ctypes.c_int, _ip, _ip, _dp, _dp], ctypes.c_int),
+    "lfd_system_set_clock": ([_vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "lfd_system_get_state": ([_vp, _dp, _dp, _dp, _dp], ctypes.c_int),
     "lfd_system_get_cells": ([_vp, _ip, _ip], ctypes.c_int),
     "lfd_system_get_blocks": ([_vp, ctypes.c_int, _dp, _dp, _ip, _ip], ctypes.c_int),
@@ -260,6 +263,7 @@ class System:
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         rad = np.ascontiguousarray(radius, dtype=np.float64)
         self.np = len(rad)
+        self.radius = rad
         self.L = np.ascontiguousarray(L, dtype=np.float64)
         sd = np.ascontiguousarray(shear_disp, dtype=np.float64)
         h = _vp()
@@ -319,6 +323,28 @@ class System:
         n = _check(lib().lfd_system_get_pairs(self.h, cap, _i(p0), _i(p1), _i(st), _d(geom), _d(dt)))
         return dict(p0=p0[:n], p1=p1[:n], state=st[:n], nvec=geom[:n, :3], r=geom[:n, 3],
                     gap=geom[:n, 4], disp_tan=dt[:n])
+
+    def contacts(self):
+        """Active contacts (System::getContacts): p0, p1, disp_tan (n, 3), disp_rolling (n, 3)."""
+        n = _check(lib().lfd_system_get_contacts(self.h, 0, None, None, None, None))
+        cap = max(n, 1)
+        p0 = np.zeros(cap, dtype=np.int32)
+        p1 = np.zeros(cap, dtype=np.int32)
+        dt = np.zeros((cap, 3))
+        dr = np.zeros((cap, 3))
+        n = _check(lib().lfd_system_get_contacts(self.h, cap, _i(p0), _i(p1), _d(dt), _d(dr)))
+        return dict(p0=p0[:n], p1=p1[:n], disp_tan=dt[:n], disp_rolling=dr[:n])
+
+    def set_contacts(self, p0, p1, disp_tan, disp_rolling=None):
+        """System::setContacts, after first_update."""
+        p0 = np.ascontiguousarray(p0, dtype=np.int32)
+        p1 = np.ascontiguousarray(p1, dtype=np.int32)
+        dt = np.ascontiguousarray(disp_tan, dtype=np.float64).reshape(-1, 3)
+        dr = None if disp_rolling is None else np.ascontiguousarray(disp_rolling, dtype=np.float64).reshape(-1, 3)
+        _check(lib().lfd_system_set_contacts(self.h, len(p0), _i(p0), _i(p1), _d(dt), None if dr is None else _d(dr)))
+
+    def set_clock(self, time, strain):
+        _check(lib().lfd_system_set_clock(self.h, float(time), float(strain)))
 
     def state(self):
         pos = np.zeros((self.np, 3))

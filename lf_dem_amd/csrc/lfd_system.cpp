@@ -1016,6 +1016,72 @@ static int inst_get_pairs(lfd_batch* B, int id, int cap, int* p0, int* p1, int* 
     return np_;
 }
 
+// System::getContacts (System.cpp:306-318): the state of every active
+// contact, in pair order.  With p0 null only the count is returned.
+static int inst_get_contacts(lfd_batch* B, int id, int cap, int* p0, int* p1, double* disp_tan,
+                             double* disp_rolling)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h) return LFD_ERR_ARG;
+    int rc = batch_check(B, id);
+    if (rc) return rc;
+    int k0 = 0, k1 = 0;
+    if ((rc = pair_range(B, *h, k0, k1))) return rc;
+    const int off = h->ic.poff;
+    std::vector<PairRec> v(k1-k0);
+    if (k1 > k0) HIPCHK(hipMemcpy(v.data(), B->pairs[B->cur]+k0, (k1-k0)*sizeof(PairRec), hipMemcpyDeviceToHost));
+    int m = 0;
+    for (const PairRec& q : v) {
+        if (!(q.flags & F_CONTACT)) continue;
+        if (p0) {
+            if (m >= cap) return fail(LFD_ERR_ARG, "get_contacts: cap too small");
+            p0[m] = q.p0-off;
+            p1[m] = q.p1-off;
+            for (int c = 0; c < 3; c++) {
+                if (disp_tan) disp_tan[3*m+c] = q.disp_tan[c];
+                if (disp_rolling) disp_rolling[3*m+c] = 0.0;     // rolling friction is not modelled
+            }
+        }
+        m++;
+    }
+    return m;
+}
+
+// System::setContacts (System.cpp:285-304): the tangential displacement of
+// each listed contact is set on its interaction, which must exist (pairs are
+// found from the positions first, as setupConfiguration does before it).  The
+// pairs are then updated with dt = 0, so that the contact forces follow the
+// restored displacements before the first velocity computation.
+static int inst_set_contacts(lfd_batch* B, int id, int n, const int* p0, const int* p1, const double* disp_tan,
+                             const double* disp_rolling)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h || n < 0 || (n > 0 && (!p0 || !p1 || !disp_tan))) return fail(LFD_ERR_ARG, "set_contacts: bad argument");
+    int rc = batch_check(B, id);
+    if (rc) return rc;
+    int k0 = 0, k1 = 0;
+    if ((rc = pair_range(B, *h, k0, k1))) return rc;
+    const int off = h->ic.poff;
+    std::vector<PairRec> v(k1-k0);
+    if (k1 > k0) HIPCHK(hipMemcpy(v.data(), B->pairs[B->cur]+k0, (k1-k0)*sizeof(PairRec), hipMemcpyDeviceToHost));
+    for (int c = 0; c < n; c++) {
+        if (disp_rolling)
+            for (int d = 0; d < 3; d++)
+                if (disp_rolling[3*c+d] != 0.0)
+                    return fail(LFD_ERR_UNSUPPORTED, "set_contacts: rolling displacements (rolling friction) are not modelled");
+        const int a = std::min(p0[c], p1[c])+off, b = std::max(p0[c], p1[c])+off;
+        auto it = std::lower_bound(v.begin(), v.end(), std::make_pair(a, b),
+                                   [](const PairRec& q, const std::pair<int, int>& k) {
+                                       return q.p0 < k.first || (q.p0 == k.first && q.p1 < k.second); });
+        if (it == v.end() || it->p0 != a || it->p1 != b)
+            return fail(LFD_ERR_ARG, "set_contacts: no interaction for a listed contact");
+        for (int d = 0; d < 3; d++) it->disp_tan[d] = disp_tan[3*c+d];
+    }
+    if (k1 > k0) HIPCHK(hipMemcpy(B->pairs[B->cur]+k0, v.data(), (k1-k0)*sizeof(PairRec), hipMemcpyHostToDevice));
+    bat_update(B, 1, 0.0, id);
+    return batch_check(B, id);
+}
+
 static void fill_scalars8(const DevScalars& h, double* sc8)
 {
     sc8[0] = h.shear_disp[0]; sc8[1] = h.shear_disp[1]; sc8[2] = h.shear_disp[2];
@@ -1081,7 +1147,8 @@ static int inst_get_stress(lfd_batch* B, int id, double* groups24, double* total
 }
 
 static int inst_set_scalars(lfd_batch* B, int id, const double* target_stress, const double* strain_limit,
-                            const double* rate_limit, const double* strain_end)
+                            const double* rate_limit, const double* strain_end, const double* time = nullptr,
+                            const double* strain = nullptr)
 {
     if (!live_inst(B, id)) return LFD_ERR_ARG;
     DevScalars h;
@@ -1091,6 +1158,8 @@ static int inst_set_scalars(lfd_batch* B, int id, const double* target_stress, c
     if (strain_limit) h.init_strain_limit = *strain_limit;
     if (rate_limit) h.init_rate_limit = *rate_limit;
     if (strain_end) h.strain_end = *strain_end;
+    if (time) h.time = *time;
+    if (strain) h.strain = *strain;
     HIPCHK(hipMemcpyAsync(B->sc+id, &h, sizeof(h), hipMemcpyHostToDevice, B->s));
     HIPCHK(hipStreamSynchronize(B->s));
     return LFD_OK;
@@ -1284,6 +1353,26 @@ extern "C" int lfd_system_get_pairs(lfd_system* y, int cap, int* p0, int* p1, in
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
     return inst_get_pairs(y, 0, cap, p0, p1, state, geom5, disp_tan);
+}
+
+extern "C" int lfd_system_get_contacts(lfd_system* y, int cap, int* p0, int* p1, double* disp_tan,
+                                       double* disp_rolling)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_get_contacts(y, 0, cap, p0, p1, disp_tan, disp_rolling);
+}
+
+extern "C" int lfd_system_set_contacts(lfd_system* y, int n, const int* p0, const int* p1, const double* disp_tan,
+                                       const double* disp_rolling)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_set_contacts(y, 0, n, p0, p1, disp_tan, disp_rolling);
+}
+
+extern "C" int lfd_system_set_clock(lfd_system* y, double time, double strain)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_set_scalars(y, 0, nullptr, nullptr, nullptr, nullptr, &time, &strain);
 }
 
 extern "C" int lfd_system_get_state(lfd_system* y, double* pos, double* vel6, double* na6, double* sc8)
