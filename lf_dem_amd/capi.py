@@ -164,6 +164,23 @@ def default_params(**kw):
     return p
 
 
+def material_functions(total, contact, shear_rate):
+    """Normal-stress viscosities and particle pressures of a stress tensor
+    (xx, xy, xz, yz, yy, zz) in internal units, as Simulation::outputData
+    forms them (Simulation.cpp:694-699): N1 = (s_xx - s_zz) / rate,
+    N2 = (s_zz - s_yy) / rate, pressure = -trace / 3 (also of the contact
+    group).  LF_DEM prints viscosities times 6 pi and stresses times 6 pi
+    times the internal / output force ratio (OutputData.h:118-127)."""
+    t, c = np.asarray(total, dtype=float), np.asarray(contact, dtype=float)
+    out = dict(particle_pressure=-(t[0]+t[4]+t[5])/3, particle_pressure_contact=-(c[0]+c[4]+c[5])/3)
+    if shear_rate != 0:
+        out["N1_viscosity"] = (t[0]-t[5])/shear_rate
+        out["N2_viscosity"] = (t[5]-t[4])/shear_rate
+    else:
+        out["N1_viscosity"] = out["N2_viscosity"] = float("nan")
+    return out
+
+
 def default_opts(**kw):
     o = SolveOpts()
     lib().lfd_default_solve_opts(ctypes.byref(o))
@@ -416,6 +433,7 @@ class System:
         _check(lib().lfd_system_get_stress(self.h, _d(g), _d(t), ctypes.byref(rate), ctypes.byref(visc),
                                            _d(ps) if particle else None))
         out = dict(groups=dict(zip(self.GROUPS, g)), total=t, shear_rate=rate.value, viscosity=visc.value)
+        out.update(material_functions(t, out["groups"]["contact"], rate.value))
         if particle:
             out["particle"] = ps
         return out

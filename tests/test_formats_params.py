@@ -163,3 +163,10 @@ def test_reference_stress_check_reproduces_its_golden_row():
     st = R.stress()
     assert st["shear_rate"] == pytest.approx(1.08058, abs=5e-6)
     assert 6 * np.pi * st["viscosity"] == pytest.approx(4.62714, abs=5e-6)
+    # N1, N2 viscosities and particle pressures of the same row (columns 11-14)
+    from lf_dem_amd.capi import material_functions
+    m = material_functions(st["total"], st["groups"]["contact"], st["shear_rate"])
+    assert 6 * np.pi * m["N1_viscosity"] == pytest.approx(-0.102812, abs=5e-7)
+    assert 6 * np.pi * m["N2_viscosity"] == pytest.approx(-0.0637103, abs=5e-8)
+    assert 6 * np.pi * m["particle_pressure"] == pytest.approx(0.136904, abs=5e-7)
+    assert m["particle_pressure_contact"] == 0

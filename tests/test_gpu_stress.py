@@ -61,6 +61,10 @@ def test_stress_control_golden_row_on_gpu(golden, oracle, capi):
     assert g["shear_rate"] == pytest.approx(o["shear_rate"], rel=REL)
     assert g["shear_rate"] == pytest.approx(1.08058, abs=5e-6)
     assert 6*np.pi*g["viscosity"] == pytest.approx(4.62714, abs=5e-6)
+    # columns 11-13 of the same golden row: N1, N2 viscosities, particle pressure
+    assert 6*np.pi*g["N1_viscosity"] == pytest.approx(-0.102812, abs=5e-7)
+    assert 6*np.pi*g["N2_viscosity"] == pytest.approx(-0.0637103, abs=5e-8)
+    assert 6*np.pi*g["particle_pressure"] == pytest.approx(0.136904, abs=5e-7)
     cmp_stress(g, o)
     cmp_vel(S, R)
 
