@@ -170,3 +170,10 @@ def test_reference_stress_check_reproduces_its_golden_row():
     assert 6 * np.pi * m["N2_viscosity"] == pytest.approx(-0.0637103, abs=5e-8)
     assert 6 * np.pi * m["particle_pressure"] == pytest.approx(0.136904, abs=5e-7)
     assert m["particle_pressure_contact"] == 0
+    # columns 15-16 (energy, min gap) and 20-22 (contacts, interactions)
+    from lf_dem_amd import observables as ob
+    pr, p = R.pairs(), S.params
+    assert ob.potential_energy(pr, c.radius, p["kn"], p["kt"], p["repulsion"], p["repulsive_length"]) == \
+        pytest.approx(0.0620998, abs=5e-8)
+    assert ob.min_gap(pr, c.radius, 2 + p["lub_max_gap"]) == pytest.approx(0.0575972, abs=5e-8)
+    assert ob.contact_numbers(pr, 200) == (0, 0) and len(pr["p0"]) == 745

@@ -65,6 +65,13 @@ def test_stress_control_golden_row_on_gpu(golden, oracle, capi):
     assert 6*np.pi*g["N1_viscosity"] == pytest.approx(-0.102812, abs=5e-7)
     assert 6*np.pi*g["N2_viscosity"] == pytest.approx(-0.0637103, abs=5e-8)
     assert 6*np.pi*g["particle_pressure"] == pytest.approx(0.136904, abs=5e-7)
+    # columns 15-16: potential energy and min gap of the GPU pair list
+    from lf_dem_amd import observables as ob
+    P = golden_params(capi)
+    pr = S.pairs()
+    assert ob.potential_energy(pr, args[1], P.kn, P.kt, P.repulsion, P.repulsive_length) == \
+        pytest.approx(0.0620998, abs=5e-8)
+    assert ob.min_gap(pr, args[1], 2 + P.lub_max_gap) == pytest.approx(0.0575972, abs=5e-8)
     cmp_stress(g, o)
     cmp_vel(S, R)
 
