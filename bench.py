@@ -175,7 +175,7 @@ def run_c5(args):
     gen_s = time.time() - t0
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
-    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond, max_rhs=args.max_rhs)
 
     def to_stress_control(S):
         S.compute_stress(False)
@@ -211,6 +211,7 @@ def run_c5(args):
                    "parallelism": "single GPU"},
         "solver_form": {0: "mixed, LDS-staged", 1: "plain FP64", 2: "mixed, HBM-streamed"}[opts.kind],
         "precond_in_use": "aggregate" if S.precond_waves() > 0 else "6x6 block-Jacobi",
+        "rhs_per_launch": 1 if opts.max_rhs == 1 else 2,
         "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],
         "solver": {"solve_launches": launches, "avg_solve_ms": round(solve_ms/max(launches, 1), 4),
                    "inner_iters_per_solve": round((st1["total_inner"]-st0["total_inner"])/max(launches, 1), 1)},
@@ -476,6 +477,8 @@ def main():
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
     ap.add_argument("--precond", type=int, default=None,
                     help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0); the autotuner overrides it")
+    ap.add_argument("--max-rhs", type=int, default=0,
+                    help="c5: right-hand sides per launch of the block-Jacobi solver (0: two, 1: one)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="c2: time eager steps only (no captured step graphs)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",

@@ -82,6 +82,10 @@ typedef struct lfd_solve_opts {
 	int regroup_interval;   /* LFD_PRECOND_AGGREGATE: steps between regroupings (the block inverses
 	                           are recomputed at every assembly) */
 	int launch;             /* lfd_launch */
+	int max_rhs;            /* right-hand sides one launch of the LDS-staged solver (block-Jacobi)
+	                           takes together when a step solves several with one matrix (the rate
+	                           independent and rate proportional parts of stress control, the four
+	                           parts of a stress evaluation): 0 = two, 1 = one */
 } lfd_solve_opts;
 
 typedef struct lfd_solve_stats {

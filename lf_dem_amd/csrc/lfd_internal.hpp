@@ -164,6 +164,7 @@ struct SolveOpts {
     int block;             // workgroup size
     int groups;            // workgroups of the persistent kernel (0 = auto)
     int launch;            // lfd_launch
+    int max_rhs;           // right-hand sides one LDS solver launch takes together (1 or 2)
 };
 
 // Launch of a persistent solver kernel of G workgroups when at most `cap`
@@ -235,13 +236,14 @@ void launch_copy_double(double* dst, const double* src, size_t n, hipStream_t s)
 int persistent_groups(const SolveOpts& o);
 hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s);
-// Batched LDS-resident solver: one cooperative launch of nwg workgroups
-// described by jobs[0..nwg) (all instances of the launch).  Partial records
-// and barrier words of an instance sit at 6 and 128 per row block from its
-// first row block (row0 / 128).
+// Batched LDS-resident solver: one persistent launch of nwg workgroups
+// described by jobs[0..nwg) (all instances of the launch) for nrhs (1 or 2)
+// right-hand sides at b + k*rs (solutions at x + k*rs).  Partial records and
+// barrier words of an instance sit at 6*nrhs and 128 per row block from its
+// first row block (row0 / 128); the FP32 vectors w.z, w.q hold nrhs vectors.
 static constexpr int LDS_ROWS = 128;
-hipError_t solve_lds(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
-                     const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
+hipError_t solve_lds(const BsrView& A, const double* b, double* x, int nrhs, size_t rs, SolverWork& w,
+                     DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
 int lds_solver_max_groups();
 // aggregate preconditioner (lfd_agg.hip)
 // Grouping of the instances in jobs that are due (never grouped, or grouped

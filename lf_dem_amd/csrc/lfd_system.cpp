@@ -71,6 +71,7 @@ extern "C" void lfd_default_solve_opts(lfd_solve_opts* o)
     o->precond = LFD_PRECOND_AGGREGATE;
     o->regroup_interval = 4;
     o->launch = LFD_LAUNCH_AUTO;
+    o->max_rhs = 0;
 }
 
 extern "C" int lfd_device_count(void)
@@ -97,6 +98,7 @@ static SolveOpts to_opts(const lfd_solve_opts* o)
     SolveOpts so;
     so.tol = o->tol; so.inner_tol = o->inner_tol; so.max_outer = o->max_outer; so.max_inner = o->max_inner;
     so.lanes_per_row = o->lanes_per_row; so.block = 256; so.groups = o->groups; so.launch = o->launch;
+    so.max_rhs = o->max_rhs == 1 ? 1 : 2;
     return so;
 }
 
@@ -172,11 +174,12 @@ struct Solver {
         A(dalloc(&p0, pcap)); A(dalloc(&p1, pcap)); A(dalloc(&tidx, pcap));
         A(dalloc(&row_ptr, n+1)); A(dalloc(&tptr, n+1));
         A(dalloc(&w.r64, m)); A(dalloc(&w.x64, m));
-        A(dalloc(&w.r, m)); A(dalloc(&w.z, m)); A(dalloc(&w.p, m)); A(dalloc(&w.q, m)); A(dalloc(&w.w, m));
+        // w.z and w.q hold two vectors for the LDS solver's paired right-hand sides
+        A(dalloc(&w.r, m)); A(dalloc(&w.z, 2*m)); A(dalloc(&w.p, m)); A(dalloc(&w.q, 2*m)); A(dalloc(&w.w, m));
         A(dalloc(&w.d, m));
         A(dalloc(&w.pr, m)); A(dalloc(&w.pz, m)); A(dalloc(&w.pp, m)); A(dalloc(&w.pq, m));
         w.max_groups = 2048;
-        A(dalloc(&w.partials, std::max<size_t>(8*(size_t)w.max_groups, 6*(size_t)rb+64)));
+        A(dalloc(&w.partials, std::max<size_t>(8*(size_t)w.max_groups, 12*(size_t)rb+64)));
         // tree-barrier words: 128 per row block for the batched solver, 16384 for the global form
         const size_t nbar = std::max<size_t>(16384, 128*(size_t)rb+256);
         A(dalloc(&w.barrier, nbar));
@@ -340,10 +343,21 @@ static int check_scalars(DevScalars* sc_dev, hipStream_t s, DevScalars* out = nu
 
 // One solve of every instance: the batched LDS launches, or for a lone
 // instance the plain form or the global-memory form.
+// nrhs right-hand sides at b + k*rs, solutions at x + k*rs; the LDS solver
+// takes them two per launch (unless opts.max_rhs is 1), the other forms one
+// at a time
 static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* x, DevScalars* sc,
-                     hipStream_t s)
+                     hipStream_t s, int nrhs = 1, size_t rs = 0)
 {
     BsrView A = S.view(npairs_dev);
+    const bool lds_only = S.kind == LFD_SOLVER_MIXED && S.agg_launches.empty() && !S.launches.empty();
+    if (nrhs > 1 && !(lds_only && S.opts.max_rhs > 1)) {
+        for (int k = 0; k < nrhs; k++) {
+            int rc = run_solve(S, npairs_dev, b+k*rs, x+k*rs, sc, s);
+            if (rc) return rc;
+        }
+        return LFD_OK;
+    }
     if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL ||
         (S.launches.empty() && S.agg_launches.empty())) {
         if (S.single < 0) return fail(LFD_ERR_UNSUPPORTED, "this solver form takes a single instance");
@@ -371,7 +385,11 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
     if (S.kind == LFD_SOLVER_MIXED && !(S.launches.empty() && S.agg_launches.empty())) {
         for (auto& L : S.agg_launches)
             HIPCHK(solve_agg(A, S.agg, b, x, S.w, sc, S.ajobs+L.first, L.second, S.opts, s));
-        for (auto& L : S.launches) HIPCHK(solve_lds(A, b, x, S.w, sc, S.jobs+L.first, L.second, S.opts, s));
+        for (auto& L : S.launches)
+            for (int k = 0; k < nrhs; k += 2) {
+                const int nk = std::min(2, nrhs-k);
+                HIPCHK(solve_lds(A, b+k*rs, x+k*rs, nk, rs, S.w, sc, S.jobs+L.first, L.second, S.opts, s));
+            }
     } else {
         HIPCHK(solve_mixed(A, b, x, S.w, sc, S.opts, s));
     }
@@ -748,10 +766,8 @@ static int bat_velocities(lfd_batch* B, int in_predictor, int calc_stress = 0)
     const size_t m6 = 6*(size_t)B->M;
     if (B->control == LFD_STRESS_CONTROL) launch_set_rate(B->sc, B->d_ic, B->ninst, 1.0, B->s);
     bat_assemble(B, nparts);
-    for (int p = 0; p < nparts; p++) {
-        int rc = run_solve(B->S, &B->bs->npairs, B->rhs_parts+p*m6, B->vel_parts+p*m6, B->sc, B->s);
-        if (rc) return rc;
-    }
+    int rc = run_solve(B->S, &B->bs->npairs, B->rhs_parts, B->vel_parts, B->sc, B->s, nparts, m6);
+    if (rc) return rc;
     if (B->control == LFD_STRESS_CONTROL) bat_stress(B, nparts, 0);     // new shear rate
     launch_sum_parts(B->vel_parts, nparts, B->na6, B->M, B->inst_of, B->sc, B->control == LFD_STRESS_CONTROL, B->s);
     launch_finish_velocity(B->M, B->pos, B->na6, B->vel6, B->inst_of, B->sc, B->ninst, in_predictor, B->s);
