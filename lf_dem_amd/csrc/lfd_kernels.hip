@@ -1794,16 +1794,373 @@ static constexpr int LS_ENTRY_CAP = 1536;         // staged blocks per workgroup
 
 struct LdsRow { int e0, ne; };      // staged entries of a row (ne < 0: not staged)
 
-// NR right-hand sides (1 or 2) share one launch: every grid sum carries 3 NR
-// values and every SpMV reads each staged block once for the NR vectors, so
-// the per-iteration synchronisation is paid once.  The right-hand side k
-// lives at b + k*rs and its solution at x + k*rs; a right-hand side whose
-// inner solve has stopped keeps its correction while the other one iterates,
-// and one that has converged is left alone by the later refinement passes.
+// One right-hand side (the unused size_t is the right-hand-side stride of the
+// paired form, so both take the same launch arguments).
+__global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const double* __restrict__ b,
+    double* __restrict__ x, size_t, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
+{
+    __shared__ int ent_j[LS_ENTRY_CAP];             // partner; ~partner for the transposed block
+    __shared__ float ent_b[LS_ENTRY_CAP*18];
+    __shared__ LdsRow rows[LS_ROWS];
+    __shared__ double red[3*LS_WAVES+4];
+    __shared__ int s_fail;
+    // this workgroup's instance: rows [row0 + wg*LS_ROWS, ...) below row_end;
+    // its partial records and barrier words follow from its first row block
+    const SolveJob J = jobs[blockIdx.x];
+    const int wg = J.lwg, G = J.G, n = J.row_end, rbase = J.row0+wg*LS_ROWS;
+    DevScalars* sc = scs+J.inst;
+    unsigned* bar = w.barrier+128*(size_t)J.key;
+    unsigned seq = 0;
+    double g0 = 0, g1 = 0, g2 = 0;        // wave 0: this lane's share of the polled partial sums
+    const int lr = threadIdx.x/LS_LPR, c = threadIdx.x%LS_LPR;
+    const unsigned lane = threadIdx.x & (WAVE-1), wid = threadIdx.x/WAVE;
+    const int i = rbase+lr;
+    const bool live = i < n;
+    const bool own = live && c < 6;
+    const size_t gi = 6*(size_t)i+c;
+    unsigned gen = ld_acquire(bar+32);
+    if (threadIdx.x == 0) s_fail = 0;
+    // ---- stage this workgroup's rows: degrees in parallel, serial prefix in LDS
+    if (threadIdx.x < LS_ROWS) {
+        int ii = rbase+threadIdx.x;
+        rows[threadIdx.x].ne = ii < n ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int used = 0;
+        for (int t = 0; t < LS_ROWS; t++) {
+            int ii = rbase+t, deg = rows[t].ne;
+            if (ii < n && used+deg <= LS_ENTRY_CAP) { rows[t].e0 = used; used += deg; }
+            else { rows[t].e0 = 0; rows[t].ne = ii < n ? -1 : 0; }
+        }
+    }
+    __syncthreads();
+    if (live && rows[lr].ne > 0) {
+        const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+        for (int e = c; e < rows[lr].ne; e += LS_LPR) {
+            int k, j;
+            if (e < nl) { k = A.tidx[l0+e]; j = A.p0[k]; }
+            else { k = u0+(e-nl); j = ~A.p1[k]; }
+            int s = rows[lr].e0+e;
+            ent_j[s] = j;
+            for (int m = 0; m < 18; m++) ent_b[18*s+m] = A.od32[18*(size_t)k+m];
+        }
+    }
+    // row c of D and of D^-1 in registers (lanes 0..5)
+    float Dr[6] = {0, 0, 0, 0, 0, 0}, Mr[6] = {0, 0, 0, 0, 0, 0};
+    if (own) {
+        for (int m = 0; m < 6; m++) {
+            float e6[6] = {0, 0, 0, 0, 0, 0};
+            e6[m] = 1.f;
+            float y[6] = {0, 0, 0, 0, 0, 0};
+            dmul<float, float>(A.diag32+15*(size_t)i, e6, y);   // column m of the symmetric D
+            Dr[m] = y[c];
+            Mr[m] = A.minv32[36*(size_t)i+6*c+m];
+        }
+    }
+    __syncthreads();
+    const int ne = live ? rows[lr].ne : 0, e0 = live ? rows[lr].e0 : 0;
+
+    // component c of A32 v for this row: neighbours' v from vbuf, own v from the group's lanes
+    auto spmv32 = [&](const float* vbuf, float vown) -> float {
+        float y[6] = {0, 0, 0, 0, 0, 0};
+        if (live) {
+            if (ne >= 0) {
+                for (int e = c; e < ne; e += LS_LPR) {
+                    int jj = ent_j[e0+e];
+                    const float* bl = ent_b+18*(e0+e);
+                    float xj[6];
+                    if (jj >= 0) {
+                        load6(vbuf+6*(size_t)jj, xj);
+                        omul<float, float>(bl, xj, y);
+                    } else {
+                        jj = ~jj;
+                        load6(vbuf+6*(size_t)jj, xj);
+                        omul_t<float, float>(bl, xj, y);
+                    }
+                }
+            } else {
+                const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
+                const int nt = nl+(A.row_ptr[i+1]-u0);
+                for (int e = c; e < nt; e += LS_LPR) {
+                    float xj[6];
+                    if (e < nl) {
+                        int k = A.tidx[l0+e], jj = A.p0[k];
+                        load6(vbuf+6*(size_t)jj, xj);
+                        omul<float, float>(A.od32+18*(size_t)k, xj, y);
+                    } else {
+                        int k = u0+(e-nl), jj = A.p1[k];
+                        load6(vbuf+6*(size_t)jj, xj);
+                        omul_t<float, float>(A.od32+18*(size_t)k, xj, y);
+                    }
+                }
+            }
+        }
+        for (int m = 0; m < 6; m++) y[m] = sum8f(y[m]);
+        float vv[6];
+        bcast8_6(vown, c, vv);
+        float yc = 0, mine = 0;
+        for (int m = 0; m < 6; m++) { yc += Dr[m]*vv[m]; if (m == c) mine = y[m]; }
+        return own ? mine+yc : 0.f;
+    };
+    auto precond = [&](float rv) -> float {
+        float rr[6];
+        bcast8_6(rv, c, rr);
+        float z = 0;
+        for (int m = 0; m < 6; m++) z += Mr[m]*rr[m];
+        return own ? z : 0.f;
+    };
+
+    double* part = w.partials+6*(size_t)J.key;
+    int buf = 0;
+    // Grid sums of three values.  Block sums (DPP tree, fixed order) are
+    // published as tagged partial records in the current buffer after every
+    // wave's vector stores have completed and a release fence; wave 0 then
+    // polls all records with device-scope loads until each carries this
+    // round's tag, keeps its lanes' shares in g0..g2, and issues the acquire
+    // fence that makes the other workgroups' vector stores visible.
+    auto publish3 = [&](double a, double bs, double cs) {
+        a = wave_sum_d(a); bs = wave_sum_d(bs); cs = wave_sum_d(cs);
+        if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bs; red[3*wid+2] = cs; }
+        drain_vmem();
+        __syncthreads();
+        double* pb = part+(size_t)buf*3*G;
+        seq++;
+        if (wid == 0) {
+            double t0 = lane < LS_WAVES ? red[3*lane] : 0.0;
+            double t1 = lane < LS_WAVES ? red[3*lane+1] : 0.0;
+            double t2 = lane < LS_WAVES ? red[3*lane+2] : 0.0;
+            const unsigned tg = seq & 0xFF;
+            t0 = tag_partial(wave_sum_d(t0), tg);
+            t1 = tag_partial(wave_sum_d(t1), tg);
+            t2 = tag_partial(wave_sum_d(t2), tg);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
+            const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
+            bool ok = false;
+            for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
+                double a0 = 0, a1 = 0, a2 = 0;
+                bool done = true;
+                for (int q = lane; q < G; q += WAVE) {
+                    const unsigned long long v0 = ld_dev(pq+3*q), v1 = ld_dev(pq+3*q+1), v2 = ld_dev(pq+3*q+2);
+                    done &= (v0 & 0xFF) == tg && (v1 & 0xFF) == tg && (v2 & 0xFF) == tg;
+                    a0 += untag_partial(v0); a1 += untag_partial(v1); a2 += untag_partial(v2);
+                }
+                if (__all(done)) { g0 = a0; g1 = a1; g2 = a2; ok = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (!ok && lane == 0) s_fail = 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+    };
+    auto gather3_start = [&](double& t0, double& t1, double& t2) { t0 = g0; t1 = g1; t2 = g2; };
+    auto gather3_finish = [&](double& t0, double& t1, double& t2) {
+        if (wid == 0) {
+            t0 = wave_sum_d(t0); t1 = wave_sum_d(t1); t2 = wave_sum_d(t2);
+            if (lane == 0) { red[3*LS_WAVES] = t0; red[3*LS_WAVES+1] = t1; red[3*LS_WAVES+2] = t2; }
+        }
+        __syncthreads();
+        t0 = red[3*LS_WAVES]; t1 = red[3*LS_WAVES+1]; t2 = red[3*LS_WAVES+2];
+        __syncthreads();
+        buf ^= 1;
+    };
+    auto gsum3 = [&](double& a, double& bs, double& cs) {
+        publish3(a, bs, cs);
+        gather3_start(a, bs, cs);
+        gather3_finish(a, bs, cs);
+    };
+
+    // Coarse space of the three uniform translations z_a (U_i = e_a, W_i = 0).
+    // Pair resistances vanish on a rigid translation, so only the Stokes drag
+    // holds these modes; with a small drag coefficient they are near-null and
+    // the FP32 inner solve cannot resolve them.  Each refinement pass first
+    // applies the Galerkin correction x += Z E^-1 Z^T r, E = Z^T A Z.
+    // s_az[a][thread]: (A z_a) at row i, component c; s_ei: E^-1
+    __shared__ double s_az[3][LS_BLOCK];
+    __shared__ double s_ei[9];
+    double AZ[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        double y[6] = {0, 0, 0, 0, 0, 0};
+        if (live) row_translation<LS_LPR>(A, i, c, a, y);
+        for (int m = 0; m < 6; m++) if (m == c && own) AZ[a] = y[m];
+        s_az[a][threadIdx.x] = AZ[a];
+    }
+    bool use_coarse;
+    {
+        double e00 = c == 0 ? AZ[0] : 0.0, e01 = c == 0 ? AZ[1] : 0.0, e02 = c == 0 ? AZ[2] : 0.0;
+        double e11 = c == 1 ? AZ[1] : 0.0, e12 = c == 1 ? AZ[2] : 0.0, e22 = c == 2 ? AZ[2] : 0.0;
+        gsum3(e00, e11, e22);
+        gsum3(e01, e02, e12);
+        const double c00 = e11*e22-e12*e12, c01 = e02*e12-e01*e22, c02 = e01*e12-e02*e11;
+        const double det = e00*c00+e01*c01+e02*c02;
+        use_coarse = det > 0 && isfinite(det) && e00 > 0;
+        const double id = use_coarse ? 1.0/det : 0.0;
+        if (threadIdx.x == 0) {
+            s_ei[0] = c00*id; s_ei[1] = c01*id; s_ei[2] = c02*id;
+            s_ei[4] = (e00*e22-e02*e02)*id; s_ei[5] = (e01*e02-e00*e12)*id; s_ei[8] = (e00*e11-e01*e01)*id;
+            s_ei[3] = s_ei[1]; s_ei[6] = s_ei[2]; s_ei[7] = s_ei[5];
+        }
+        __syncthreads();
+    }
+
+    double bb = 0, rr64 = 0;
+    int outer, inner_total = 0;
+    bool converged = false, use_classic = false;
+    for (outer = 0; outer <= o.max_outer; outer++) {
+        // FP64 residual from the FP64 blocks in HBM
+        double yv[6] = {0, 0, 0, 0, 0, 0};
+        if (live) spmv_row<double, double, LS_LPR>(A, A.od64, A.diag64+15*(size_t)i, x, i, c, yv);
+        double r64 = 0, bi = 0;
+        if (own) {
+            for (int m = 0; m < 6; m++) if (m == c) r64 = yv[m];
+            bi = b[gi];
+            r64 = bi-r64;
+        }
+        double s_rr = r64*r64, s_bb = bi*bi, zr0 = c == 0 ? r64 : 0.0;
+        gsum3(s_rr, s_bb, zr0);
+        if (s_fail) break;
+        if (outer == 0) bb = s_bb;
+        if (bb == 0) {                     // zero right-hand side: the solution is zero
+            if (own) x[gi] = 0.0;
+            rr64 = 0;
+            converged = true;
+            break;
+        }
+        if (outer > 0 && !(s_rr < 0.25*rr64)) use_classic = true;
+        rr64 = s_rr;
+        converged = sqrt(rr64) <= o.tol*sqrt(bb);
+        if (converged || outer == o.max_outer) break;
+        if (use_coarse) {
+            double zr1 = c == 1 ? r64 : 0.0, zr2 = c == 2 ? r64 : 0.0, z3 = 0;
+            gsum3(zr1, zr2, z3);
+            double cv[3];
+            for (int a = 0; a < 3; a++) cv[a] = s_ei[3*a]*zr0+s_ei[3*a+1]*zr1+s_ei[3*a+2]*zr2;
+            if (own) {
+                if (c < 3) x[gi] += cv[c];
+                r64 -= s_az[0][threadIdx.x]*cv[0]+s_az[1][threadIdx.x]*cv[1]+s_az[2][threadIdx.x]*cv[2];
+            }
+        }
+        const double scale = 1.0/sqrt(rr64);
+        float d = 0.f;
+        int it = 0;
+        if (!use_classic) {
+            float* mbuf[2] = {w.z, w.q};
+            float r = own ? (float)(r64*scale) : 0.f;
+            float u = precond(r);
+            if (own) mbuf[1][gi] = u;
+            double s_r2 = (double)r*r, z0 = 0, z1 = 0;
+            gsum3(s_r2, z0, z1);
+            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            float wv = spmv32(mbuf[1], u);
+            float zv = 0, qv = 0, sv = 0, pv = 0;
+            double gamma_old = 1, alpha_old = 1;
+            int par = 0;
+            for (it = 0; it < o.max_inner; it++) {
+                float mv = precond(wv);
+                if (own) mbuf[par][gi] = mv;
+                publish3((double)r*u, (double)wv*u, (double)r*r);
+                double g, dl, rho;
+                gather3_start(g, dl, rho);
+                float nv = spmv32(mbuf[par], mv);
+                gather3_finish(g, dl, rho);
+                if (s_fail || rho <= stop2) break;
+                double beta, alpha;
+                if (it > 0) {
+                    beta = g/gamma_old;
+                    double den = dl-beta*g/alpha_old;
+                    if (!(den > 0)) break;
+                    alpha = g/den;
+                } else {
+                    if (!(dl > 0)) break;
+                    beta = 0;
+                    alpha = g/dl;
+                }
+                const float bf = (float)beta, af = (float)alpha;
+                zv = nv+bf*zv;
+                qv = mv+bf*qv;
+                sv = wv+bf*sv;
+                pv = u+bf*pv;
+                d += af*pv;
+                r -= af*sv;
+                u -= af*qv;
+                wv -= af*zv;
+                gamma_old = g;
+                alpha_old = alpha;
+                par ^= 1;
+            }
+        } else {
+            // classic PCG, two barriers per iteration
+            float r = own ? (float)(r64*scale) : 0.f;
+            float z = precond(r), p = z;
+            if (own) w.z[gi] = z;
+            double s_rz = (double)r*z, s_r2 = (double)r*r, z0 = 0;
+            gsum3(s_rz, s_r2, z0);
+            double rz = s_rz;
+            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            float q = spmv32(w.z, z);
+            double s_pq = (double)p*q, z1 = 0, z2 = 0;
+            gsum3(s_pq, z1, z2);
+            double pq = s_pq;
+            for (it = 0; it < o.max_inner; it++) {
+                if (s_fail || !(pq > 0)) break;
+                const float alpha = (float)(rz/pq);
+                d += alpha*p;
+                r -= alpha*q;
+                z = precond(r);
+                if (own) w.z[gi] = z;
+                double s_rz2 = (double)r*z, s_rr2 = (double)r*r, z3 = 0;
+                gsum3(s_rz2, s_rr2, z3);
+                if (s_rr2 <= stop2) { it++; break; }
+                const float beta = (float)(s_rz2/rz);
+                rz = s_rz2;
+                float wz = spmv32(w.z, z);
+                p = z+beta*p;
+                q = wz+beta*q;
+                double s_pq2 = (double)p*q, z4 = 0, z5 = 0;
+                gsum3(s_pq2, z4, z5);
+                pq = s_pq2;
+            }
+        }
+        inner_total += it;
+        if (own) x[gi] += (double)d*sqrt(rr64);
+        tree_barrier(bar, G, wg, gen, &s_fail);
+        if (s_fail) break;
+    }
+    if (wg == 0 && threadIdx.x == 0) {
+        sc->res_norm = sqrt(rr64);
+        sc->rhs_norm = sqrt(bb);
+        sc->iters_inner = inner_total;
+        sc->iters_outer = outer;
+        sc->total_inner += inner_total;
+        if (!converged) atomicOr(&sc->error, E_NOT_CONVERGED);
+    }
+    if (threadIdx.x == 0 && s_fail) atomicOr(&sc->error, E_SYNC_TIMEOUT);
+}
+
+// A workgroup-uniform double held in scalar registers (several right-hand
+// sides need every vector register the loop can spare).
+__device__ inline double uni(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// k_solve_lds for NR >= 2 right-hand sides in one launch (one right-hand
+// side: k_solve_lds_one, whose register allocation measured faster for it):
+// every grid sum carries 3 NR values and every SpMV reads each staged block
+// once for the NR vectors, so the per-iteration synchronisation is paid
+// once.  The right-hand side k lives at b + k*rs and its solution at
+// x + k*rs; a right-hand side whose inner solve has stopped keeps its
+// correction while the other one iterates, and one that has converged is
+// left alone by the later refinement passes.
 template <int NR>
 __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double* __restrict__ b,
     double* __restrict__ x, size_t rs, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
 {
+    static_assert(NR >= 2, "one right-hand side: k_solve_lds_one");
     constexpr int NV = 3*NR;                         // values of one grid sum
     __shared__ int ent_j[LS_ENTRY_CAP];             // partner; ~partner for the transposed block
     __shared__ float ent_b[LS_ENTRY_CAP*18];
@@ -1928,8 +2285,8 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     // polls all records with device-scope loads until each carries this
     // round's tag, sums them (DPP tree) into red[NV*LS_WAVES..], and issues
     // the acquire fence that makes the other workgroups' vector stores
-    // visible.  The totals are read after the publishing barrier; they are
-    // next written after the following publish's first barrier.
+    // visible.  Every thread reads the totals after the publishing barrier;
+    // wave 0 next writes them after the following publish's first barrier.
     auto publish = [&](double* v) {
         for (int k = 0; k < NV; k++) v[k] = wave_sum_d(v[k]);
         if (lane == 0)
@@ -2000,18 +2357,12 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     }
     bool use_coarse;
     {
-        // E = Z^T A Z: (e00, e11, e22, e01, e02, e12) in one or two grid sums
-        double ev[6] = {c == 0 ? AZ[0] : 0.0, c == 1 ? AZ[1] : 0.0, c == 2 ? AZ[2] : 0.0,
-                        c == 0 ? AZ[1] : 0.0, c == 0 ? AZ[2] : 0.0, c == 1 ? AZ[2] : 0.0};
-        if constexpr (NV >= 6) {
-            double v[NV];
-            for (int k = 0; k < NV; k++) v[k] = k < 6 ? ev[k] : 0.0;
-            gsum(v);
-            for (int k = 0; k < 6; k++) ev[k] = v[k];
-        } else {
-            gsum(ev);
-            gsum(ev+3);
-        }
+        // E = Z^T A Z: (e00, e11, e22, e01, e02, e12) in one grid sum
+        double ev[NV];
+        for (int k = 0; k < NV; k++) ev[k] = 0.0;
+        ev[0] = c == 0 ? AZ[0] : 0.0; ev[1] = c == 1 ? AZ[1] : 0.0; ev[2] = c == 2 ? AZ[2] : 0.0;
+        ev[3] = c == 0 ? AZ[1] : 0.0; ev[4] = c == 0 ? AZ[2] : 0.0; ev[5] = c == 1 ? AZ[2] : 0.0;
+        gsum(ev);
         const double e00 = ev[0], e11 = ev[1], e22 = ev[2], e01 = ev[3], e02 = ev[4], e12 = ev[5];
         const double c00 = e11*e22-e12*e12, c01 = e02*e12-e01*e22, c02 = e01*e12-e02*e11;
         const double det = e00*c00+e01*c01+e02*c02;
@@ -2101,7 +2452,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             }
             gsum(v);
             for (int k = 0; k < NR; k++) {
-                stop2[k] = o.inner_tol*o.inner_tol*v[3*k];
+                stop2[k] = uni(o.inner_tol*o.inner_tol*v[3*k]);
                 zv[k] = qv[k] = sv[k] = pv[k] = 0;
                 gamma_old[k] = alpha_old[k] = 1;
             }
@@ -2120,7 +2471,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                 bool any = false;
                 for (int k = 0; k < NR; k++) {
                     if (!act[k]) continue;
-                    const double g = v[3*k], dl = v[3*k+1], rho = v[3*k+2];
+                    const double g = uni(v[3*k]), dl = uni(v[3*k+1]), rho = uni(v[3*k+2]);
                     if (rho <= stop2[k]) { act[k] = false; continue; }
                     double beta, alpha;
                     if (it > 0) {
@@ -2143,7 +2494,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                     u[k] -= af*qv[k];
                     wv[k] -= af*zv[k];
                     gamma_old[k] = g;
-                    alpha_old[k] = alpha;
+                    alpha_old[k] = uni(alpha);
                     itk[k]++;
                     any = true;
                 }
@@ -2261,7 +2612,7 @@ hipError_t solve_lds(const BsrView& A, const double* b, double* x, int nrhs, siz
     // no stale round tags in any instance's partial records (6 nrhs per row block)
     launch_fill_double(w.partials, 0.0, 6*(size_t)nrhs*((A.n+LS_ROWS-1)/LS_ROWS), s);
     void* args[] = {&Av, (void*)&b, (void*)&x, &rs, &wv, &sc, &ov, (void*)&jobs};
-    const void* fn = nrhs == 2 ? (const void*)k_solve_lds<2> : (const void*)k_solve_lds<1>;
+    const void* fn = nrhs == 2 ? (const void*)k_solve_lds<2> : (const void*)k_solve_lds_one;
     return launch_persistent(fn, nwg, LS_BLOCK, args, o, lds_solver_max_groups(), s);
 }
 
