@@ -111,3 +111,19 @@ def test_batch_paired_matches_single(capi):
         S.sync()
         for a, b in zip(ip, is_):
             _same(P.state(a), S.state(b))
+
+
+def test_large_instance_paired_matches_single(capi):
+    """An instance of 71 workgroups (N=9000): two stress-controlled steps,
+    paired against one launch per right-hand side; both reach the FP64
+    residual target."""
+    pos, rad, L = conf(9000, 0.5, 5)
+    P, S = _pair(capi, pos, rad, L)
+    for k in range(2):
+        print("N=9000 stress-controlled step", k, flush=True)
+        P.step(1)
+        S.step(1)
+        for s in (P, S):
+            st = s.solve_stats()
+            assert st["res_norm"] <= 1e-12 * st["rhs_norm"] * 1.0000001
+        _same(P.state(), S.state())
