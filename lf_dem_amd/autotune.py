@@ -44,6 +44,20 @@ SYSTEM_CANDIDATES = [
     ("hbm blocks, 4 lanes per row", dict(kind=2, precond=0, lanes_per_row=4)),
     ("hbm blocks, 8 lanes per row", dict(kind=2, precond=0, lanes_per_row=8)),
 ]
+# largest instance the aggregate preconditioner groups (LFD_PRECOND_AGGREGATE
+# in include/lfdem_amd.h); a larger one runs 6x6 block-Jacobi whatever is asked
+AGGREGATE_MAX_N = 8192
+
+
+def system_candidates(n):
+    """SYSTEM_CANDIDATES that differ for an instance of n particles: without
+    the aggregate ones above AGGREGATE_MAX_N, where they would all time the
+    block-Jacobi solver."""
+    if n <= AGGREGATE_MAX_N:
+        return list(SYSTEM_CANDIDATES)
+    return [c for c in SYSTEM_CANDIDATES if c[1].get("precond", 1) != 1]
+
+
 # (label, SolveOpts overrides) tried for a batch
 BATCH_CANDIDATES = [
     ("block-jacobi", dict(precond=0)),
@@ -152,7 +166,7 @@ def tune_system(pos, rad, L, params=None, base=None, warmup=3, steps=10, candida
     candidate's system (e.g. to switch it to stress control)."""
     params = params if params is not None else capi.default_params()
     sig = _signature("system", [len(rad)], [_phi(rad, L)], control)
-    return _tune(sig, candidates or SYSTEM_CANDIDATES,
+    return _tune(sig, candidates or system_candidates(len(rad)),
                  lambda o: _time_system(pos, rad, L, params, o, warmup, steps, prepare), base, cache_path,
                  use_cache)
 

@@ -46,3 +46,15 @@ def test_signature_groups_similar_workloads():
     b = autotune._signature("batch", [300, 400], [0.51, 0.51], "rate")
     assert a == b
     assert a != autotune._signature("system", [700], [0.51], "rate")
+
+
+def test_aggregate_candidates_dropped_above_the_grouping_limit():
+    """Above 8192 particles the aggregate preconditioner is unavailable, so
+    its candidates would time block-Jacobi under another name."""
+    from lf_dem_amd import autotune
+    small = autotune.system_candidates(4000)
+    large = autotune.system_candidates(32000)
+    assert small == autotune.SYSTEM_CANDIDATES
+    assert all(c[1].get("precond") == 0 for c in large)
+    assert {c[0] for c in large} == {c[0] for c in small if c[1].get("precond") == 0}
+    assert autotune.system_candidates(autotune.AGGREGATE_MAX_N) == small
