@@ -30,10 +30,10 @@ def _same(a, b):
         assert abs(a[k]-b[k]) <= 1e-9*abs(b[k])+1e-300, k
 
 
-def _pair(capi, pos, rad, L, params=None):
+def _pair(capi, pos, rad, L, params=None, launch=0):
     out = []
     for m in (0, 1):
-        s = capi.System(pos, rad, L, params=params, opts=capi.default_opts(precond=0, max_rhs=m))
+        s = capi.System(pos, rad, L, params=params, opts=capi.default_opts(precond=0, max_rhs=m, launch=launch))
         s.first_update()
         out.append(s)
     out[0].compute_stress(False)
@@ -127,3 +127,17 @@ def test_large_instance_paired_matches_single(capi):
             st = s.solve_stats()
             assert st["res_norm"] <= 1e-12 * st["rhs_norm"] * 1.0000001
         _same(P.state(), S.state())
+
+
+def test_paired_cooperative_launch(capi):
+    """Both forms launched cooperatively (as at N=32000, where the grid takes
+    nearly every CU), against each other and against the plain launch."""
+    pos, rad, L = conf(1500, 0.56, 4)
+    P, S = _pair(capi, pos, rad, L, launch=1)
+    Q, _ = _pair(capi, pos, rad, L, launch=2)
+    for k in range(3):
+        print("cooperative stress-controlled step", k, flush=True)
+        for s in (P, S, Q):
+            s.step(1)
+        _same(P.state(), S.state())
+        _same(P.state(), Q.state())
