@@ -211,7 +211,8 @@ def run_c5(args):
                    "parallelism": "single GPU"},
         "solver_form": {0: "mixed, LDS-staged", 1: "plain FP64", 2: "mixed, HBM-streamed"}[opts.kind],
         "precond_in_use": "aggregate" if S.precond_waves() > 0 else "6x6 block-Jacobi",
-        "rhs_per_launch": 1 if opts.max_rhs == 1 else 2,
+        # the LDS-staged block-Jacobi solver pairs the two right-hand sides
+        "rhs_per_launch": 2 if (opts.kind == 0 and opts.max_rhs != 1 and S.precond_waves() == 0) else 1,
         "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],
         "solver": {"solve_launches": launches, "avg_solve_ms": round(solve_ms/max(launches, 1), 4),
                    "inner_iters_per_solve": round((st1["total_inner"]-st0["total_inner"])/max(launches, 1), 1)},
