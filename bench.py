@@ -176,6 +176,8 @@ def run_c5(args):
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
     opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond, max_rhs=args.max_rhs)
+    if args.inner_tol is not None:
+        opts.inner_tol = args.inner_tol
 
     def to_stress_control(S):
         S.compute_stress(False)
@@ -478,6 +480,8 @@ def main():
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
     ap.add_argument("--precond", type=int, default=None,
                     help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0); the autotuner overrides it")
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="c5: residual drop per FP32 inner solve (default: the library's)")
     ap.add_argument("--max-rhs", type=int, default=0,
                     help="c5: right-hand sides per launch of the block-Jacobi solver (0: two, 1: one)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",

@@ -38,6 +38,7 @@ SYSTEM_CANDIDATES = [
     ("aggregate, regroup 2", dict(precond=1, regroup_interval=2)),
     ("aggregate, regroup 4, inner 3e-3", dict(precond=1, regroup_interval=4, inner_tol=3e-3)),
     ("block-jacobi", dict(precond=0)),
+    ("block-jacobi, inner 3e-3", dict(precond=0, inner_tol=3e-3)),
     # blocks streamed from HBM every iteration (no LDS staging), with the
     # SpMV lanes per 6x6 block row as the tuned tile parameter
     ("hbm blocks, 2 lanes per row", dict(kind=2, precond=0, lanes_per_row=2)),
