@@ -42,6 +42,8 @@ typedef struct lfd_params {
 	double repulsive_length, disp_max, dt_max, dt_min;
 	int friction_model;     /* 0 frictionless, 1 Coulomb (Contact.cpp:255-291) */
 	int lubrication_model;  /* 1 normal, 2 tangential */
+	int fixed_dt;           /* 1: every step is dt_max, no adaptTimeStep (ParameterSet::fixed_dt,
+	                           System.cpp:607-610, 1094, 1130, 1238-1243) */
 } lfd_params;
 
 typedef enum lfd_solver_kind {

@@ -107,7 +107,7 @@ class Params(ctypes.Structure):
         "shear_rate", "theta_shear", "lub_max_gap", "lub_reduce", "kn", "kt",
         "mu_static", "mu_dynamic", "tau_n", "tau_t", "sd_coeff", "repulsion",
         "repulsive_length", "disp_max", "dt_max", "dt_min")] + [
-        ("friction_model", ctypes.c_int), ("lubrication_model", ctypes.c_int)]
+        ("friction_model", ctypes.c_int), ("lubrication_model", ctypes.c_int), ("fixed_dt", ctypes.c_int)]
 
 
 class SolveOpts(ctypes.Structure):

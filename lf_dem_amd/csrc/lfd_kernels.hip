@@ -1426,13 +1426,19 @@ void launch_finish_velocity(int n, const double4* pos, const double* na6, double
 }
 
 // System::adaptTimeStep (System.cpp:1013-1066), time_end disabled; one
-// thread per instance
+// thread per instance.  With fixed_dt the step stays dt_max (adaptTimeStep is
+// not called, System.cpp:1094, 1130).
 __global__ void k_adapt_dt(DevScalars* scs, const InstConst* IC, int ninst, int advance_time)
 {
     const int k = blockIdx.x*blockDim.x+threadIdx.x;
     if (k >= ninst || IC[k].n <= 0) return;
     DevScalars* sc = scs+k;
     const Params& P = IC[k].P;
+    if (P.fixed_dt) {
+        sc->dt = P.dt_max;
+        if (advance_time) sc->time += P.dt_max;
+        return;
+    }
     const double strain_end = sc->strain_end;
     double rate = sc->flow.rate;
     double mv = sqrt(bits_to_double(sc->max_vel_bits));

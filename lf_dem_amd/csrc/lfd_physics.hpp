@@ -17,6 +17,7 @@ struct Params {
     double mu_static, mu_dynamic, tau_n, tau_t, sd_coeff, repulsion;
     double repulsive_length, disp_max, dt_max, dt_min;
     int friction_model, lubrication_model;
+    int fixed_dt;                            // constant step dt_max (System.cpp:607-610, 1238-1243)
 };
 
 // Derived constants shared by all kernels of one instance.

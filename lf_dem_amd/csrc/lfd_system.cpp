@@ -89,7 +89,7 @@ static Params to_params(const lfd_params* p)
     P.mu_dynamic = p->mu_dynamic; P.tau_n = p->tau_n; P.tau_t = p->tau_t; P.sd_coeff = p->sd_coeff;
     P.repulsion = p->repulsion; P.repulsive_length = p->repulsive_length; P.disp_max = p->disp_max;
     P.dt_max = p->dt_max; P.dt_min = p->dt_min; P.friction_model = p->friction_model;
-    P.lubrication_model = p->lubrication_model;
+    P.lubrication_model = p->lubrication_model; P.fixed_dt = p->fixed_dt;
     return P;
 }
 
@@ -913,7 +913,7 @@ static int bat_add(lfd_batch* B, int np, const double* pos, const double* radius
     for (auto& v : nb) v += coff;
     DevScalars hs;
     std::memset(&hs, 0, sizeof(hs));
-    hs.dt = 1e-4;
+    hs.dt = p->fixed_dt ? p->dt_max : 1e-4;     // dt = p.dt (System.cpp:607)
     for (int k = 0; k < 3; k++) hs.shear_disp[k] = sd[k];
     make_flow(ic.C, ic.P.shear_rate, hs.flow);
     hs.group_age = -1;
@@ -1283,11 +1283,14 @@ extern "C" int lfd_system_time_evolution(lfd_system* y, double strain_end, int* 
     if (!y || !(strain_end >= 0)) return fail(LFD_ERR_ARG, "lfd_system_time_evolution: bad argument");
     launch_set_strain_end(y->sc, y->ninst, strain_end, y->s);
     int n = 0;
+    const bool fixed = y->inst[0].ic.P.fixed_dt != 0;
     for (;;) {
         DevScalars h;
         int rc = batch_check(y, 0, &h);
         if (rc) return rc;
-        if (h.strain > strain_end-1e-8) break;
+        // keepRunning(strain_end - loop_time_adjust) (System.cpp:1189-1201, 1237-1242)
+        const double se = strain_end-(fixed ? h.dt : 0.0);
+        if (h.strain > se-1e-8 && se >= 0) break;
         if ((rc = bat_step(y))) return rc;
         n++;
     }

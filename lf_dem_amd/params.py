@@ -207,6 +207,8 @@ class Setup:
         self.rate = 1.0           # rate control: shear rate in the internal unit
         self.target_stress = 0.0  # stress control: internal stress unit (LF_DEM's input / (6 pi))
         self.internal_unit = "hydro"
+        self.output_unit = "hydro"   # unit of the data files: the control value's (SimulationInit.cpp:471)
+        self.output_ratio = 1.0      # internal force unit / output force unit (Simulation.cpp:654-660)
         self.strain_end = None    # time_end given as a strain
         self.time_end = None      # time_end given as a time (internal unit)
         self.output_data = None   # (value, "strain" | "time")
@@ -273,6 +275,8 @@ def resolve(params, control, value):
         S.internal_unit = "hydro"
         S.rate = vals["hydro"][0] if "hydro" in vals else 1.0
         S.control = "rate"
+        S.output_unit = scale
+        S.output_ratio = 1.0 / ratios[scale + "/hydro"] if scale != "hydro" else 1.0
     elif control == "stress":
         if scale == "hydro":
             raise ParameterError("please give a stress in non-hydro units")
@@ -286,6 +290,7 @@ def resolve(params, control, value):
         S.internal_unit = scale
         S.rate = 1.0
         S.control = "stress"
+        S.output_unit = scale
     else:
         raise ParameterError("unknown control variable %r" % control)
     te = vals["time_end"]
@@ -305,7 +310,7 @@ def resolve(params, control, value):
     rep = vals["repulsion"][0] if "repulsion" in vals else 0.0
     S.fixed_dt = raw["fixed_dt"]
     dt_min, dt_max = raw["dt_min"], raw["dt_max"]
-    if S.fixed_dt:                                  # a fixed step: adaptTimeStep clamped to [dt, dt]
+    if S.fixed_dt:                                  # the constant step dt (System.cpp:607-610) is dt_max
         dt_min = dt_max = raw["dt"]
     S.params = dict(shear_rate=S.rate, theta_shear=raw["theta_shear"], lub_max_gap=raw["lub_max_gap"],
                     lub_reduce=raw["lub_reduce_parameter"], kn=vals["kn"][0], kt=vals["kt"][0],
@@ -313,7 +318,7 @@ def resolve(params, control, value):
                     tau_t=vals["contact_relaxation_time_tan"][0], sd_coeff=raw["sd_coeff"], repulsion=rep,
                     repulsive_length=raw["repulsive_length"] if rep else 0.0, disp_max=raw["disp_max"],
                     dt_max=dt_max, dt_min=dt_min, friction_model=raw["friction_model"],
-                    lubrication_model=lub)
+                    lubrication_model=lub, fixed_dt=int(bool(S.fixed_dt)))
     return S
 
 

@@ -1659,7 +1659,7 @@ orc_system *orc_create(int np, const double *pos, const double *radius, const do
 	s->friction = p->friction_model != 0;
 	s->repulsive = p->repulsive_length > 0;
 	s->tangential_coeff = dashpot_tangential_coeff(p);
-	s->dt = 1e-4;
+	s->dt = p->fixed_dt ? p->dt_max : 1e-4;   /* dt = p.dt (System.cpp:607) */
 	s->init_strain_limit = 0;       /* System.cpp:90-91 */
 	s->init_rate_limit = 999;
 	s->comp_vel = calloc(ORC_NCOMP*3*(size_t)np, sizeof(double));
@@ -1750,8 +1750,8 @@ int orc_compute_velocities(orc_system *s)
 		memcpy(s->na_ang+3*i, x+6*i+3, 3*sizeof(double));
 	}
 	free(rhs);
-	if (s->in_predictor) {
-		/* computeMaxNAVelocity (System.cpp:1887-1902) */
+	if (!s->p.fixed_dt && s->in_predictor) {
+		/* computeMaxNAVelocity (System.cpp:1887-1902, called as at 2309) */
 		double m = 0;
 		for (i = 0; i < s->np; i++) {
 			const double *v = s->na_vel+3*i;
@@ -1773,8 +1773,10 @@ int orc_compute_velocities(orc_system *s)
 
 static void adapt_time_step(orc_system *s, double strain_end)
 {
-	/* System::adaptTimeStep (System.cpp:1013-1066), time_end disabled */
+	/* System::adaptTimeStep (System.cpp:1013-1066), time_end disabled; never
+	 * called with fixed_dt (System.cpp:1094, 1130, 1243) */
 	double rate = s->p.shear_rate;
+	if (s->p.fixed_dt) return;
 	if (s->max_velocity > 0 || s->max_sliding_velocity > 0) {
 		if (s->max_velocity > s->max_sliding_velocity) s->dt = s->p.disp_max/s->max_velocity;
 		else s->dt = s->p.disp_max/s->max_sliding_velocity;
@@ -2114,7 +2116,7 @@ int orc_compute_velocities_components(orc_system *s)
 		if (!s->comp_present[c]) continue;
 		for (i = 0; i < (int)np; i++) { add3(s->na_vel+3*i, v+3*i); add3(s->na_ang+3*i, w+3*i); }
 	}
-	if (s->in_predictor) {
+	if (!s->p.fixed_dt && s->in_predictor) {   /* System.cpp:2309 */
 		double m = 0;
 		for (i = 0; i < (int)np; i++) {
 			const double *v = s->na_vel+3*i;
@@ -2263,9 +2265,11 @@ int orc_time_evolution(orc_system *s, double strain_end)
 {
 	/* System::timeEvolution (System.cpp:1203-1279), time_end disabled:
 	 * steps while keepRunning (System.cpp:1189-1201), then the last step
-	 * with the stress, without a new adaptTimeStep */
+	 * with the stress, without a new adaptTimeStep.  With fixed_dt the loop
+	 * stops one step earlier (loop_time_adjust, System.cpp:1237-1242). */
 	int n = 0;
-	while (!(s->strain > strain_end-1e-8 && strain_end >= 0)) {
+	const double se = strain_end-(s->p.fixed_dt ? s->dt : 0);
+	while (!(s->strain > se-1e-8 && se >= 0)) {
 		if (orc_step_ex(s, strain_end, 0, 1) < 0) return -1;
 		n++;
 	}

@@ -38,6 +38,8 @@ typedef struct orc_params {
 	double disp_max, dt_max, dt_min;
 	int friction_model;       /* 0 frictionless, 1 Coulomb */
 	int lubrication_model;    /* 1 normal, 2 tangential */
+	int fixed_dt;             /* 1: constant step dt_max, no adaptTimeStep and no maximum-velocity
+	                             update (System.cpp:607-610, 1094, 1130, 1238-1243, 2309) */
 } orc_params;
 
 void orc_default_params(orc_params *p);
