@@ -1320,9 +1320,11 @@ int orc_build_rhs(orc_system *s, double *rhs)
 int orc_build_rhs_component(orc_system *s, int comp, double *rhs)
 {
 	/* System::setSolverRHS (System.cpp:1858-1871): force and torque set,
-	 * torque zero for a component without torque */
+	 * torque zero for a component without torque; U_inf of the current
+	 * positions first, as computeVelocities does (System.cpp:2290-2292) */
 	int np = s->np, i, tq;
 	double *F = malloc(6*(size_t)np*sizeof(double)), *T = F+3*(size_t)np;
+	compute_uinf(s);
 	tq = component_forces(s, comp, F, T);
 	memset(rhs, 0, 6*(size_t)np*sizeof(double));
 	if (tq >= 0)
@@ -2336,5 +2338,12 @@ int orc_set_positions(orc_system *s, const double *pos, const double sd[3])
 	memcpy(s->pos, pos, 3*(size_t)s->np*sizeof(double));
 	if (sd) memcpy(s->shear_disp, sd, 3*sizeof(double));
 	if (s->nbox) for (i = 0; i < s->np; i++) if (box_particle(s, i)) return -1;
+	return 0;
+}
+
+int orc_set_velocities(orc_system *s, const double *vel, const double *ang)
+{
+	memcpy(s->vel, vel, 3*(size_t)s->np*sizeof(double));
+	memcpy(s->ang, ang, 3*(size_t)s->np*sizeof(double));
 	return 0;
 }

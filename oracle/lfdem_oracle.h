@@ -129,6 +129,8 @@ int orc_set_pair_history(orc_system *s, int n, const int *p0, const int *p1,
 int orc_get_state(const orc_system *s, double *pos, double *vel, double *ang_vel,
                   double *na_vel, double *na_ang, double *scalars /* 8 */);
 int orc_set_positions(orc_system *s, const double *pos, const double shear_disp[3]);
+/* total velocities and angular velocities (np x 3 each) as the next pair update reads them */
+int orc_set_velocities(orc_system *s, const double *vel, const double *ang);
 int orc_get_box_info(const orc_system *s, int *dims /* 4: nx ny nz boxed */, int *labels);
 int orc_get_neighbor_cells(const orc_system *s, int cell, int *out, int cap);
 

@@ -66,6 +66,7 @@ def lib():
         L.orc_set_pair_history.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _ip, _dp, _dp]
         L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 6
         L.orc_set_positions.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_set_velocities.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_get_box_info.argtypes = [ctypes.c_void_p, _ip, _ip]
         L.orc_get_neighbor_cells.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int]
         L.orc_build_blocks.argtypes = [ctypes.c_void_p, _dp, _ip, _ip, _dp, ctypes.c_int]
@@ -323,6 +324,9 @@ class System:
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         sd = None if shear_disp is None else P(shear_disp)
         return self._check(lib().orc_set_positions(self.h, pos.ctypes.data_as(_dp), sd))
+
+    def set_velocities(self, vel, ang):
+        return lib().orc_set_velocities(self.h, P(vel), P(ang))
 
     @property
     def npairs(self):
