@@ -225,33 +225,6 @@ def run_c5(args):
     print(json.dumps(out), flush=True)
 
 
-def sweep_specs(count, seed):
-    """Batched sweep: N log-uniform in [250, 2000], phi uniform in [0.45, 0.58],
-    shear rate log-uniform over 3 decades in repulsive-force units
-    (10^U(-2, 1), i.e. repulsion = 1/rate in hydrodynamic units); instance k
-    is seeded seed + k."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(count):
-        n = int(round(np.exp(rng.uniform(np.log(250), np.log(2000)))))
-        phi = float(rng.uniform(0.45, 0.58))
-        rate_r = float(10 ** rng.uniform(-2, 1))
-        out.append((n, phi, seed + k, 1.0 / rate_r))
-    return out
-
-
-def _gen(spec):
-    from lf_dem_amd import confgen
-    n, phi, seed, _ = spec
-    return confgen.generate(n, phi, seed)
-
-
-def generate_many(specs, workers=8):
-    from concurrent.futures import ProcessPoolExecutor
-    with ProcessPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(_gen, specs))
-
-
 def run_c3(args):
     """Batched sweep of 64 independent instances stepped together: one set of
     kernels per phase and the solver packed into cooperative launches.  The
@@ -260,9 +233,10 @@ def run_c3(args):
     from lf_dem_amd import capi
     if capi.lib().lfd_device_count() < 1:
         raise SystemExit("no GPU visible")
-    specs = sweep_specs(args.instances, args.seed)
+    from lf_dem_amd import workloads
+    specs = workloads.sweep_specs(args.instances, args.seed)
     t0 = time.time()
-    confs = generate_many(specs)
+    confs = workloads.generate_many(specs)
     gen_s = time.time() - t0
     print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
@@ -335,22 +309,6 @@ def run_c3(args):
     print(json.dumps(out), flush=True)
 
 
-def serving_requests(seed, rate, duration, strain):
-    """Poisson arrivals (mean `rate` per second) over `duration` seconds; N
-    log-uniform in [200, 8000], phi uniform in [0.50, 0.58], each running
-    `strain` strain units."""
-    rng = np.random.default_rng(seed)
-    t, out = 0.0, []
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration:
-            break
-        n = int(round(np.exp(rng.uniform(np.log(200), np.log(8000)))))
-        phi = float(rng.uniform(0.50, 0.58))
-        out.append((t, n, phi, seed * 1000 + len(out), strain))
-    return out
-
-
 def run_c4(args):
     """Serving: Poisson-arriving simulation requests admitted into one batch
     by footprint as room frees up, stepped together, retired at their strain
@@ -361,12 +319,13 @@ def run_c4(args):
     if capi.lib().lfd_device_count() < 1:
         raise SystemExit("no GPU visible")
     from lf_dem_amd import confgen
-    arr = serving_requests(args.seed, args.rate, args.duration, args.strain)
+    from lf_dem_amd import workloads
+    arr = workloads.serving_requests(args.seed, args.rate, args.duration, args.strain)
     # the first requests' configurations are prepared before the clock starts;
     # later ones (if the loop reaches them) are generated at admission
     k = min(len(arr), args.serve_pregen)
     t0 = time.time()
-    confs = generate_many([(n, phi, sd, 0.0) for (_, n, phi, sd, _) in arr[:k]], workers=16)
+    confs = workloads.generate_many([(n, phi, sd) for (_, n, phi, sd, _) in arr[:k]], workers=16)
     gen_s = time.time() - t0
     print("%d of %d request configurations generated in %.1f s" % (k, len(arr), gen_s), file=sys.stderr,
           flush=True)

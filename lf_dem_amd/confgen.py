@@ -33,14 +33,25 @@ def box_size(radius, phi):
 
 
 def _pairs(pos, reach, L):
+    """Pairs (i < j) closer than reach, sorted by (i, j)."""
     from scipy.spatial import cKDTree
     tree = cKDTree(np.mod(pos, L), boxsize=L)
-    return tree.query_pairs(r=reach, output_type="ndarray")
+    pr = tree.query_pairs(r=reach, output_type="ndarray")
+    if len(pr):
+        pr = np.sort(pr, axis=1)
+        pr = pr[np.lexsort((pr[:, 1], pr[:, 0]))]
+    return pr
 
 
 def generate(n, phi, seed, a1=1.0, a2=1.4, overlap_tol=2e-3, max_iter=20000):
     """Returns (positions (n,3) in [0,L)^3, radii (n,), L) with max overlap
-    (a_i + a_j - r_ij)/(a_i + a_j) below overlap_tol."""
+    (a_i + a_j - r_ij)/(a_i + a_j) below overlap_tol.
+
+    The candidate pairs are a Verlet list: all pairs within the overlap
+    reach plus a skin, rebuilt when the particles' largest accumulated
+    displacement (twice, for two particles) plus the growth of the reach
+    since the last build exceeds the skin, so every overlapping pair is in
+    the list."""
     rng = np.random.default_rng(seed)
     n1, n2 = bidisperse_counts(n, a1, a2)
     radius = np.concatenate([np.full(n1, a1), np.full(n2, a2)])
@@ -49,14 +60,20 @@ def generate(n, phi, seed, a1=1.0, a2=1.4, overlap_tol=2e-3, max_iter=20000):
     scale = 0.6
     it = 0
     step = 0.4
+    skin = 0.6 * a2
+    cand, moved, reach0 = None, 0.0, 0.0
+    worst = 1.0
     while it < max_iter:
         it += 1
         r = radius * scale
-        pr = _pairs(pos, 2 * r.max(), L)
-        if len(pr) == 0:
+        reach = 2 * r.max()
+        if cand is None or 2 * moved + (reach - reach0) > skin:
+            cand = _pairs(pos, reach + skin, L)
+            moved, reach0 = 0.0, reach
+        if len(cand) == 0:
             worst = 0.0
         else:
-            i, j = pr[:, 0], pr[:, 1]
+            i, j = cand[:, 0], cand[:, 1]
             d = pos[j] - pos[i]
             d -= L * np.round(d / L)
             dist = np.sqrt(np.einsum("ij,ij->i", d, d))
@@ -68,11 +85,12 @@ def generate(n, phi, seed, a1=1.0, a2=1.4, overlap_tol=2e-3, max_iter=20000):
             if len(ov):
                 u = d / np.maximum(dist, 1e-12)[:, None]
                 push = (step * ov)[:, None] * u
-                disp = np.zeros_like(pos)
-                np.add.at(disp, i, -push)
-                np.add.at(disp, j, push)
+                disp = np.empty_like(pos)
+                for k in range(3):
+                    disp[:, k] = np.bincount(j, push[:, k], minlength=n) - np.bincount(i, push[:, k], minlength=n)
                 pos += disp
                 pos %= L
+                moved += float(np.sqrt(np.einsum("ij,ij->i", disp, disp).max()))
         if scale < 1.0:
             if worst < 0.05:
                 scale = min(1.0, scale + 0.01)
