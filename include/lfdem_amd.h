@@ -116,8 +116,10 @@ int lfd_stokes_reset(lfd_stokes *s, int nb_blocks_hint, const double *drag_dbloc
 /* StokesSolver::startNewColumn (StokesSolver.cpp:921-935) */
 int lfd_stokes_start_new_column(lfd_stokes *s);
 /* StokesSolver::addResistanceBlocks (StokesSolver.cpp:121-129): i is the
- * current column, j > i ascending within the column; dbi/dbj DBlocks (15),
- * odb the ODBlock (18) at (row j, column i). */
+ * current column, j > i in any order within the column (StokesSolver.h:104-105;
+ * a row twice in one column fails at lfd_stokes_fill_done); dbi/dbj DBlocks
+ * (15) added to the diagonal blocks of i and j, odb the ODBlock (18) at
+ * (row j, column i). */
 int lfd_stokes_add_blocks(lfd_stokes *s, int i, int j, const double *dbi, const double *dbj, const double *odb);
 /* StokesSolver::matrixFillingDone (StokesSolver.cpp:937-948): uploads the
  * matrix and builds the preconditioner (the "factorization"). */

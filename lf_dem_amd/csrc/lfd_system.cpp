@@ -1759,8 +1759,6 @@ extern "C" int lfd_stokes_add_blocks(lfd_stokes* st, int i, int j, const double*
     if (!st || !dbi || !dbj || !odb) return fail(LFD_ERR_ARG, "addResistanceBlocks: bad argument");
     if (i != st->current_column-1) return fail(LFD_ERR_ARG, "addResistanceBlocks: i is not the current column");
     if (j <= i || j >= st->np) return fail(LFD_ERR_ARG, "addResistanceBlocks: need i < j < np");
-    if (!st->op0.empty() && st->op0.back() == i && st->op1.back() >= j)
-        return fail(LFD_ERR_ARG, "addResistanceBlocks: rows within a column must be ascending");
     for (int m = 0; m < 15; m++) { st->dblocks[15*(size_t)i+m] += dbi[m]; st->dblocks[15*(size_t)j+m] += dbj[m]; }
     st->op0.push_back(i);
     st->op1.push_back(j);
@@ -1768,9 +1766,42 @@ extern "C" int lfd_stokes_add_blocks(lfd_stokes* st, int i, int j, const double*
     return LFD_OK;
 }
 
+// Rows of a column may arrive in any order (StokesSolver.h:104-105): the
+// column's entries are put in ascending row order here, the order the
+// device's pair list and its transpose index assume; a row given twice in
+// one column is an error.
+static int sort_columns(lfd_stokes* st)
+{
+    const int P = (int)st->op0.size();
+    std::vector<int> perm(P);
+    for (int k = 0; k < P; k++) perm[k] = k;
+    bool moved = false;
+    for (int a = 0; a < P;) {
+        int b = a;
+        while (b < P && st->op0[b] == st->op0[a]) b++;
+        std::stable_sort(perm.begin()+a, perm.begin()+b, [&](int x, int y) { return st->op1[x] < st->op1[y]; });
+        for (int k = a+1; k < b; k++)
+            if (st->op1[perm[k]] == st->op1[perm[k-1]])
+                return fail(LFD_ERR_ARG, "matrixFillingDone: a row appears twice in column " + std::to_string(st->op0[a]));
+        for (int k = a; k < b; k++) moved |= perm[k] != k;
+        a = b;
+    }
+    if (!moved) return LFD_OK;
+    std::vector<int> r1(P);
+    std::vector<double> od(18*(size_t)P);
+    for (int k = 0; k < P; k++) {
+        r1[k] = st->op1[perm[k]];
+        std::memcpy(&od[18*(size_t)k], &st->odb[18*(size_t)perm[k]], 18*sizeof(double));
+    }
+    st->op1.swap(r1);
+    st->odb.swap(od);
+    return LFD_OK;
+}
+
 extern "C" int lfd_stokes_fill_done(lfd_stokes* st)
 {
     if (!st) return fail(LFD_ERR_ARG, "null handle");
+    if (int rc = sort_columns(st)) return rc;
     int n = st->np, P = (int)st->op0.size();
     if (P > st->cap) {
         int ncap = std::max(P, 2*st->cap);

@@ -137,23 +137,63 @@ def test_plain_and_cooperative_launch_agree(kind, precond, capi):
         assert np.array_equal(out[0][key], out[1][key]), key
 
 
+def _pair_blocks(oracle, R, rad):
+    """Per pair (p0, p1, DBlock of p0, DBlock of p1, ODBlock) as
+    Interaction::RFU_DBlocks / RFU_ODBlock give them (Interaction.cpp:238-311):
+    dashpot blocks in contact, lubrication blocks otherwise."""
+    pr = R.pairs()
+    out = []
+    for k in range(len(pr["p0"])):
+        i, j = int(pr["p0"][k]), int(pr["p1"][k])
+        if pr["state"][k] >= 1:
+            b = oracle.dashpot_blocks(rad[i], rad[j], pr["nvec"][k], R.params)
+        else:
+            b = oracle.lub_blocks(rad[i], rad[j], pr["gap"][k], pr["nvec"][k], R.params)
+        out.append((i, j) + tuple(b))
+    return out
+
+
+def _drag(rad, sd_coeff):
+    """The drag seed of System.cpp:553-575 as packed DBlocks."""
+    d = np.zeros((len(rad), 15))
+    d[:, [0, 5, 8]] = (sd_coeff * rad)[:, None]
+    d[:, [9, 12, 14]] = (sd_coeff * 4.0 / 3 * rad ** 3)[:, None]
+    return d
+
+
 def test_stokes_dropin_matches_direct(golden, oracle, capi):
-    """Blocks handed over through the StokesSolver column protocol."""
+    """The system handed over through the StokesSolver column protocol as
+    System::buildResistanceMatrix does (System.cpp:1429-1473): drag blocks
+    at the reset, then per column every pair's two DBlocks and its ODBlock,
+    the rows of a column in a shuffled order (StokesSolver.h:104-105 allows
+    any order).  The diagonal accumulation must rebuild the oracle's
+    assembled matrix; the solve must match its direct solve."""
     pos, rad, L, sd = _c1()
     R = oracle.System(pos, rad, L, sd)
     R.first_update()
     db, cp, rows, od = R.blocks()
     b = R.rhs()
     n = len(rad)
+    blocks = _pair_blocks(oracle, R, rad)
+    drag = _drag(rad, R.params.sd_coeff)
+    acc = drag.copy()
+    for i, j, di, dj, _ in blocks:
+        acc[i] += di
+        acc[j] += dj
+    assert np.abs(acc - db).max() <= 1e-12 * np.abs(db).max()
+    assert any(1 <= s for s in R.pairs()["state"])           # contacts: dashpot blocks take part
     ss = capi.StokesSolver(n)
-    # the oracle's assembled diagonal blocks enter as the reset blocks, the
-    # pair diagonals as zeros: the column protocol then carries the ODBlocks
-    ss.resetResistanceMatrix(len(rows), db, True)
-    zero = np.zeros(15)
+    ss.resetResistanceMatrix(len(blocks), drag, True)
+    rng = np.random.default_rng(5)
+    by_col = {}
+    for blk in blocks:
+        by_col.setdefault(blk[0], []).append(blk)
     for i in range(n - 1):
         ss.startNewColumn()
-        for k in range(cp[i], cp[i + 1]):
-            ss.addResistanceBlocks(i, rows[k], zero, zero, od[k])
+        col = by_col.get(i, [])
+        for t in rng.permutation(len(col)):
+            _, j, di, dj, o = col[t]
+            ss.addResistanceBlocks(i, j, di, dj, o)
     ss.matrixFillingDone()
     x_ref = oracle.Chol(n, db, cp, rows, od).solve(b)
     A = oracle.assemble_dense(n, db, cp, rows, od)
