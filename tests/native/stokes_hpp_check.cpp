@@ -53,10 +53,10 @@ int main(int argc, char** argv)
             double c = 0.3+0.01*i+0.05*step;
             DBlock d;
             d.col0[0] = d.col1[0] = d.col2[0] = c;
-            d.col3[0] = d.col4[0] = d.col5[0] = 0.5*c;
+            d.col3[0] = d.col4[0] = d.col5[0] = 0.5*c;       // C00, C11, C22 of the packed DBlock
             ODBlock o;
             o.col0[0] = o.col1[0] = o.col2[0] = -c;
-            o.col3[0] = o.col4[0] = o.col5[0] = -0.5*c;
+            o.col3[2] = o.col4[1] = o.col5[0] = -0.5*c;      // C00, C11, C22 of the packed ODBlock
             ss.addResistanceBlocks(i, j, std::make_pair(d, d), o);
             for (int k = 0; k < 6; k++) {
                 double v = k < 3 ? c : 0.5*c;
