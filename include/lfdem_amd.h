@@ -141,6 +141,32 @@ int lfd_stokes_solving_done(lfd_stokes *s);
 /* y = R x with the current matrix (StokesSolver::multiplySolutionByResMat's role) */
 int lfd_stokes_multiply(lfd_stokes *s, const double *x, double *y);
 
+/* Allocation-free solve.  No counterpart in LF_DEM: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It does what lfd_stokes_fill_done
+ * followed by lfd_stokes_solve does, with the mixed FP32/FP64 6x6
+ * block-Jacobi solver, on a matrix, right-hand side and solution held by the
+ * caller in device memory:
+ *   dblocks   np summed diagonal DBlocks (15 doubles each: drag plus the pair
+ *             contributions, as after the column protocol),
+ *   p0, p1    nblocks pairs p0 < p1 < np, sorted by (p0, p1), no repeats,
+ *   odblocks  their ODBlocks (18 doubles) at (row p1, column p0),
+ *   rhs       6 np doubles [F xyz, T xyz] per particle,
+ *   x         6 np doubles, receives [U xyz, W xyz] per particle.
+ * opts may be NULL (defaults; kind, precond, launch and max_rhs are ignored);
+ * stream is the caller's hipStream_t (NULL: the null stream); the call ends
+ * with a synchronisation of that stream to return its status.  All work
+ * arrays sit in a static device buffer of LFD_NOALLOC_BYTES per GPU, held
+ * until the process ends: at most LFD_NOALLOC_MAX_NP particles and
+ * LFD_NOALLOC_MAX_BLOCKS blocks, larger sizes return LFD_ERR_CAPACITY.  Calls
+ * on one GPU share that buffer: calls that overlap on one GPU race. */
+#define LFD_NOALLOC_MAX_NP 16384
+#define LFD_NOALLOC_MAX_BLOCKS 131072
+#define LFD_NOALLOC_BYTES 25165824
+int lfd_stokes_solve_noalloc(int np, int nblocks, const double *dblocks, const int *p0, const int *p1,
+                             const double *odblocks, const double *rhs, double *x, const lfd_solve_opts *opts,
+                             void *stream, lfd_solve_stats *stats);
+
 /* ------------------------------------------------------------------ */
 /* Device-resident System step loop                                     */
 /* ------------------------------------------------------------------ */

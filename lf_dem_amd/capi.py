@@ -35,6 +35,7 @@ SIGNATURES = {
     "lfd_stokes_start_new_column": ([_vp], ctypes.c_int),
     "lfd_stokes_add_blocks": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
     "lfd_stokes_fill_done": ([_vp], ctypes.c_int),
+    "lfd_stokes_solve_noalloc": ([ctypes.c_int, ctypes.c_int] + [_vp] * 9, ctypes.c_int),
     "lfd_stokes_reset_rhs": ([_vp], ctypes.c_int),
     "lfd_stokes_reset_rhs_torque": ([_vp], ctypes.c_int),
     "lfd_stokes_add_rhs_force": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
@@ -180,6 +181,17 @@ def material_functions(total, contact, shear_rate):
     else:
         out["N1_viscosity"] = out["N2_viscosity"] = float("nan")
     return out
+
+
+def solve_noalloc(np_, nblocks, dblocks, p0, p1, odblocks, rhs, x, opts=None, stream=None):
+    """lfd_stokes_solve_noalloc on device arrays given by their addresses
+    (integers, e.g. a device tensor's data_ptr()); returns the solve stats."""
+    st = SolveStats()
+    _check(lib().lfd_stokes_solve_noalloc(int(np_), int(nblocks), dblocks, p0, p1, odblocks, rhs, x,
+                                          ctypes.byref(opts) if opts is not None else None, stream,
+                                          ctypes.byref(st)))
+    return {"iters_inner": st.iters_inner, "iters_outer": st.iters_outer, "res_norm": st.res_norm,
+            "rhs_norm": st.rhs_norm}
 
 
 def default_opts(**kw):

@@ -178,6 +178,9 @@ inline hipError_t launch_persistent(const void* fn, int G, int block, void** arg
     return hipLaunchCooperativeKernel(fn, dim3(G), dim3(block), args, 0, s);
 }
 
+// Records msg as the calling thread's lfd_last_error() and returns code.
+int set_last_error(int code, const std::string& msg);
+
 // ---------------- launchers (lfd_kernels.hip) ----------------
 void launch_fill_int(int* p, int v, size_t n, hipStream_t s);
 void launch_fill_double(double* p, double v, size_t n, hipStream_t s);

@@ -21,6 +21,8 @@ static int fail(int code, const std::string& msg)
     return code;
 }
 
+int lfd::set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+
 #define HIPCHK(expr)                                                                      \
     do {                                                                                  \
         hipError_t e_ = (expr);                                                           \
