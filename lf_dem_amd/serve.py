@@ -109,15 +109,21 @@ class Server:
                 self.done.append(req)
         return ms
 
-    def run(self, requests, duration=None, clock=time.perf_counter):
+    def run(self, requests, duration=None, drain=0.0, clock=time.perf_counter):
         """Feeds `requests` (sorted by arrival, seconds from the start) through
-        the loop until all are done or `duration` seconds have passed."""
+        the loop until all are done or `duration` seconds have passed; with
+        `drain` > 0 the requests that arrived by then are served for up to
+        `drain` seconds more (no new arrivals)."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         t_start = clock()
         while pending or self.waiting or self.live:
             now = clock() - t_start
             if duration is not None and now > duration:
-                break
+                while pending and pending[0].arrival <= duration:
+                    self.submit(pending.popleft())
+                pending.clear()
+                if now > duration + drain or not (self.waiting or self.live):
+                    break
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
             self.admit(now)
@@ -128,18 +134,24 @@ class Server:
         return clock() - t_start
 
     def summary(self, elapsed):
-        lat = np.array(self.step_ms) if self.step_ms else np.zeros(1)
+        """Step latency, throughput and, over the completed requests, response
+        time (arrival to retirement) and admission wait; percentiles of an
+        empty sample are None."""
+        def pct(x):
+            if len(x) == 0:
+                return None
+            return {"p50": float(np.percentile(x, 50)), "p99": float(np.percentile(x, 99))}
         fin = [r for r in self.done if r.status == 0]
-        resp = np.array([r.finished - r.arrival for r in fin]) if fin else np.zeros(1)
         adm = self.done+list(self.live.values())
-        wait = np.array([r.admitted - r.arrival for r in adm]) if adm else np.zeros(1)
         return {
             "batch_steps": len(self.step_ms),
-            "step_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "step_ms": pct(self.step_ms),
             "instance_steps_per_s": self.instance_steps / elapsed,
             "particle_steps_per_s": self.particle_steps / elapsed,
             "completed": len(fin), "failed": len(self.done) - len(fin),
+            "completed_per_s": len(fin) / elapsed,
             "live_at_end": len(self.live), "waiting_at_end": len(self.waiting),
-            "response_s": {"p50": float(np.percentile(resp, 50)), "p99": float(np.percentile(resp, 99))},
-            "admission_wait_s": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))},
+            "response_s": pct([r.finished - r.arrival for r in fin]),
+            "steps_per_request": pct([r.steps for r in fin]),
+            "admission_wait_s": pct([r.admitted - r.arrival for r in adm]),
         }
