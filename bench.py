@@ -214,7 +214,8 @@ def run_c5(args):
         "solver_form": {0: "mixed, LDS-staged", 1: "plain FP64", 2: "mixed, HBM-streamed"}[opts.kind],
         "precond_in_use": "aggregate" if S.precond_waves() > 0 else "6x6 block-Jacobi",
         # the LDS-staged block-Jacobi solver pairs the two right-hand sides
-        "rhs_per_launch": 2 if (opts.kind == 0 and opts.max_rhs != 1 and S.precond_waves() == 0) else 1,
+        # from the launches that ran: a stress-controlled step solves 4 right-hand sides
+        "rhs_per_launch": round(4 * args.steps / launches, 2) if launches else None,
         "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],
         "solver": {"solve_launches": launches, "avg_solve_ms": round(solve_ms/max(launches, 1), 4),
                    "inner_iters_per_solve": round((st1["total_inner"]-st0["total_inner"])/max(launches, 1), 1)},
@@ -411,9 +412,35 @@ def run_copy(args):
           flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _host_cores():
+    """Cores this process may use: its affinity set, capped by the thread
+    budget the environment sets (OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
 def cpu_leg(pos, rad, L, params, opts, args):
-    """FP64 direct (oracle) solve at t=0: error of the GPU velocities and the
-    host-core time of one assembly + sparse Cholesky factorisation + solve."""
+    """FP64 direct (oracle) solve at t=0 for the error of the GPU velocities;
+    then the CPU baseline: one predictor-corrector step of the oracle (two
+    FP64 assemblies and sparse Cholesky factorisations and solves, pair and
+    cell updates) on the host, by phase, on 1 core, and the same step run
+    concurrently on every core this process may use (independent copies:
+    the aggregate rate of one simulation per core)."""
+    import subprocess
+    import tempfile
     from lf_dem_amd import capi
     from oracle import oracle as O
     S = capi.System(pos, rad, L, params=params, opts=opts)
@@ -423,14 +450,36 @@ def cpu_leg(pos, rad, L, params, opts, args):
     R = O.System(pos, rad, L)
     R.first_update()
     R.set_in_predictor(True)
-    t0 = time.perf_counter()
     R.compute_velocities()
-    t_cv = time.perf_counter() - t0
     r = R.state()
     err = {k: float(np.linalg.norm(g[k] - r[k]) / np.linalg.norm(r[k])) for k in ("na_vel", "na_ang_vel")}
-    cpu = {"value": round(1.0 / (2 * t_cv), 5), "unit": "steps/s", "cores": 1, "kind": "port",
-           "sample": "one FP64 assembly + minimum-degree sparse Cholesky factorisation + solve at N=%d "
-                     "(%.1f s); a step is two of these" % (args.n, t_cv)}
+    del S, R
+    with tempfile.TemporaryDirectory() as tmp:
+        conf = os.path.join(tmp, "conf.npz")
+        np.savez(conf, pos=pos, radius=rad, L=L)
+        cmd = [sys.executable, "-m", "oracle.cpu_baseline", conf, "1"]
+
+        def run(k):
+            procs = [subprocess.Popen(cmd, cwd=ROOT, stdout=subprocess.PIPE, text=True) for _ in range(k)]
+            t0 = time.perf_counter()
+            outs = [json.loads(p.communicate()[0].strip().splitlines()[-1]) for p in procs]
+            if any(p.returncode for p in procs):
+                raise SystemExit("cpu baseline: the oracle step failed")
+            return outs, time.perf_counter() - t0
+        print("cpu baseline: one oracle step on 1 core", file=sys.stderr, flush=True)
+        one, _ = run(1)
+        cores = _host_cores()
+        print("cpu baseline: %d concurrent oracle steps" % cores, file=sys.stderr, flush=True)
+        many, wall = run(cores) if cores > 1 else (one, one[0]["step_s"])
+    t1 = one[0]["step_s"]
+    cpu = {"value": round(1.0 / t1, 5), "unit": "steps/s", "cores": 1, "kind": "port",
+           "sample": "one predictor-corrector step of the oracle at N=%d (FP64 assembly, minimum-degree sparse "
+                     "Cholesky, two factorisations and solves, %.1f s) from the t=0 state" % (args.n, t1),
+           "phases_s": {k: round(v, 4) for k, v in one[0]["phases"].items()},
+           "all_cores": {"value": round(cores / wall, 5), "unit": "steps/s", "cores": cores,
+                         "sample": "%d independent copies of the same step run concurrently (%.1f s wall)"
+                                   % (cores, wall)},
+           "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
     return err, cpu
 
 

@@ -30,6 +30,9 @@ import numpy as np
 from . import capi
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "lf_dem_amd", "autotune.json")
+# short trials separate candidates only by more than their run-to-run noise:
+# among the candidates within MARGIN of the fastest, the one listed first wins
+MARGIN = 0.03
 
 # (label, SolveOpts overrides) tried for one instance
 SYSTEM_CANDIDATES = [
@@ -151,7 +154,8 @@ def _tune(sig, candidates, timer, base, cache_path, use_cache):
     ok = [(t["steps_per_s"], i) for i, t in enumerate(trials) if t["steps_per_s"]]
     if not ok:
         raise RuntimeError("autotune: no candidate could run this workload")
-    best = candidates[max(ok)[1]]
+    top = max(r for r, _ in ok)
+    best = candidates[min(i for r, i in ok if r >= top / (1 + MARGIN))]
     out = {"choice": best[0], "options": best[1], "trials": trials, "seconds": round(time.time() - t0, 2),
            "cached": False}
     if cache_path:

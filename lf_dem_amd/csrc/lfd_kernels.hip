@@ -2574,13 +2574,15 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         if (s_fail) break;
     }
     if (wg == 0 && threadIdx.x == 0) {
-        // the right-hand side with the largest relative residual
-        int kw = 0;
+        // the right-hand side with the largest relative residual among the
+        // nonzero ones (a zero one is solved exactly by x = 0)
+        int kw = -1;
         bool converged = true;
         for (int k = 0; k < NR; k++) {
             converged &= conv[k];
-            if (k > 0 && rr64[k]*bb[kw] > rr64[kw]*bb[k]) kw = k;
+            if (bb[k] > 0 && (kw < 0 || rr64[k]*bb[kw] > rr64[kw]*bb[k])) kw = k;
         }
+        if (kw < 0) kw = 0;
         sc->res_norm = sqrt(rr64[kw]);
         sc->rhs_norm = sqrt(bb[kw]);
         sc->iters_inner = inner_total;

@@ -17,12 +17,27 @@
  *    left-looking 6x6-block factorisation) standing in for CHOLMOD, which is
  *    not part of the reference tree.
  */
+#define _POSIX_C_SOURCE 199309L   /* clock_gettime under -std=c99 */
 #include "lfdem_oracle.h"
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <stdint.h>
+#include <time.h>
+
+/* Wall time per phase of the step path, accumulated over all systems of the
+ * process (the CPU baseline's per-phase split; single-threaded use). */
+static double g_phase[ORC_NPHASE];
+static double ph_now(void)
+{
+	struct timespec t;
+	clock_gettime(CLOCK_MONOTONIC, &t);
+	return (double)t.tv_sec+1e-9*(double)t.tv_nsec;
+}
+void orc_phase_times(double out[ORC_NPHASE]) { memcpy(out, g_phase, sizeof(g_phase)); }
+void orc_phase_reset(void) { memset(g_phase, 0, sizeof(g_phase)); }
+#define PH(k, stmt) do { double t0_ = ph_now(); stmt; g_phase[k] += ph_now()-t0_; } while (0)
 
 /* ======================================================================= */
 /* small vector helpers, same operation order as vec3d.h                   */
@@ -1370,6 +1385,7 @@ static int chol6(double *A)
 orc_chol *orc_chol_factor(int np, const double *dblocks, const int *cp, const int *rows,
                           const double *odb, int *status)
 {
+	const double t_sym = ph_now();
 	orc_chol *c = calloc(1, sizeof(orc_chol));
 	int n = np, words = (np+63)/64;
 	uint64_t *adj = calloc((size_t)n*words, sizeof(uint64_t));
@@ -1446,8 +1462,10 @@ orc_chol *orc_chol_factor(int np, const double *dblocks, const int *cp, const in
 		}
 	}
 	free(pat); free(pat_start);
+	g_phase[ORC_PH_ORDER] += ph_now()-t_sym;
 	/* numeric: left-looking over block columns */
 	{
+		const double t_num = ph_now();
 		size_t nnz = (size_t)c->cp[n];
 		double *W;
 		int *map = malloc(n*sizeof(int));
@@ -1554,6 +1572,7 @@ orc_chol *orc_chol_factor(int np, const double *dblocks, const int *cp, const in
 		}
 		free(W); free(map); free(head); free(nxt); free(ptr);
 		free(acnt); free(aidx); free(ablk); free(afl);
+		g_phase[ORC_PH_FACTOR] += ph_now()-t_num;
 	}
 	free(nb);
 	return c;
@@ -1733,9 +1752,9 @@ int orc_solve_current(orc_system *s, const double *rhs, double *x)
 	orc_chol *c;
 	int st;
 	ensure_block_scratch(s);
-	orc_build_blocks(s, s->dblocks, s->od_cp, s->od_rows, s->odblocks, s->od_cap);
+	PH(ORC_PH_ASSEMBLE, orc_build_blocks(s, s->dblocks, s->od_cp, s->od_rows, s->odblocks, s->od_cap));
 	c = orc_chol_factor(s->np, s->dblocks, s->od_cp, s->od_rows, s->odblocks, &st);
-	orc_chol_solve(c, rhs, x);
+	PH(ORC_PH_SOLVE, orc_chol_solve(c, rhs, x));
 	orc_chol_free(c);
 	return st;
 }
@@ -1745,7 +1764,7 @@ int orc_compute_velocities(orc_system *s)
 	/* System::computeVelocities, rate control, no components (System.cpp:2273-2319) */
 	int i, st;
 	double *rhs = malloc(12*(size_t)s->np*sizeof(double)), *x = rhs+6*(size_t)s->np;
-	orc_build_rhs(s, rhs);
+	PH(ORC_PH_ASSEMBLE, orc_build_rhs(s, rhs));
 	st = orc_solve_current(s, rhs, x);
 	for (i = 0; i < s->np; i++) {
 		memcpy(s->na_vel+3*i, x+6*i, 3*sizeof(double));
@@ -2087,13 +2106,13 @@ int orc_compute_velocities_components(orc_system *s)
 	if (s->control == 1) set_shear_rate(s, 1);
 	compute_uinf(s);
 	ensure_block_scratch(s);
-	orc_build_blocks(s, s->dblocks, s->od_cp, s->od_rows, s->odblocks, s->od_cap);
+	PH(ORC_PH_ASSEMBLE, orc_build_blocks(s, s->dblocks, s->od_cp, s->od_rows, s->odblocks, s->od_cap));
 	ch = orc_chol_factor(s->np, s->dblocks, s->od_cp, s->od_rows, s->odblocks, &st);
 	for (c = 0; c < ORC_NCOMP; c++) {
 		double *v = s->comp_vel+3*np*c, *w = s->comp_ang+3*np*c;
-		s->comp_present[c] = orc_build_rhs_component(s, c, rhs) == 0;
+		PH(ORC_PH_ASSEMBLE, s->comp_present[c] = orc_build_rhs_component(s, c, rhs) == 0);
 		if (!s->comp_present[c]) continue;
-		orc_chol_solve(ch, rhs, x);
+		PH(ORC_PH_SOLVE, orc_chol_solve(ch, rhs, x));
 		for (i = 0; i < (int)np; i++) {
 			memcpy(v+3*i, x+6*i, 3*sizeof(double));
 			memcpy(w+3*i, x+6*i+3, 3*sizeof(double));
@@ -2224,6 +2243,7 @@ int orc_step_ex(orc_system *s, double strain_end, int calc_stress, int adapt_fir
 	/* System::timeEvolution loop body + timeEvolutionPredictorCorrectorMethod
 	 * (System.cpp:1241-1256, 888-983, 1124-1187) */
 	int i, st = 0;
+	double t0;
 	if (adapt_first) adapt_time_step(s, strain_end);
 	/* predictor */
 	s->in_predictor = 1;
@@ -2231,12 +2251,14 @@ int orc_step_ex(orc_system *s, double strain_end, int calc_stress, int adapt_fir
 	if (calc_stress) orc_calc_stress(s);
 	adapt_time_step(s, strain_end);
 	s->time += s->dt;
+	t0 = ph_now();
 	if (time_step_boxing(s)) return -1;
 	for (i = 0; i < s->np; i++) {
 		double dr[3] = {s->vel[3*i]*s->dt, s->vel[3*i+1]*s->dt, s->vel[3*i+2]*s->dt};
 		if (displacement(s, i, dr)) return -1;
 	}
-	orc_update_interactions(s);
+	g_phase[ORC_PH_MOVE] += ph_now()-t0;
+	PH(ORC_PH_PAIRS, orc_update_interactions(s));
 	memcpy(s->vel_pred, s->vel, 3*(size_t)s->np*sizeof(double));
 	memcpy(s->ang_pred, s->ang, 3*(size_t)s->np*sizeof(double));
 	/* corrector */
@@ -2247,14 +2269,16 @@ int orc_step_ex(orc_system *s, double strain_end, int calc_stress, int adapt_fir
 		s->vel[i] = 0.5*(s->vel[i]+s->vel_pred[i]);
 		s->ang[i] = 0.5*(s->ang[i]+s->ang_pred[i]);
 	}
+	t0 = ph_now();
 	for (i = 0; i < s->np; i++) {
 		double dr[3] = {(s->vel[3*i]-s->vel_pred[3*i])*s->dt,
 		                (s->vel[3*i+1]-s->vel_pred[3*i+1])*s->dt,
 		                (s->vel[3*i+2]-s->vel_pred[3*i+2])*s->dt};
 		if (displacement(s, i, dr)) return -1;
 	}
-	orc_check_new_interaction(s);
-	orc_update_interactions(s);
+	g_phase[ORC_PH_MOVE] += ph_now()-t0;
+	PH(ORC_PH_NEIGHBOR, orc_check_new_interaction(s));
+	PH(ORC_PH_PAIRS, orc_update_interactions(s));
 	return st;
 }
 

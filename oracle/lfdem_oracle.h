@@ -134,6 +134,18 @@ int orc_set_velocities(orc_system *s, const double *vel, const double *ang);
 int orc_get_box_info(const orc_system *s, int *dims /* 4: nx ny nz boxed */, int *labels);
 int orc_get_neighbor_cells(const orc_system *s, int cell, int *out, int cap);
 
+/* ---- wall time per phase (the CPU baseline's split), seconds ---- */
+enum { ORC_PH_MOVE = 0,      /* displacements, Lees-Edwards boxing, cell membership */
+       ORC_PH_NEIGHBOR = 1,  /* checkNewInteraction */
+       ORC_PH_PAIRS = 2,     /* updateInteractions */
+       ORC_PH_ASSEMBLE = 3,  /* resistance blocks and right-hand sides */
+       ORC_PH_ORDER = 4,     /* minimum-degree ordering and symbolic pattern */
+       ORC_PH_FACTOR = 5,    /* numeric factorisation */
+       ORC_PH_SOLVE = 6,     /* triangular solves */
+       ORC_NPHASE = 7 };
+void orc_phase_times(double out[ORC_NPHASE]);
+void orc_phase_reset(void);
+
 /* ---- sparse Cholesky on 6x6-block SPD matrices ---- */
 typedef struct orc_chol orc_chol;
 orc_chol *orc_chol_factor(int np, const double *dblocks, const int *od_col_ptr,

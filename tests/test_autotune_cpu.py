@@ -58,3 +58,15 @@ def test_aggregate_candidates_dropped_above_the_grouping_limit():
     assert all(c[1].get("precond") == 0 for c in large)
     assert {c[0] for c in large} == {c[0] for c in small if c[1].get("precond") == 0}
     assert autotune.system_candidates(autotune.AGGREGATE_MAX_N) == small
+
+
+def test_near_ties_keep_the_candidate_listed_first():
+    """Trials that differ by less than the margin (run-to-run noise) keep the
+    earlier candidate; a clear win takes over."""
+    cands = [("a", dict(precond=0)), ("b", dict(precond=1))]
+    near = {0: 100.0, 1: 100.0 * (1 + autotune.MARGIN * 0.5)}
+    opts, rep = autotune._tune("s", cands, lambda o: near[o.precond], None, None, False)
+    assert rep["choice"] == "a"
+    far = {0: 100.0, 1: 100.0 * (1 + autotune.MARGIN * 2)}
+    opts, rep = autotune._tune("s", cands, lambda o: far[o.precond], None, None, False)
+    assert rep["choice"] == "b"

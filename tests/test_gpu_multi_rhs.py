@@ -83,6 +83,9 @@ def test_zero_right_hand_side_beside_a_nonzero_one(capi):
         P.step(1)
         S.step(1)
         _same(P.state(), S.state())
+        # the paired launch reports the nonzero side's residual, not 0 / 0
+        st = P.solve_stats()
+        assert st["rhs_norm"] > 0 and st["res_norm"] <= 1e-12 * st["rhs_norm"] * 1.0000001
     assert np.isfinite(P.state()["vel"]).all()
 
 
