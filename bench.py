@@ -41,6 +41,16 @@ def solver_bytes_per_iter(n, p, waves=0):
     return spmv + precond + vec
 
 
+def solver_bytes_per_iter_rhs(n, p, nrhs):
+    """As solver_bytes_per_iter with the 6x6 block-Jacobi preconditioner for
+    nrhs right-hand sides solved together: the blocks and inverses are read
+    once per iteration for all of them, the vectors once per right-hand side."""
+    spmv = 2 * p * (72 + 4) + n * (60 + 8) + nrhs * 6 * 4 * n
+    precond = 144 * n + nrhs * 2 * 6 * 4 * n
+    vec = nrhs * (n * 6 * 4 * (4 + 3) + n * 6 * 4 * 4)
+    return spmv + precond + vec
+
+
 def solver_bytes_per_outer(n, p):
     """FP64 residual r = b - R x: packed FP64 blocks both ways, diag, x, b, r."""
     return 2 * p * (144 + 4) + n * (120 + 8) + n * 6 * 8 * 3
@@ -66,8 +76,9 @@ def run(args):
     from lf_dem_amd import capi, confgen
     if capi.lib().lfd_device_count() < 1:
         raise SystemExit("no GPU visible")
+    from lf_dem_amd import workloads
     t0 = time.time()
-    pos, rad, L = confgen.generate(args.n, args.phi, args.seed)
+    pos, rad, L = workloads.generate(args.n, args.phi, args.seed)
     gen_s = time.time() - t0
     params = capi.default_params()
     opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
@@ -169,9 +180,10 @@ def run_c5(args):
     from lf_dem_amd import capi, confgen
     if capi.lib().lfd_device_count() < 1:
         raise SystemExit("no GPU visible")
+    from lf_dem_amd import workloads
     n, phi = 32000, 0.58
     t0 = time.time()
-    pos, rad, L = confgen.generate(n, phi, args.seed)
+    pos, rad, L = workloads.generate(n, phi, args.seed)
     gen_s = time.time() - t0
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
@@ -203,6 +215,17 @@ def run_c5(args):
     S.step_stress()
     sr = S.stress()
     sps = args.steps/elapsed
+    # roofline of the solver launch (block-Jacobi LDS solver, one or two
+    # right-hand sides per launch as they ran)
+    nrhs = max(1, int(round(4 * args.steps / launches))) if launches else 1
+    avg_ms = solve_ms / max(launches, 1)
+    iters = (st1["total_inner"] - st0["total_inner"]) / max(launches, 1)
+    npairs = S.npairs
+    alg = iters * solver_bytes_per_iter_rhs(n, npairs, nrhs) + \
+        max(st1["iters_outer"], 1) * nrhs * solver_bytes_per_outer(n, npairs)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    kernel = "k_solve_lds_pair" if nrhs == 2 else "k_solve_lds"
+    traffic = measured_traffic(kernel, "n32000_phi058") if (args.seed == 1 and S.precond_waves() == 0) else None
     out = {
         "metric": "steps/s at N=32000 phi=0.58 stress control (2 RHS per solve)", "value": round(sps, 3),
         "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -219,6 +242,9 @@ def run_c5(args):
         "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],
         "solver": {"solve_launches": launches, "avg_solve_ms": round(solve_ms/max(launches, 1), 4),
                    "inner_iters_per_solve": round((st1["total_inner"]-st0["total_inner"])/max(launches, 1), 1)},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": round(traffic) if traffic else None, "algorithmic_bytes_per_launch": round(alg)},
         "confgen_s": round(gen_s, 1),
     }
     if tuned:

@@ -5,8 +5,10 @@ names and call protocol of StokesSolver.h:198-285; ``System`` exposes the
 device-resident step loop of System.cpp:888-983.  There is no CPU fallback:
 if the library is missing or no GPU is present the calls fail loudly.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -144,6 +146,26 @@ def lib():
     return _lib
 
 
+# Handles still open when the interpreter exits are destroyed by an exit hook
+# of this module, while the HIP runtime is still up (before the runtime's own
+# teardown and any profiler's finalisation run), newest first.
+_LIVE = weakref.WeakValueDictionary()
+_SERIAL = [0]
+
+
+def _track(obj):
+    _SERIAL[0] += 1
+    _LIVE[_SERIAL[0]] = obj
+
+
+@atexit.register
+def _release_all():
+    for key in sorted(list(_LIVE.keys()), reverse=True):
+        obj = _LIVE.get(key)
+        if obj is not None:
+            obj.close()
+
+
 def _check(rc):
     if rc < 0:
         raise LfdError(rc, lib().lfd_last_error().decode())
@@ -210,15 +232,19 @@ class StokesSolver:
         _check(lib().lfd_stokes_create(np_total, np_total if np_mobile is None else np_mobile,
                                        ctypes.byref(h)))
         self.h = h
+        _track(self)
         self.np = np_total
         if opts is not None:
             _check(lib().lfd_stokes_set_opts(self.h, ctypes.byref(opts)))
         self.last_stats = SolveStats()
 
-    def __del__(self):
+    def close(self):
+        """Releases the device memory and stream (also done on deletion and at exit)."""
         if getattr(self, "h", None):
             lib().lfd_stokes_destroy(self.h)
             self.h = None
+
+    __del__ = close
 
     def resetResistanceMatrix(self, nb_blocks_hint, drag_dblocks, pattern_changed=True):
         d = np.ascontiguousarray(drag_dblocks, dtype=np.float64)
@@ -301,11 +327,15 @@ class System:
                                        ctypes.byref(self.params), ctypes.byref(self.opts),
                                        ctypes.byref(h)))
         self.h = h
+        _track(self)
 
-    def __del__(self):
+    def close(self):
+        """Releases the device memory and stream (also done on deletion and at exit)."""
         if getattr(self, "h", None):
             lib().lfd_system_destroy(self.h)
             self.h = None
+
+    __del__ = close
 
     def set_opts(self, opts):
         self.opts = opts
@@ -499,12 +529,16 @@ class Batch:
         _check(lib().lfd_batch_create(int(slot_cap), int(cell_cap), int(inst_cap), {"rate": 0, "stress": 1}[control],
                                       ctypes.byref(self.opts), ctypes.byref(h)))
         self.h = h
+        _track(self)
         self.np = {}
 
-    def __del__(self):
+    def close(self):
+        """Releases the device memory and stream (also done on deletion and at exit)."""
         if getattr(self, "h", None):
             lib().lfd_batch_destroy(self.h)
             self.h = None
+
+    __del__ = close
 
     def room(self, slots, cells):
         return bool(_check(lib().lfd_batch_room(self.h, int(slots), int(cells))))

@@ -2576,15 +2576,16 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
     if (wg == 0 && threadIdx.x == 0) {
         // the right-hand side with the largest relative residual among the
         // nonzero ones (a zero one is solved exactly by x = 0)
-        int kw = -1;
         bool converged = true;
+        double best = -1.0, rsel = rr64[0], bsel = bb[0];
+#pragma unroll
         for (int k = 0; k < NR; k++) {
             converged &= conv[k];
-            if (bb[k] > 0 && (kw < 0 || rr64[k]*bb[kw] > rr64[kw]*bb[k])) kw = k;
+            const double q = bb[k] > 0 ? rr64[k]/bb[k] : -1.0;
+            if (q > best) { best = q; rsel = rr64[k]; bsel = bb[k]; }
         }
-        if (kw < 0) kw = 0;
-        sc->res_norm = sqrt(rr64[kw]);
-        sc->rhs_norm = sqrt(bb[kw]);
+        sc->res_norm = sqrt(rsel);
+        sc->rhs_norm = sqrt(bsel);
         sc->iters_inner = inner_total;
         sc->iters_outer = outer;
         sc->total_inner += inner_total;

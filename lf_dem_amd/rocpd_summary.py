@@ -5,8 +5,13 @@ when rocprofv3's own CSV stats are not written (it can stop with a fault at
 process exit after cooperative launches, with the database already complete).
 
 usage: python -m lf_dem_amd.rocpd_summary RESULTS.db [--top N]
+       python -m lf_dem_amd.rocpd_summary --pmc COUNTERS.csv --kernel REGEX
+(the second form: mean / min / max of each counter over the dispatches of the
+kernels matching REGEX in a ``rocprofv3 --pmc ... -f csv`` counter file)
 """
 import argparse
+import csv
+import re
 import sqlite3
 
 
@@ -24,12 +29,37 @@ def summarize(path, top=30):
     return "\n".join(out)
 
 
+def summarize_pmc(path, kernel):
+    """Per counter: dispatches, mean, min, max over the kernels matching the regex."""
+    pat = re.compile(kernel)
+    vals = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            name = row.get("Kernel_Name") or row.get("Kernel-Name") or ""
+            if not pat.search(name):
+                continue
+            key = (row.get("Dispatch_Id"), row.get("Counter_Name"))
+            vals.setdefault(row.get("Counter_Name"), {}).setdefault(key, 0.0)
+            vals[row.get("Counter_Name")][key] += float(row.get("Counter_Value") or 0)
+    out = []
+    for cname, per in sorted(vals.items()):
+        v = list(per.values())
+        out.append("%s  %s  %d dispatches  mean %.1f  min %.1f  max %.1f" % (
+            kernel, cname, len(v), sum(v) / len(v), min(v), max(v)))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("db")
+    ap.add_argument("db", nargs="?")
     ap.add_argument("--top", type=int, default=30)
+    ap.add_argument("--pmc")
+    ap.add_argument("--kernel", default=".")
     a = ap.parse_args()
-    print(summarize(a.db, a.top))
+    if a.pmc:
+        print(summarize_pmc(a.pmc, a.kernel))
+    else:
+        print(summarize(a.db, a.top))
 
 
 if __name__ == "__main__":

@@ -11,6 +11,8 @@
 All bidisperse (radii 1.0 / 1.4, 50:50 by volume) and generated by
 lf_dem_amd.confgen.
 """
+import os
+
 import numpy as np
 
 from . import confgen
@@ -46,6 +48,25 @@ def serving_requests(seed, rate, duration, strain):
         phi = float(rng.uniform(0.50, 0.58))
         out.append((t, n, phi, seed * 1000 + len(out), strain))
     return out
+
+
+def generate(n, phi, seed):
+    """confgen.generate, kept as conf_<n>_<phi>_<seed>.npz in the directory
+    named by LFD_CONF_CACHE when that is set (several runs of one job
+    generating the same large configuration)."""
+    cache = os.environ.get("LFD_CONF_CACHE")
+    if not cache:
+        return confgen.generate(n, phi, seed)
+    path = os.path.join(cache, "conf_%d_%.4f_%d.npz" % (n, phi, seed))
+    if os.path.exists(path):
+        d = np.load(path)
+        return d["pos"], d["radius"], d["L"]
+    pos, rad, L = confgen.generate(n, phi, seed)
+    os.makedirs(cache, exist_ok=True)
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, pos=pos, radius=rad, L=L)
+    os.replace(tmp, path)
+    return pos, rad, L
 
 
 def _gen(spec):
