@@ -391,7 +391,7 @@ def _serve_phase(args, rate, duration, drain, seed, pregen):
 def run_c4(args):
     """Serving: Poisson-arriving simulation requests admitted into one batch
     by footprint as room frees up, stepped together, retired at their strain
-    (lf_dem_amd.serve).  Two runs: below capacity (--sub-rate requests/s for
+    (lf_dem_amd.serve).  Two runs: below capacity (--sub-rate requests/s, 0.25, for
     --sub-duration s, then served to completion) for request response times,
     and at the offered load of the workload (--rate, 20/s, for --duration s),
     which exceeds the capacity, for the saturated throughput and batch-step
@@ -524,9 +524,9 @@ def main():
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--rate", type=float, default=20.0, help="c4: request arrivals per second (saturation run)")
     ap.add_argument("--duration", type=float, default=25.0, help="c4: seconds of arrivals and service (saturation)")
-    ap.add_argument("--sub-rate", type=float, default=1.0, help="c4: arrivals per second of the run below capacity")
-    ap.add_argument("--sub-duration", type=float, default=60.0, help="c4: seconds of arrivals below capacity")
-    ap.add_argument("--sub-drain", type=float, default=90.0,
+    ap.add_argument("--sub-rate", type=float, default=0.25, help="c4: arrivals per second of the run below capacity")
+    ap.add_argument("--sub-duration", type=float, default=80.0, help="c4: seconds of arrivals below capacity")
+    ap.add_argument("--sub-drain", type=float, default=200.0,
                     help="c4: seconds to finish the requests of the run below capacity after the arrivals")
     ap.add_argument("--strain", type=float, default=0.5, help="c4: strain units per request")
     ap.add_argument("--serve-slots", type=int, default=131072, help="c4: particle slots of the batch")
