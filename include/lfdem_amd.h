@@ -88,6 +88,10 @@ typedef struct lfd_solve_opts {
 	                           takes together when a step solves several with one matrix (the rate
 	                           independent and rate proportional parts of stress control, the four
 	                           parts of a stress evaluation): 0 = two, 1 = one */
+	int agg_waves;          /* LFD_PRECOND_AGGREGATE: waves of 8 particles per workgroup of the solver,
+	                           8 or 12 (0 = 12).  More workgroups with fewer waves each balance
+	                           the wavefronts over the SIMDs (two per SIMD) for a lone instance;
+	                           a batch packs more particles per launch with 12. */
 } lfd_solve_opts;
 
 typedef struct lfd_solve_stats {

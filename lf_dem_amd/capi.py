@@ -118,7 +118,7 @@ class SolveOpts(ctypes.Structure):
                 ("max_outer", ctypes.c_int), ("max_inner", ctypes.c_int),
                 ("lanes_per_row", ctypes.c_int), ("groups", ctypes.c_int), ("kind", ctypes.c_int),
                 ("precond", ctypes.c_int), ("regroup_interval", ctypes.c_int), ("launch", ctypes.c_int),
-                ("max_rhs", ctypes.c_int)]
+                ("max_rhs", ctypes.c_int), ("agg_waves", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):

@@ -518,8 +518,7 @@ void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStrea
 }
 
 // ------------------------------------------------------------------ persistent solver
-static constexpr int AS_WAVES = AGG_WAVES_PER_WG, AS_BLOCK = AS_WAVES*WAVE, AS_LPR = 8;
-static constexpr int AS_ROWS = AS_BLOCK/AS_LPR;     // solver slots per workgroup (12 waves of 8)
+static constexpr int AS_LPR = 8;
 static constexpr int AS_ENTRY_CAP = 1024;           // staged blocks per workgroup (78 KB)
 
 struct AsRow { int e0, ne; };       // staged entries of a slot row (ne < 0: not staged)
@@ -533,9 +532,14 @@ struct AsRow { int e0, ne; };       // staged entries of a slot row (ne < 0: not
 //  * the preconditioner z = W^-1 r for the wave W of the lane's wavefront:
 //    the 48 residual components go through a per-wave LDS vector and every
 //    lane forms its row of the packed symmetric inverse.
-__global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, const double* __restrict__ b,
+// AS_WAVES waves (wavefronts) per workgroup: 12 (three per SIMD) or 8 (two
+// per SIMD, more workgroups for the same waves).
+template <int AS_WAVES>
+__global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView Gv, const double* __restrict__ b,
     double* __restrict__ x, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
 {
+    constexpr int AS_BLOCK = AS_WAVES*WAVE;
+    constexpr int AS_ROWS = AS_BLOCK/AS_LPR;        // solver slots per workgroup
     __shared__ int ent_j[AS_ENTRY_CAP];             // partner slot; ~slot for the transposed block
     __shared__ int ent_k[AS_ENTRY_CAP];             // pair index (FP64 block)
     __shared__ float ent_b[AS_ENTRY_CAP*18];
@@ -1011,24 +1015,30 @@ __global__ __launch_bounds__(AS_BLOCK) void k_solve_agg(BsrView A, AggView Gv, c
     if (threadIdx.x == 0 && s_fail) atomicOr(&sc->error, E_SYNC_TIMEOUT);
 }
 
-int agg_solver_max_groups()
+template <int W> static int agg_groups_of()
 {
     static int cached = 0;
     if (!cached) {
         int dev = 0, ncu = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_agg, AS_BLOCK, 0) != hipSuccess || per < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_solve_agg<W>, W*WAVE, 0) != hipSuccess || per < 1)
             per = 1;
         cached = ncu*per;
     }
     return cached;
 }
 
+int agg_solver_max_groups(int waves)
+{
+    return waves == AGG_WAVES_MIN ? agg_groups_of<AGG_WAVES_MIN>() : agg_groups_of<AGG_WAVES_PER_WG>();
+}
+
 hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double* x, SolverWork& w,
                      DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s)
 {
-    if (nwg > agg_solver_max_groups()) return hipErrorCooperativeLaunchTooLarge;
+    const int W = o.agg_waves == AGG_WAVES_MIN ? AGG_WAVES_MIN : AGG_WAVES_PER_WG;
+    if (nwg > agg_solver_max_groups(W)) return hipErrorCooperativeLaunchTooLarge;
     BsrView Av = A;
     AggView Gv = G;
     SolverWork wv = w;
@@ -1036,7 +1046,9 @@ hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double
     // no stale round tags in any instance's partial records
     launch_fill_double(G.part, 0.0, G.part_len, s);
     void* args[] = {&Av, &Gv, (void*)&b, (void*)&x, &wv, &sc, &ov, (void*)&jobs};
-    return launch_persistent((const void*)k_solve_agg, nwg, AS_BLOCK, args, o, agg_solver_max_groups(), s);
+    const void* fn = W == AGG_WAVES_MIN ? (const void*)k_solve_agg<AGG_WAVES_MIN>
+                                        : (const void*)k_solve_agg<AGG_WAVES_PER_WG>;
+    return launch_persistent(fn, nwg, W*WAVE, args, o, agg_solver_max_groups(W), s);
 }
 
 }  // namespace lfd

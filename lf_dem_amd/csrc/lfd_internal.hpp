@@ -94,10 +94,12 @@ static constexpr int AGG_CAP = 8;                  // particles per wave
 static constexpr int AGG_DIM = 6*AGG_CAP;          // rows of a wave block
 static constexpr int AGG_PACK = AGG_DIM*(AGG_DIM+1)/2;
 static constexpr int AGG_SLOTS_PER_BLOCK = 160;
-static constexpr int AGG_WAVES_PER_WG = 12;        // wavefronts (waves) per solver workgroup
+static constexpr int AGG_WAVES_PER_WG = 12;        // wavefronts (waves) per solver workgroup (default)
+static constexpr int AGG_WAVES_MIN = 8;            // the other choice of lfd_solve_opts.agg_waves
 static constexpr int AGG_ADJ_K = 8;               // strongest couplings kept per particle for the grouping
 static constexpr int AGG_NMAX = 8192;              // largest instance the grouping kernel takes
-static constexpr int AGG_PART_PER_BLOCK = 64;      // partial-record doubles per 128-slot block
+static constexpr int AGG_PART_PER_BLOCK = 128;     // partial-record doubles per 128-slot block (22 per
+                                                   // workgroup, up to 2.5 workgroups per block plus one)
 
 struct AggView {
     int* perm;             // [solver slots] particle slot, -1 for padding
@@ -165,6 +167,7 @@ struct SolveOpts {
     int groups;            // workgroups of the persistent kernel (0 = auto)
     int launch;            // lfd_launch
     int max_rhs;           // right-hand sides one LDS solver launch takes together (1 or 2)
+    int agg_waves;         // waves per workgroup of the aggregate solver (8 or 12)
 };
 
 // Launch of a persistent solver kernel of G workgroups when at most `cap`
@@ -256,9 +259,10 @@ void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, in
 // one step more since each instance's grouping
 void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s);
 void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStream_t s);
+// o.agg_waves waves (8 or 12) per workgroup
 hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double* x, SolverWork& w,
                      DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
-int agg_solver_max_groups();
+int agg_solver_max_groups(int waves);
 hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
 void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);

@@ -74,6 +74,7 @@ extern "C" void lfd_default_solve_opts(lfd_solve_opts* o)
     o->regroup_interval = 4;
     o->launch = LFD_LAUNCH_AUTO;
     o->max_rhs = 0;
+    o->agg_waves = 0;
 }
 
 extern "C" int lfd_device_count(void)
@@ -101,6 +102,7 @@ static SolveOpts to_opts(const lfd_solve_opts* o)
     so.tol = o->tol; so.inner_tol = o->inner_tol; so.max_outer = o->max_outer; so.max_inner = o->max_inner;
     so.lanes_per_row = o->lanes_per_row; so.block = 256; so.groups = o->groups; so.launch = o->launch;
     so.max_rhs = o->max_rhs == 1 ? 1 : 2;
+    so.agg_waves = o->agg_waves == AGG_WAVES_MIN ? AGG_WAVES_MIN : AGG_WAVES_PER_WG;
     return so;
 }
 
@@ -203,7 +205,7 @@ struct Solver {
             agg.part_len = (size_t)AGG_PART_PER_BLOCK*rb+64;
             A(dalloc(&agg.part, agg.part_len));
             A(dalloc(&agg_jobs, rb+1));
-            ajobs_cap = 2*rb+64;
+            ajobs_cap = 3*rb+std::max(icap, 1)+64;    // up to ceil(2.5 m) workgroups per instance of m blocks
             A(dalloc(&ajobs, ajobs_cap));
             agg.wave_inst = wave_inst;
             agg.wave_local = wave_local;
@@ -256,7 +258,8 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     S.single = spans.size() == 1 ? spans[0].inst : -1;
     if (spans.empty()) return LFD_OK;
     const int maxg = std::min(lds_solver_max_groups(), S.w.max_groups);
-    const int amaxg = S.scap > 0 ? agg_solver_max_groups() : 0;
+    const int aw = S.opts.agg_waves;
+    const int amaxg = S.scap > 0 ? agg_solver_max_groups(aw) : 0;
     std::vector<SolveJob> jobs, ajobs;
     std::vector<AggJob> gjobs;
     std::fill(S.h_wave_inst.begin(), S.h_wave_inst.end(), -1);
@@ -264,7 +267,7 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     for (const InstSpan& sp : spans) {
         const int m = (sp.n+LDS_ROWS-1)/LDS_ROWS;
         const int bw = m*AGG_SLOTS_PER_BLOCK/AGG_CAP;
-        const int bg = (bw+AGG_WAVES_PER_WG-1)/AGG_WAVES_PER_WG;
+        const int bg = (bw+aw-1)/aw;
         if (S.scap > 0 && S.precond == LFD_PRECOND_AGGREGATE && S.kind == LFD_SOLVER_MIXED && sp.n <= AGG_NMAX &&
             bg <= amaxg) {
             const int soff = sp.row0/LDS_ROWS*AGG_SLOTS_PER_BLOCK;
@@ -821,7 +824,7 @@ static bool capturable(const lfd_batch* B)
     if (S.kind != LFD_SOLVER_MIXED || S.profiling || (S.launches.empty() && S.agg_launches.empty())) return false;
     auto plain = [&](int nwg, int cap) { return S.opts.launch == LFD_LAUNCH_PLAIN ||
                                                 (S.opts.launch == LFD_LAUNCH_AUTO && 2*nwg <= cap); };
-    for (auto& L : S.agg_launches) if (!plain(L.second, agg_solver_max_groups())) return false;
+    for (auto& L : S.agg_launches) if (!plain(L.second, agg_solver_max_groups(S.opts.agg_waves))) return false;
     for (auto& L : S.launches) if (!plain(L.second, lds_solver_max_groups())) return false;
     return true;
 }

@@ -94,17 +94,18 @@ def test_blocks_and_rhs(case, golden, oracle, capi):
     assert np.abs(r_gpu - r_orc).max() <= REL_BLOCK * np.abs(r_orc).max()
 
 
-# (solver kind, preconditioner): mixed with the aggregate preconditioner (the
-# default), mixed with 6x6 block-Jacobi, plain FP64, mixed from HBM
-SOLVERS = [(0, 1), (0, 0), (1, 0), (2, 0)]
+# (solver kind, preconditioner, aggregate waves per workgroup): mixed with
+# the aggregate preconditioner (the default, and with 8 waves per
+# workgroup), mixed with 6x6 block-Jacobi, plain FP64, mixed from HBM
+SOLVERS = [(0, 1, 0), (0, 1, 8), (0, 0, 0), (1, 0, 0), (2, 0, 0)]
 
 
-@pytest.mark.parametrize("kind,precond", SOLVERS)
+@pytest.mark.parametrize("kind,precond,agg_waves", SOLVERS)
 @pytest.mark.parametrize("case", ALL)
-def test_velocities_vs_direct(case, kind, precond, golden, oracle, capi):
+def test_velocities_vs_direct(case, kind, precond, agg_waves, golden, oracle, capi):
     pos, rad, L, sd = _make(case, golden)
-    print("case", case, "solver kind", kind, "preconditioner", precond, flush=True)
-    opts = capi.default_opts(kind=kind, precond=precond)
+    print("case", case, "solver kind", kind, "preconditioner", precond, "waves", agg_waves, flush=True)
+    opts = capi.default_opts(kind=kind, precond=precond, agg_waves=agg_waves)
     S = capi.System(pos, rad, L, sd, opts=opts)
     R = oracle.System(pos, rad, L, sd)
     S.first_update()
