@@ -170,7 +170,7 @@ def run(args):
         "config": {"workload": "N=4000 bidisperse 1.0/1.4 phi=0.56 rate-controlled shear, mixed precision",
                    "N": args.n, "phi": args.phi, "seed": args.seed, "pairs": npairs, "tol": args.tol,
                    "precond": ["block-jacobi", "aggregate"][opts.precond], "regroup_interval": opts.regroup_interval,
-                   "inner_tol": opts.inner_tol, "agg_waves": opts.agg_waves or 12,
+                   "inner_tol": opts.inner_tol, "agg_waves": opts.agg_waves or 8,
                    "parallelism": "single GPU"},
         "stepping": "captured step graphs" if graph_sps else "eager",
         "eager_steps_per_s": round(eager_sps, 3),

@@ -89,9 +89,10 @@ typedef struct lfd_solve_opts {
 	                           independent and rate proportional parts of stress control, the four
 	                           parts of a stress evaluation): 0 = two, 1 = one */
 	int agg_waves;          /* LFD_PRECOND_AGGREGATE: waves of 8 particles per workgroup of the solver,
-	                           8 or 12 (0 = 12).  More workgroups with fewer waves each balance
-	                           the wavefronts over the SIMDs (two per SIMD) for a lone instance;
-	                           a batch packs more particles per launch with 12. */
+	                           8 or 12 (0 = 8).  With 8 (two wavefronts per SIMD) every lane keeps
+	                           its row of the wave's 48x48 inverse in registers; with 12 the
+	                           inverses are staged in LDS and one launch takes 1.5x the waves
+	                           (fewer cooperative launches for a large batch). */
 } lfd_solve_opts;
 
 typedef struct lfd_solve_stats {

@@ -40,9 +40,8 @@ SYSTEM_CANDIDATES = [
     ("aggregate, regroup 8", dict(precond=1, regroup_interval=8)),
     ("aggregate, regroup 2", dict(precond=1, regroup_interval=2)),
     ("aggregate, regroup 4, inner 3e-3", dict(precond=1, regroup_interval=4, inner_tol=3e-3)),
-    # 8 waves per solver workgroup (two wavefronts per SIMD, more workgroups)
-    ("aggregate, regroup 4, 8 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=8)),
-    ("aggregate, regroup 2, 8 waves per workgroup", dict(precond=1, regroup_interval=2, agg_waves=8)),
+    # 12 waves per solver workgroup (inverses in LDS, fewer workgroups)
+    ("aggregate, regroup 4, 12 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=12)),
     ("block-jacobi", dict(precond=0)),
     ("block-jacobi, inner 3e-3", dict(precond=0, inner_tol=3e-3)),
     # blocks streamed from HBM every iteration (no LDS staging), with the
@@ -69,7 +68,7 @@ def system_candidates(n):
 BATCH_CANDIDATES = [
     ("block-jacobi", dict(precond=0)),
     ("aggregate, regroup 4", dict(precond=1, regroup_interval=4)),
-    ("aggregate, regroup 4, 8 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=8)),
+    ("aggregate, regroup 4, 12 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=12)),
 ]
 
 

@@ -530,20 +530,24 @@ struct AsRow { int e0, ne; };       // staged entries of a slot row (ne < 0: not
 //    component c of particle perm[s]; b and x stay in particle order, the
 //    exchanged Krylov vectors are in slot order;
 //  * the preconditioner z = W^-1 r for the wave W of the lane's wavefront:
-//    the 48 residual components go through a per-wave LDS vector and every
-//    lane forms its row of the packed symmetric inverse.
+//    every lane forms its row of the symmetric inverse with the 48 residual
+//    components read from their lanes into scalar registers.
 // AS_WAVES waves (wavefronts) per workgroup: 12 (three per SIMD) or 8 (two
-// per SIMD, more workgroups for the same waves).
+// per SIMD, more workgroups for the same waves).  With 12 the wave inverses
+// are staged in LDS (packed lower triangle); with 8 a wavefront may hold 256
+// VGPRs, and each lane keeps its 48-entry row of the inverse in registers,
+// so an application reads no LDS.
 template <int AS_WAVES>
 __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView Gv, const double* __restrict__ b,
     double* __restrict__ x, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
 {
     constexpr int AS_BLOCK = AS_WAVES*WAVE;
     constexpr int AS_ROWS = AS_BLOCK/AS_LPR;        // solver slots per workgroup
+    constexpr bool INV_REG = AS_WAVES <= 8;         // inverse rows in registers (else LDS)
     __shared__ int ent_j[AS_ENTRY_CAP];             // partner slot; ~slot for the transposed block
     __shared__ int ent_k[AS_ENTRY_CAP];             // pair index (FP64 block)
     __shared__ float ent_b[AS_ENTRY_CAP*18];
-    __shared__ float s_inv[AS_WAVES*AGG_PACK];
+    __shared__ float s_inv[INV_REG ? 1 : AS_WAVES*AGG_PACK];
     __shared__ AsRow rows[AS_ROWS];
     __shared__ double red[3*AS_WAVES+4];
     __shared__ double red8[8*AS_WAVES+8];
@@ -575,7 +579,16 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
         const int ii = ss < send ? Gv.perm[ss] : -1;
         rows[threadIdx.x].ne = ii >= 0 ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
     }
-    {
+    // row kk of this wavefront's inverse (register form; zero for lanes 6, 7
+    // of a row group and for waves beyond the instance's)
+    float Mr[INV_REG ? AGG_DIM : 1];
+    if constexpr (INV_REG) {
+        const bool wlive = (int)wid < nw-(sbase-J.row0)/AGG_CAP;
+        const float* src = Gv.inv+(size_t)(sbase/AGG_CAP+(wlive ? wid : 0))*AGG_PACK;
+        const int kb = kk*(kk+1)/2;
+#pragma unroll
+        for (int m = 0; m < AGG_DIM; m++) Mr[m] = (wlive && c < 6) ? src[m <= kk ? kb+m : m*(m+1)/2+kk] : 0.f;
+    } else {
         const float* src = Gv.inv+(size_t)(sbase/AGG_CAP)*AGG_PACK;
         const int nload = min(AS_WAVES, nw-(sbase-J.row0)/AGG_CAP)*AGG_PACK;
         for (int a = threadIdx.x; a < AS_WAVES*AGG_PACK; a += AS_BLOCK) s_inv[a] = a < nload ? src[a] : 0.f;
@@ -659,18 +672,26 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
     // inverse (one read per product)
     auto precond = [&](float rv) -> float {
         const int x = __float_as_int(own ? rv : 0.f);
-        const float* Mw = s_inv+wid*AGG_PACK;
-        int kq = kk;
-        asm volatile("" : "+v"(kq));       // the addresses are recomputed per call, not held in registers
-        const int kb = kq*(kq+1)/2;
-        float z = 0.f;
+        if constexpr (INV_REG) {
+            float z = 0.f;
 #pragma unroll
-        for (int m = 0; m < AGG_DIM; m++) {
-            const float sm = __int_as_float(__builtin_amdgcn_readlane(x, 8*(m/6)+m%6));
-            const int a = m <= kq ? kb+m : m*(m+1)/2+kq;
-            z += Mw[a]*sm;
+            for (int m = 0; m < AGG_DIM; m++)
+                z += Mr[m]*__int_as_float(__builtin_amdgcn_readlane(x, 8*(m/6)+m%6));
+            return own ? z : 0.f;
+        } else {
+            const float* Mw = s_inv+wid*AGG_PACK;
+            int kq = kk;
+            asm volatile("" : "+v"(kq));   // the addresses are recomputed per call, not held in registers
+            const int kb = kq*(kq+1)/2;
+            float z = 0.f;
+#pragma unroll
+            for (int m = 0; m < AGG_DIM; m++) {
+                const float sm = __int_as_float(__builtin_amdgcn_readlane(x, 8*(m/6)+m%6));
+                const int a = m <= kq ? kb+m : m*(m+1)/2+kq;
+                z += Mw[a]*sm;
+            }
+            return own ? z : 0.f;
         }
-        return own ? z : 0.f;
     };
 
     double* part = Gv.part+AGG_PART_PER_BLOCK*(size_t)J.key;
@@ -1031,13 +1052,13 @@ template <int W> static int agg_groups_of()
 
 int agg_solver_max_groups(int waves)
 {
-    return waves == AGG_WAVES_MIN ? agg_groups_of<AGG_WAVES_MIN>() : agg_groups_of<AGG_WAVES_PER_WG>();
+    return waves == AGG_WAVES_WIDE ? agg_groups_of<AGG_WAVES_WIDE>() : agg_groups_of<AGG_WAVES_REG>();
 }
 
 hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double* x, SolverWork& w,
                      DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s)
 {
-    const int W = o.agg_waves == AGG_WAVES_MIN ? AGG_WAVES_MIN : AGG_WAVES_PER_WG;
+    const int W = o.agg_waves == AGG_WAVES_WIDE ? AGG_WAVES_WIDE : AGG_WAVES_REG;
     if (nwg > agg_solver_max_groups(W)) return hipErrorCooperativeLaunchTooLarge;
     BsrView Av = A;
     AggView Gv = G;
@@ -1046,8 +1067,8 @@ hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double
     // no stale round tags in any instance's partial records
     launch_fill_double(G.part, 0.0, G.part_len, s);
     void* args[] = {&Av, &Gv, (void*)&b, (void*)&x, &wv, &sc, &ov, (void*)&jobs};
-    const void* fn = W == AGG_WAVES_MIN ? (const void*)k_solve_agg<AGG_WAVES_MIN>
-                                        : (const void*)k_solve_agg<AGG_WAVES_PER_WG>;
+    const void* fn = W == AGG_WAVES_WIDE ? (const void*)k_solve_agg<AGG_WAVES_WIDE>
+                                         : (const void*)k_solve_agg<AGG_WAVES_REG>;
     return launch_persistent(fn, nwg, W*WAVE, args, o, agg_solver_max_groups(W), s);
 }
 

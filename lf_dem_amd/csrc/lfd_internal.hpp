@@ -94,8 +94,10 @@ static constexpr int AGG_CAP = 8;                  // particles per wave
 static constexpr int AGG_DIM = 6*AGG_CAP;          // rows of a wave block
 static constexpr int AGG_PACK = AGG_DIM*(AGG_DIM+1)/2;
 static constexpr int AGG_SLOTS_PER_BLOCK = 160;
-static constexpr int AGG_WAVES_PER_WG = 12;        // wavefronts (waves) per solver workgroup (default)
-static constexpr int AGG_WAVES_MIN = 8;            // the other choice of lfd_solve_opts.agg_waves
+static constexpr int AGG_WAVES_REG = 8;            // wavefronts (waves) per solver workgroup (default):
+                                                   // inverse rows in registers
+static constexpr int AGG_WAVES_WIDE = 12;          // the other choice of lfd_solve_opts.agg_waves:
+                                                   // inverses in LDS, more waves per launch
 static constexpr int AGG_ADJ_K = 8;               // strongest couplings kept per particle for the grouping
 static constexpr int AGG_NMAX = 8192;              // largest instance the grouping kernel takes
 static constexpr int AGG_PART_PER_BLOCK = 128;     // partial-record doubles per 128-slot block (22 per

@@ -228,7 +228,7 @@ extern "C" int lfd_stokes_solve_noalloc(int np, int nblocks, const double* dbloc
     SolveOpts o;
     o.tol = uo.tol; o.inner_tol = uo.inner_tol; o.max_outer = uo.max_outer; o.max_inner = uo.max_inner;
     o.lanes_per_row = uo.lanes_per_row; o.block = 256; o.groups = uo.groups; o.max_rhs = 1;
-    o.agg_waves = AGG_WAVES_PER_WG;
+    o.agg_waves = AGG_WAVES_REG;
     // the persistent solver's workgroups wait for each other: a plain launch
     // needs them all resident, which the library takes as given when the grid
     // needs at most half of the device (a cooperative launch is not used here)

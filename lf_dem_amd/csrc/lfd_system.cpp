@@ -102,7 +102,7 @@ static SolveOpts to_opts(const lfd_solve_opts* o)
     so.tol = o->tol; so.inner_tol = o->inner_tol; so.max_outer = o->max_outer; so.max_inner = o->max_inner;
     so.lanes_per_row = o->lanes_per_row; so.block = 256; so.groups = o->groups; so.launch = o->launch;
     so.max_rhs = o->max_rhs == 1 ? 1 : 2;
-    so.agg_waves = o->agg_waves == AGG_WAVES_MIN ? AGG_WAVES_MIN : AGG_WAVES_PER_WG;
+    so.agg_waves = o->agg_waves == AGG_WAVES_WIDE ? AGG_WAVES_WIDE : AGG_WAVES_REG;
     return so;
 }
 

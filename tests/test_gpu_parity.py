@@ -95,9 +95,10 @@ def test_blocks_and_rhs(case, golden, oracle, capi):
 
 
 # (solver kind, preconditioner, aggregate waves per workgroup): mixed with
-# the aggregate preconditioner (the default, and with 8 waves per
-# workgroup), mixed with 6x6 block-Jacobi, plain FP64, mixed from HBM
-SOLVERS = [(0, 1, 0), (0, 1, 8), (0, 0, 0), (1, 0, 0), (2, 0, 0)]
+# the aggregate preconditioner (the default 8 waves per workgroup with the
+# inverses in registers, and 12 with the inverses in LDS), mixed with 6x6
+# block-Jacobi, plain FP64, mixed from HBM
+SOLVERS = [(0, 1, 0), (0, 1, 12), (0, 0, 0), (1, 0, 0), (2, 0, 0)]
 
 
 @pytest.mark.parametrize("kind,precond,agg_waves", SOLVERS)

@@ -38,10 +38,18 @@ def cmp_stress(g, o):
         assert np.abs(g["particle"]-o["particle"]).max() <= REL*ps
 
 
-def cmp_vel(S, R):
+def cmp_vel(S, R, trans_rel=None):
+    """Velocities within REL of the oracle's.  With trans_rel, the mean
+    translation velocity (a near-null mode of the resistance matrix when the
+    Stokes drag is tiny) is held to trans_rel and the rest to REL."""
     g, o = S.state(), R.state()
     for k in ("na_vel", "na_ang_vel"):
-        assert np.linalg.norm(g[k]-o[k]) <= REL*np.linalg.norm(o[k]), k
+        d = g[k]-o[k]
+        ref = np.linalg.norm(o[k])
+        if trans_rel is not None and k == "na_vel":
+            assert np.linalg.norm(d.mean(0))*np.sqrt(len(d)) <= trans_rel*ref, k
+            d = d-d.mean(0)
+        assert np.linalg.norm(d) <= REL*ref, k
 
 
 def test_stress_control_golden_row_on_gpu(golden, oracle, capi):
@@ -73,7 +81,10 @@ def test_stress_control_golden_row_on_gpu(golden, oracle, capi):
         pytest.approx(0.0620998, abs=5e-8)
     assert ob.min_gap(pr, args[1], 2 + P.lub_max_gap) == pytest.approx(0.0575972, abs=5e-8)
     cmp_stress(g, o)
-    cmp_vel(S, R)
+    # sd_coeff = 1e-8: the uniform translations have eigenvalue 1.1e-8
+    # (condition number 2.5e9) and FP64 direct solves agree on them only to
+    # ~1e-8 (test_oracle_stress.py pins both), so they are held to kappa * eps
+    cmp_vel(S, R, trans_rel=3e-7)
 
 
 @pytest.fixture(scope="module")
