@@ -520,6 +520,8 @@ void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStrea
 // ------------------------------------------------------------------ persistent solver
 static constexpr int AS_LPR = 8;
 static constexpr int AS_ENTRY_CAP = 1024;           // staged blocks per workgroup (78 KB)
+static constexpr int AS_POLL_ROUNDS = 3;            // records per lane per poll of the pipelined sums
+static constexpr int AS_POLL_ROUNDS8 = 12;          // entries per lane per poll of the 8-value sums
 
 struct AsRow { int e0, ne; };       // staged entries of a slot row (ne < 0: not staged)
 
@@ -716,15 +718,10 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
-                double a0 = 0, a1 = 0, a2 = 0;
-                bool dn = true;
-                for (int q = lane; q < G; q += WAVE) {
-                    const unsigned long long v0 = ld_dev(pq+3*q), v1 = ld_dev(pq+3*q+1), v2 = ld_dev(pq+3*q+2);
-                    dn &= (v0 & 0xFF) == tg && (v1 & 0xFF) == tg && (v2 & 0xFF) == tg;
-                    a0 += untag_partial(v0); a1 += untag_partial(v1); a2 += untag_partial(v2);
-                }
+                double ac[3] = {0, 0, 0};
+                const bool dn = poll_records<3, AS_POLL_ROUNDS>(pq, G, tg, lane, ac);
                 if (__all(dn)) {
-                    a0 = wave_sum_d(a0); a1 = wave_sum_d(a1); a2 = wave_sum_d(a2);
+                    const double a0 = wave_sum_d(ac[0]), a1 = wave_sum_d(ac[1]), a2 = wave_sum_d(ac[2]);
                     if (lane == 0) { red[3*AS_WAVES] = a0; red[3*AS_WAVES+1] = a1; red[3*AS_WAVES+2] = a2; }
                     ok = true;
                     break;
@@ -777,12 +774,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
                 double acc = 0;
-                bool dn = true;
-                for (int e = lane; e < 8*G; e += WAVE) {
-                    const unsigned long long u = ld_dev(pq+e);
-                    dn &= (u & 0xFF) == tg;
-                    acc += untag_partial(u);
-                }
+                const bool dn = poll_records<1, AS_POLL_ROUNDS8>(pq, 8*G, tg, lane, &acc);
                 if (__all(dn)) {
                     acc += __shfl_xor(acc, 8);
                     acc += __shfl_xor(acc, 16);
