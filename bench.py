@@ -71,6 +71,8 @@ def measured_traffic(kernel, workload):
 
 
 AGG_PACK = 48 * 49 // 2
+# lfd_solve_opts.launch of the persistent solvers (lfd_launch)
+LAUNCHES = {"auto": 0, "cooperative": 1, "plain": 2}
 
 
 def solve_error_at_state(S, opts, capi, tight=1e-13):
@@ -103,7 +105,7 @@ def run(args):
     pos, rad, L = workloads.generate(args.n, args.phi, args.seed)
     gen_s = time.time() - t0
     params = capi.default_params()
-    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond)
     tuned = None
     if args.autotune:
         from lf_dem_amd import autotune
@@ -168,7 +170,7 @@ def run(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "N=4000 bidisperse 1.0/1.4 phi=0.56 rate-controlled shear, mixed precision",
-                   "N": args.n, "phi": args.phi, "seed": args.seed, "pairs": npairs, "tol": args.tol,
+                   "N": args.n, "phi": args.phi, "seed": args.seed, "pairs": npairs, "tol": args.tol, "launch": args.launch,
                    "precond": ["block-jacobi", "aggregate"][opts.precond], "regroup_interval": opts.regroup_interval,
                    "inner_tol": opts.inner_tol, "agg_waves": opts.agg_waves or 8,
                    "parallelism": "single GPU"},
@@ -213,7 +215,8 @@ def run_c5(args):
     gen_s = time.time() - t0
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
-    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond, max_rhs=args.max_rhs)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond,
+                             max_rhs=args.max_rhs)
     if args.inner_tol is not None:
         opts.inner_tol = args.inner_tol
 
@@ -259,7 +262,7 @@ def run_c5(args):
         "ms_per_step": round(1e3/sps, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "N=32000 bidisperse 1.0/1.4 phi=0.58 stress-controlled shear", "N": n, "phi": phi,
-                   "seed": args.seed, "pairs": S.npairs, "target_stress": target, "tol": args.tol,
+                   "seed": args.seed, "pairs": S.npairs, "target_stress": target, "tol": args.tol, "launch": args.launch,
                    "parallelism": "single GPU"},
         "solver_form": {0: "mixed, LDS-staged", 1: "plain FP64", 2: "mixed, HBM-streamed"}[opts.kind],
         "precond_in_use": "aggregate" if S.precond_waves() > 0 else "6x6 block-Jacobi",
@@ -294,7 +297,7 @@ def run_c3(args):
     confs = workloads.generate_many(specs)
     gen_s = time.time() - t0
     print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
-    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond)
     params = [capi.default_params(repulsion=s[3]) for s in specs]
     tuned = None
     if args.autotune:
@@ -317,10 +320,22 @@ def run_c3(args):
     B.sync()
     elapsed = time.perf_counter() - t0
     solve_ms, launches = B.profile(False)
+    eager_bsps = args.steps / elapsed
+    # the same steps replayed from captured step graphs when every solver
+    # launch of the batch can be captured (plain launches)
+    graph_bsps = None
+    if args.graph:
+        B.step(2, graph=True)
+        B.sync()
+        if B.graphs() > 0:
+            t0 = time.perf_counter()
+            B.step(args.steps, graph=True)
+            B.sync()
+            graph_bsps = args.steps / (time.perf_counter() - t0)
     lat = []
     for _ in range(args.latency_steps):
         t1 = time.perf_counter()
-        B.step(1)
+        B.step(1, graph=graph_bsps is not None)
         B.sync()
         lat.append((time.perf_counter() - t1) * 1e3)
     lat = np.array(lat)
@@ -337,7 +352,7 @@ def run_c3(args):
         lone_s += time.perf_counter() - t1
         del S
     total_n = sum(s[0] for s in specs)
-    bsps = args.steps / elapsed
+    bsps = graph_bsps if graph_bsps else eager_bsps
     lone_sweep_sps = lone_steps / lone_s          # sweeps of all instances per second, one at a time
     out = {
         "metric": "batched sweep: steps/s of 64 instances stepped together", "value": round(bsps, 3),
@@ -346,7 +361,9 @@ def run_c3(args):
         "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "64 instances, N log-uniform [250, 2000], phi U(0.45, 0.58), rate 10^U(-2, 1) "
                                "repulsive units", "instances": len(specs), "total_N": total_n,
-                   "seed": args.seed, "tol": args.tol, "parallelism": "single GPU"},
+                   "seed": args.seed, "tol": args.tol, "launch": args.launch, "parallelism": "single GPU"},
+        "stepping": "captured step graphs" if graph_bsps else "eager",
+        "eager_batch_steps_per_s": round(eager_bsps, 3),
         "instance_steps_per_s": round(bsps * len(specs), 1),
         "particle_steps_per_s": round(bsps * total_n, 0),
         "one_at_a_time_steps_per_s": round(lone_sweep_sps, 3),
@@ -377,7 +394,7 @@ def _serve_phase(args, rate, duration, drain, seed, pregen):
     gen_s = time.time() - t0
     print("%d of %d request configurations generated in %.1f s" % (k, len(arr), gen_s), file=sys.stderr,
           flush=True)
-    opts = capi.default_opts(kind=0, tol=args.tol, precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond)
     reqs = []
     for i, a in enumerate(arr):
         if i < k:
@@ -437,7 +454,7 @@ def run_c4(args):
         "ms_per_step": sub["step_ms"]["p50"] if sub["step_ms"] else None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": "Poisson requests, N log-uniform [200, 8000], phi U(0.50, 0.58), %.2f strain each"
-                               % args.strain, "seed": args.seed, "tol": args.tol, "parallelism": "single GPU"},
+                               % args.strain, "seed": args.seed, "tol": args.tol, "launch": args.launch, "parallelism": "single GPU"},
         "latency_ms": sub["step_ms"],
         "response_s": sub["response_s"],
         "below_capacity": sub,
@@ -545,6 +562,9 @@ def main():
     ap.add_argument("--n", type=int, default=4000)
     ap.add_argument("--phi", type=float, default=0.56)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--launch", choices=sorted(LAUNCHES), default="auto",
+                    help="persistent solver launches: the library default (auto: plain when a launch fits the "
+                         "device at once), always cooperative, or always plain")
     ap.add_argument("--tol", type=float, default=1e-10,
                     help="relative residual of the FP64 refinement (the N=4000 workload states 1e-8)")
     ap.add_argument("--plain-steps", type=int, default=2)

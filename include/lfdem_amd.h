@@ -62,13 +62,16 @@ typedef enum lfd_precond {
 /* How the persistent solver kernels are launched.  They synchronise their
  * workgroups with bounded spin waits, so every workgroup of a launch must be
  * resident at once.  A cooperative launch guarantees it and costs about 20 us
- * of idle device time per solve; a plain launch relies on the device having
- * room, which holds when no other solve runs on the device at the same time
- * (otherwise the solve ends with LFD_ERR_SYNC_TIMEOUT, never hangs).  The
+ * of idle device time per solve, and it cannot be captured in a step graph; a
+ * plain launch relies on the device having room, which holds when no other
+ * solve runs on the device at the same time (otherwise the solve ends with
+ * LFD_ERR_SYNC_TIMEOUT, never hangs).  A host that runs solves on one device
+ * from several streams at once should ask for cooperative launches.  The
  * solver that streams the matrix from HBM (LFD_SOLVER_MIXED_GLOBAL) is always
  * launched cooperatively. */
 typedef enum lfd_launch {
-	LFD_LAUNCH_AUTO = 0,           /* plain when the launch needs at most half of the device, else cooperative */
+	LFD_LAUNCH_AUTO = 0,           /* plain when the launch fits the device at once (every launch the
+	                                  library makes does), else cooperative */
 	LFD_LAUNCH_COOPERATIVE = 1,
 	LFD_LAUNCH_PLAIN = 2
 } lfd_launch;

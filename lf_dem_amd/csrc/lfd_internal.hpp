@@ -173,12 +173,14 @@ struct SolveOpts {
 };
 
 // Launch of a persistent solver kernel of G workgroups when at most `cap`
-// fit on the device at once (cooperative, or plain when that is asked for or
-// the launch needs at most half of the device).
+// fit on the device at once: plain when that is asked for, or by default
+// when the launch fits the device (the library's kernels run one at a time
+// on its stream, so an otherwise idle device holds them all), cooperative
+// when that is asked for.
 inline hipError_t launch_persistent(const void* fn, int G, int block, void** args, const SolveOpts& o, int cap,
                                     hipStream_t s)
 {
-    const bool plain = o.launch == 2 || (o.launch == 0 && 2*G <= cap);
+    const bool plain = o.launch == 2 || (o.launch == 0 && G <= cap);
     if (plain) return hipLaunchKernel(fn, dim3(G), dim3(block), args, 0, s);
     return hipLaunchCooperativeKernel(fn, dim3(G), dim3(block), args, 0, s);
 }

@@ -823,7 +823,7 @@ static bool capturable(const lfd_batch* B)
     const Solver& S = B->S;
     if (S.kind != LFD_SOLVER_MIXED || S.profiling || (S.launches.empty() && S.agg_launches.empty())) return false;
     auto plain = [&](int nwg, int cap) { return S.opts.launch == LFD_LAUNCH_PLAIN ||
-                                                (S.opts.launch == LFD_LAUNCH_AUTO && 2*nwg <= cap); };
+                                                (S.opts.launch == LFD_LAUNCH_AUTO && nwg <= cap); };
     for (auto& L : S.agg_launches) if (!plain(L.second, agg_solver_max_groups(S.opts.agg_waves))) return false;
     for (auto& L : S.launches) if (!plain(L.second, lds_solver_max_groups())) return false;
     return true;
