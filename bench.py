@@ -407,7 +407,7 @@ def _serve_phase(args, rate, duration, drain, seed, pregen):
             reqs.append(serve.Request(i, None, rad, L, strain=a[4], arrival=a[0],
                                       loader=lambda a=a: confgen.generate(a[1], a[2], a[3])[0]))
     slot_cap = args.serve_slots
-    srv = serve.Server(slot_cap, slot_cap // 4, 1024, opts=opts, keep_state=False)
+    srv = serve.Server(slot_cap, slot_cap // 4, 1024, opts=opts, keep_state=False, graph=args.graph)
     # warm the kernels on a throwaway request
     w = serve.Request(-1, *confs[0], strain=1e-3)
     srv.submit(w)
@@ -430,7 +430,8 @@ def _serve_phase(args, rate, duration, drain, seed, pregen):
             "step_ms": r3(summ["step_ms"]), "batch_steps": summ["batch_steps"],
             "instance_steps_per_s": round(summ["instance_steps_per_s"], 1),
             "particle_steps_per_s": round(summ["particle_steps_per_s"], 0),
-            "slot_cap": info["slot_cap"], "device_bytes": info["bytes"], "confgen_s": round(gen_s, 1)}
+            "slot_cap": info["slot_cap"], "device_bytes": info["bytes"], "confgen_s": round(gen_s, 1),
+            "stepping": "captured step graphs, recaptured when the batch changes" if args.graph else "eager"}
 
 
 def run_c4(args):

@@ -50,8 +50,11 @@ class Request:
 class Server:
     """Request loop over a batch of fixed capacity (slots, cells, handles)."""
 
-    def __init__(self, slot_cap, cell_cap, inst_cap, opts=None, keep_state=True):
+    def __init__(self, slot_cap, cell_cap, inst_cap, opts=None, keep_state=True, graph=False):
+        """graph: step by replaying captured step graphs (lfd_batch_step_graph),
+        recaptured whenever the batch's membership changes."""
         self.batch = capi.Batch(slot_cap, cell_cap, inst_cap, opts=opts)
+        self.graph = graph
         self.waiting = collections.deque()
         self.live = {}                 # instance id -> Request
         self.done = []
@@ -87,7 +90,7 @@ class Server:
         if not self.live:
             return 0.0
         t0 = now_fn()
-        self.batch.step(1)
+        self.batch.step(1, graph=self.graph)
         ids, strain, status = self.batch.status()
         ms = (now_fn() - t0) * 1e3
         self.step_ms.append(ms)

@@ -41,14 +41,18 @@ def _same(a, b):
         assert a[k] == b[k], k
 
 
-def test_requests_same_alone_and_in_the_mix(capi):
+@pytest.mark.parametrize("graph", [False, True])
+def test_requests_same_alone_and_in_the_mix(capi, graph):
+    """Eager batch steps, and steps replayed from step graphs recaptured at
+    every change of the batch's membership."""
     from lf_dem_amd import serve
     reqs = [_req(capi, serve, 0, 300, 0.50, 21, 5.0), _req(capi, serve, 1, 520, 0.55, 22, 0.5),
             _req(capi, serve, 2, 260, 0.45, 23, 50.0), _req(capi, serve, 3, 400, 0.52, 24, 1.0)]
     # room for requests 0 and 1 only: 2 and 3 wait for slots to free up
     slots = reqs[0].slots + reqs[1].slots
     cells = reqs[0].cells + reqs[1].cells
-    srv = serve.Server(slots, cells, 4)
+    srv = serve.Server(slots, cells, 4, graph=graph)
+    print("graph stepping", graph, flush=True)
     srv.submit(reqs[0])
     srv.admit(0)
     srv.step()

@@ -31,7 +31,7 @@ class FakeBatch:
     def remove(self, iid):
         self.used -= self.live.pop(iid)[0]
 
-    def step(self, n=1):
+    def step(self, n=1, graph=False):
         for v in self.live.values():
             v[1] += self.d * n
 
