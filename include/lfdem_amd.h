@@ -209,6 +209,20 @@ int lfd_system_compute_velocities(lfd_system *s, int in_predictor);
 /* state access */
 int lfd_system_num_pairs(lfd_system *s);
 int lfd_system_get_pairs(lfd_system *s, int cap, int *p0, int *p1, int *state, double *geom5, double *disp_tan);
+/* Per interaction, in the order of lfd_system_get_pairs, the force columns of
+ * Simulation::outputIntFileTxt (Simulation.cpp:941-1006), LFD_PAIR_OUT values:
+ * normal part of the lubrication force on p0 (positive for compression,
+ * Lubrication.cpp:659-694, zero unless the gap is positive), its tangential
+ * part (3); the contact force's normal part F.n (signed; the file prints its
+ * norm) and tangential part (3), spring + dashpot (Contact.cpp:235-253,
+ * 487-501); the repulsive force norm (RepulsiveForce.h:63-66); the contact
+ * xF's viscosity contribution (rvec (x) F_c):E_inf / rate^2.  Computed on the
+ * device from the current pairs and velocities; returns the pair count. */
+#define LFD_PAIR_OUT 10
+int lfd_system_get_pair_output(lfd_system *s, int cap, double *out);
+/* System::getNonAffineDisp (System.h:529-532): the corrector's non-affine
+ * velocity times dt summed over the steps (System.cpp:980-982), n x 3. */
+int lfd_system_get_na_disp(lfd_system *s, double *out3);
 /* System::getContacts (System.cpp:306-318): p0, p1, disp_tan[3], disp_rolling[3]
  * of every active contact (disp_rolling is zero: rolling friction is not
  * modelled); returns the count (only the count when p0 is NULL). */
@@ -315,6 +329,9 @@ int lfd_batch_query(lfd_batch *b, int id, double *scalars8, lfd_solve_stats *st,
  * returns the count (at most cap). */
 int lfd_batch_status(lfd_batch *b, int cap, int *ids, double *strain, int *status);
 int lfd_batch_get_state(lfd_batch *b, int id, double *pos, double *vel6, double *na6, double *scalars8);
+/* lfd_system_get_pair_output / lfd_system_get_na_disp of instance id */
+int lfd_batch_get_pair_output(lfd_batch *b, int id, int cap, double *out);
+int lfd_batch_get_na_disp(lfd_batch *b, int id, double *out3);
 /* As lfd_system_precond_waves, for instance id of a batch. */
 int lfd_batch_precond_waves(lfd_batch *b, int id, int *waves);
 int lfd_batch_get_pairs(lfd_batch *b, int id, int cap, int *p0, int *p1, int *state, double *geom5,

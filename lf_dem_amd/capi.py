@@ -66,6 +66,8 @@ SIGNATURES = {
     "lfd_system_set_contacts": ([_vp, ctypes.c_int, _ip, _ip, _dp, _dp], ctypes.c_int),
     "lfd_system_set_clock": ([_vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "lfd_system_get_state": ([_vp, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "lfd_system_get_pair_output": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_system_get_na_disp": ([_vp, _dp], ctypes.c_int),
     "lfd_system_get_cells": ([_vp, _ip, _ip], ctypes.c_int),
     "lfd_system_get_blocks": ([_vp, ctypes.c_int, _dp, _dp, _ip, _ip], ctypes.c_int),
     "lfd_system_get_rhs": ([_vp, _dp], ctypes.c_int),
@@ -97,6 +99,8 @@ SIGNATURES = {
     "lfd_batch_query": ([_vp, ctypes.c_int, _dp, _vp, _ip], ctypes.c_int),
     "lfd_batch_status": ([_vp, ctypes.c_int, _ip, _dp, _ip], ctypes.c_int),
     "lfd_batch_get_state": ([_vp, ctypes.c_int, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "lfd_batch_get_pair_output": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+    "lfd_batch_get_na_disp": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "lfd_batch_get_pairs": ([_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
     "lfd_batch_num_pairs": ([_vp, ctypes.c_int], ctypes.c_int),
     "lfd_batch_info": ([_vp, _ip, _ip, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
@@ -384,6 +388,23 @@ class System:
         return dict(p0=p0[:n], p1=p1[:n], state=st[:n], nvec=geom[:n, :3], r=geom[:n, 3],
                     gap=geom[:n, 4], disp_tan=dt[:n])
 
+    # columns of pair_output() (lfd_system_get_pair_output)
+    PAIR_OUT = ("lub_normal", "lub_tan_x", "lub_tan_y", "lub_tan_z", "contact_normal", "contact_tan_x",
+                "contact_tan_y", "contact_tan_z", "repulsion", "contact_xF_viscosity")
+
+    def pair_output(self):
+        """(P, 10) force columns of the interaction snapshot, in pairs() order."""
+        cap = max(self.npairs, 1)
+        out = np.zeros((cap, len(self.PAIR_OUT)))
+        n = _check(lib().lfd_system_get_pair_output(self.h, cap, _d(out)))
+        return out[:n].copy()
+
+    def na_disp(self):
+        """(N, 3) accumulated non-affine displacement (System::getNonAffineDisp)."""
+        out = np.zeros((self.np, 3))
+        _check(lib().lfd_system_get_na_disp(self.h, _d(out)))
+        return out
+
     def contacts(self):
         """Active contacts (System::getContacts): p0, p1, disp_tan (n, 3), disp_rolling (n, 3)."""
         n = _check(lib().lfd_system_get_contacts(self.h, 0, None, None, None, None))
@@ -595,6 +616,19 @@ class Batch:
         sc = np.zeros(8)
         _check(lib().lfd_batch_get_state(self.h, int(iid), _d(pos), _d(vel6), _d(na6), _d(sc)))
         return _state_dict(pos, vel6, na6, sc)
+
+    def pair_output(self, iid):
+        """System.pair_output of instance iid."""
+        cap = max(_check(lib().lfd_batch_num_pairs(self.h, int(iid))), 1)
+        out = np.zeros((cap, len(System.PAIR_OUT)))
+        n = _check(lib().lfd_batch_get_pair_output(self.h, int(iid), cap, _d(out)))
+        return out[:n].copy()
+
+    def na_disp(self, iid):
+        """System.na_disp of instance iid."""
+        out = np.zeros((self.np[iid], 3))
+        _check(lib().lfd_batch_get_na_disp(self.h, int(iid), _d(out)))
+        return out
 
     def pairs(self, iid):
         n = _check(lib().lfd_batch_num_pairs(self.h, int(iid)))

@@ -239,7 +239,13 @@ void launch_adapt_dt(DevScalars* sc, const InstConst* IC, int ninst, int advance
 void launch_boxing(DevScalars* sc, const InstConst* IC, int ninst, hipStream_t s);
 void launch_set_strain_end(DevScalars* sc, int ninst, double strain_end, hipStream_t s);
 void launch_displace(double4* pos, double* vel6, const double* pred6, int n, const int* inst_of,
-                     const InstConst* IC, DevScalars* sc, int* box_of, int mode, hipStream_t s);
+                     const InstConst* IC, DevScalars* sc, int* box_of, int mode, const double* na6, double* na_disp,
+                     hipStream_t s);
+// per-interaction force columns of the interaction snapshot (PAIR_OUT per pair)
+static constexpr int PAIR_OUT = 10;
+void launch_pair_output(const PairRec* pairs, int k0, int k1, const double4* pos, const int* inst_of,
+                        const InstConst* IC, const DevScalars* sc, const double* vel6, const double* na6, double* out,
+                        hipStream_t s);
 void launch_copy_double(double* dst, const double* src, size_t n, hipStream_t s);
 
 // solver

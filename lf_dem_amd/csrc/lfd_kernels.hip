@@ -1234,6 +1234,91 @@ __global__ __launch_bounds__(128) void k_stress_pairs(const PairRec* __restrict_
     }
 }
 
+// Per-interaction force columns of Simulation::outputIntFileTxt
+// (Simulation.cpp:941-1006), PAIR_OUT values per pair of [k0, k1):
+//  [0]   normal part of the lubrication force on p0, positive for compression:
+//        -F.n, F = Lubrication::getTotalForce (Lubrication.cpp:659-694) at the
+//        non-affine velocities plus the R_FE:E term of the pair's right-hand
+//        side; zero unless the gap is positive;
+//  [1-3] its tangential part (Lubrication.h:120-124);
+//  [4]   normal part F_c.n of the contact force (signed) and [5-7] its
+//        tangential part; F_c = spring + dashpot at the total velocities
+//        (Contact.cpp:235-253, 487-501); zero without contact;
+//  [8]   norm of the repulsive force (RepulsiveForce.h:63-66);
+//  [9]   (rvec (x) F_c) : E_inf / rate^2, the viscosity contribution of the
+//        contact xF (Contact.cpp:439-450) at this state.
+__global__ void k_pair_output(const PairRec* __restrict__ pairs, int k0, int k1, const double4* __restrict__ pos,
+    const int* __restrict__ inst_of, const InstConst* IC, const DevScalars* sc, const double* __restrict__ vel6,
+    const double* __restrict__ na6, double* __restrict__ out)
+{
+    const int k = k0+blockIdx.x*blockDim.x+threadIdx.x;
+    if (k >= k1) return;
+    const PairRec q = pairs[k];
+    const int inst = inst_of[q.p0];
+    const Consts& C = IC[inst].C;
+    const Flow F = sc[inst].flow;
+    const double4 x04 = pos[q.p0], x14 = pos[q.p1];
+    const double a0 = x04.w, a1 = x14.w;
+    const double* n = q.n;
+    double* o = out+PAIR_OUT*(size_t)(k-k0);
+    double v[PAIR_OUT];
+    for (int m = 0; m < PAIR_OUT; m++) v[m] = 0;
+    const size_t i0 = 6*(size_t)q.p0, i1 = 6*(size_t)q.p1;
+    if ((q.flags & F_LUB) && q.gap > 0) {
+        LubConst c;
+        lub_constants(a0, a1, c);
+        double lc, llc;
+        lub_coeffs(q.gap, C.lub_reduce, lc, llc);
+        const double* vi = na6+i0;
+        const double* vj = na6+i1;
+        double f[3], t[3];
+        for (int m = 0; m < 3; m++) t[m] = c.XA[0]*lc*vi[m]+c.XA[1]*lc*vj[m];
+        const double d = -dot3(t, n);
+        for (int m = 0; m < 3; m++) f[m] = d*n[m];
+        if (C.tangential) {
+            const double di = dot3(n, vi), dj = dot3(n, vj);
+            double ci[3], cj[3];
+            cross3(n, vi+3, ci);
+            cross3(n, vj+3, cj);
+            for (int m = 0; m < 3; m++) {
+                f[m] += -c.YA[0]*llc*(vi[m]-n[m]*di)-c.YA[1]*llc*(vj[m]-n[m]*dj);
+                f[m] += -c.YB[0]*llc*ci[m]-c.YB[2]*llc*cj[m];
+            }
+        }
+        RhsAcc R{};
+        const double x0[3] = {x04.x, x04.y, x04.z}, x1[3] = {x14.x, x14.y, x14.z};
+        PairRec ql = q;
+        ql.flags &= ~F_CONTACT;           // the hydro term only
+        Consts Cl = C;
+        Cl.repulsive = 0;
+        pair_rhs_side(ql, 0, a0, a1, x0, x1, Cl, F, R);
+        for (int m = 0; m < 3; m++) f[m] += R.Fh[m];
+        const double fn = dot3(f, n);
+        v[0] = -fn;
+        for (int m = 0; m < 3; m++) v[1+m] = f[m]-fn*n[m];
+    }
+    if (q.flags & F_CONTACT) {
+        DashCoef c;
+        dashpot_coeffs(a0, a1, C.normal_coeff, C.tangential_coeff, c);
+        double fd[3], fc[3], rv[3] = {n[0]*q.r, n[1]*q.r, n[2]*q.r}, st[6];
+        dashpot_force(q, c, C, F, vel6+i0, vel6+i1, vel6+i0+3, vel6+i1+3, 0, fd);
+        for (int m = 0; m < 3; m++) fc[m] = q.f_total[m]+fd[m];
+        const double d = dot3(fc, n);
+        v[4] = d;
+        for (int m = 0; m < 3; m++) v[5+m] = fc[m]-d*n[m];
+        outer_sym3(rv, fc, st);
+        if (F.rate != 0) v[9] = ddot6(st, F.E)/(F.rate*F.rate);
+    }
+    if (C.repulsive) v[8] = q.rep_norm;
+    for (int m = 0; m < PAIR_OUT; m++) o[m] = v[m];
+}
+void launch_pair_output(const PairRec* pairs, int k0, int k1, const double4* pos, const int* inst_of,
+                        const InstConst* IC, const DevScalars* sc, const double* vel6, const double* na6, double* out,
+                        hipStream_t s)
+{
+    if (k1 > k0) k_pair_output<<<(k1-k0+127)/128, 128, 0, s>>>(pairs, k0, k1, pos, inst_of, IC, sc, vel6, na6, out);
+}
+
 // Per particle and slot, the sum of its pairs' contributions in ascending
 // partner order; 64 lanes per particle, lane v < 48 owns value v.
 __global__ __launch_bounds__(256) void k_stress_gather(const double* __restrict__ pst, int n,
@@ -1497,8 +1582,11 @@ void launch_boxing(DevScalars* sc, const InstConst* IC, int ninst, hipStream_t s
 
 // System::displacement (System.cpp:2418-2443).  mode 0: predictor move by
 // U dt; mode 1: corrector (velocity average, System.cpp:1164-1187).
+// mode 0: predictor move; mode 1: corrector move, then the corrector's
+// non-affine velocity times dt into na_disp (System.cpp:980-982)
 __global__ void k_displace(double4* pos, double* vel6, const double* pred6, int n, const int* __restrict__ inst_of,
-                           const InstConst* IC, DevScalars* scs, int* box_of, int mode)
+                           const InstConst* IC, DevScalars* scs, int* box_of, int mode, const double* na6,
+                           double* na_disp)
 {
     int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= n) return;
@@ -1516,6 +1604,7 @@ __global__ void k_displace(double4* pos, double* vel6, const double* pred6, int 
         const double* vp = pred6+6*(size_t)i;
         for (int k = 0; k < 6; k++) v[k] = 0.5*(v[k]+vp[k]);
         for (int k = 0; k < 3; k++) dr[k] = (v[k]-vp[k])*dt;
+        for (int k = 0; k < 3; k++) na_disp[3*(size_t)i+k] += na6[6*(size_t)i+k]*dt;
     }
     double4 x4 = pos[i];
     double x[3] = {x4.x+dr[0], x4.y+dr[1], x4.z+dr[2]};
@@ -1531,9 +1620,10 @@ __global__ void k_displace(double4* pos, double* vel6, const double* pred6, int 
     box_of[i] = g.coff+l;
 }
 void launch_displace(double4* pos, double* vel6, const double* pred6, int n, const int* inst_of,
-                     const InstConst* IC, DevScalars* sc, int* box_of, int mode, hipStream_t s)
+                     const InstConst* IC, DevScalars* sc, int* box_of, int mode, const double* na6, double* na_disp,
+                     hipStream_t s)
 {
-    k_displace<<<(n+255)/256, 256, 0, s>>>(pos, vel6, pred6, n, inst_of, IC, sc, box_of, mode);
+    k_displace<<<(n+255)/256, 256, 0, s>>>(pos, vel6, pred6, n, inst_of, IC, sc, box_of, mode, na6, na_disp);
 }
 
 template <int LPR>

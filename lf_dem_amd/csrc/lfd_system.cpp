@@ -444,6 +444,7 @@ struct lfd_batch {
     hipStream_t s = nullptr;
     double4* pos = nullptr;
     double *vel6 = nullptr, *na6 = nullptr, *pred6 = nullptr, *rhs = nullptr;
+    double* na_disp = nullptr;    // [3 per slot] accumulated non-affine displacement
     int *inst_of = nullptr, *box_of = nullptr;
     int *snap_start = nullptr, *snap_mem = nullptr, *home_start = nullptr, *home_mem = nullptr;
     int *bin_count = nullptr, *bin_cursor = nullptr, *found = nullptr, *found_cnt = nullptr;
@@ -476,7 +477,7 @@ struct lfd_system : lfd_batch {};
 lfd_batch::~lfd_batch()
 {
     if (s) (void)hipStreamSynchronize(s);
-    void* ptrs[] = {pos, vel6, na6, pred6, rhs, inst_of, box_of, snap_start, snap_mem, home_start, home_mem,
+    void* ptrs[] = {pos, vel6, na6, pred6, rhs, na_disp, inst_of, box_of, snap_start, snap_mem, home_start, home_mem,
                     bin_count, bin_cursor, found, found_cnt, pairs[0], pairs[1], keep, keep_scan, tcnt, merge_cnt,
                     scan_tmp, probe, pdb, prhs, rhs_parts, vel_parts, pst, ps, st_partial, ca.nb, ca.nnb,
                     ca.nstatic, ca.ctype, ca.cell_inst, d_ic, sc, bs};
@@ -493,7 +494,7 @@ static int round_up(int a, int m) { return (a+m-1)/m*m; }
 static void batch_unit_bytes(size_t* per_slot, size_t* per_cell)
 {
     const size_t pp = 16;     // pair capacity per slot
-    size_t slot = sizeof(double4)+4*6*8+2*4+2*4+(size_t)64*4+4*4    // pos, vel6/na6/pred6/rhs, labels, found
+    size_t slot = sizeof(double4)+4*6*8+3*8+2*4+2*4+(size_t)64*4+4*4  // pos, vel6/na6/pred6/rhs, na_disp, labels, found
                   +8*6*8+STRESS_VALS*8                               // velocity parts, rhs parts, particle stress
                   +(15+36)*(8+4)+6*8*2+6*4*6+6*8*4+3*4;              // solver rows and vectors
     slot += pp*(2*sizeof(PairRec)+2*4+(30+42)*8+2*STRESS_VALS*8+18*(8+4)+3*4);
@@ -521,7 +522,7 @@ static int batch_init(lfd_batch* B, int slot_cap, int cell_cap, int inst_cap, co
     hipError_t e = hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking);
 #define A(x) if (e == hipSuccess) e = (x)
     A(dalloc(&B->pos, M)); A(dalloc(&B->vel6, m6)); A(dalloc(&B->na6, m6)); A(dalloc(&B->pred6, m6));
-    A(dalloc(&B->rhs, m6)); A(dalloc(&B->inst_of, M)); A(dalloc(&B->box_of, M));
+    A(dalloc(&B->rhs, m6)); A(dalloc(&B->na_disp, 3*(size_t)M)); A(dalloc(&B->inst_of, M)); A(dalloc(&B->box_of, M));
     A(dalloc(&B->snap_start, ccap+1)); A(dalloc(&B->snap_mem, M)); A(dalloc(&B->home_start, ccap+1));
     A(dalloc(&B->home_mem, M)); A(dalloc(&B->bin_count, ccap)); A(dalloc(&B->bin_cursor, ccap));
     A(dalloc(&B->found, (size_t)M*B->K)); A(dalloc(&B->found_cnt, M));
@@ -553,6 +554,7 @@ static int batch_init(lfd_batch* B, int slot_cap, int cell_cap, int inst_cap, co
     launch_fill_int((int*)B->sc, 0, B->icap*sizeof(DevScalars)/sizeof(int), s);
     launch_fill_double(B->vel6, 0.0, m6, s);
     launch_fill_double(B->na6, 0.0, m6, s);
+    launch_fill_double(B->na_disp, 0.0, 3*(size_t)M, s);
     launch_fill_double(B->pred6, 0.0, m6, s);
     launch_fill_double(B->vel_parts, 0.0, 4*m6, s);
     HIPCHK(hipStreamSynchronize(s));
@@ -795,11 +797,13 @@ static int bat_step(lfd_batch* B, int calc_stress = 0, int adapt_first = 1)
     launch_adapt_dt(B->sc, B->d_ic, B->ninst, 1, B->s);
     launch_boxing(B->sc, B->d_ic, B->ninst, B->s);
     bat_bin_snapshot(B);
-    launch_displace(B->pos, B->vel6, B->pred6, B->M, B->inst_of, B->d_ic, B->sc, B->box_of, 0, B->s);
+    launch_displace(B->pos, B->vel6, B->pred6, B->M, B->inst_of, B->d_ic, B->sc, B->box_of, 0, B->na6, B->na_disp,
+                    B->s);
     bat_update(B, 1, -1.0);
     launch_copy_double(B->pred6, B->vel6, 6*(size_t)B->M, B->s);
     if ((rc = bat_velocities(B, 0, calc_stress))) return rc;
-    launch_displace(B->pos, B->vel6, B->pred6, B->M, B->inst_of, B->d_ic, B->sc, B->box_of, 1, B->s);
+    launch_displace(B->pos, B->vel6, B->pred6, B->M, B->inst_of, B->d_ic, B->sc, B->box_of, 1, B->na6, B->na_disp,
+                    B->s);
     bat_check_new(B);
     bat_update(B, 0, -1.0);
     return LFD_OK;
@@ -937,6 +941,7 @@ static int bat_add(lfd_batch* B, int np, const double* pos, const double* radius
     launch_fill_int(B->inst_of+poff, id, np, s);
     launch_fill_double(B->vel6+o6, 0.0, s6, s);
     launch_fill_double(B->na6+o6, 0.0, s6, s);
+    launch_fill_double(B->na_disp+3*(size_t)poff, 0.0, 3*(size_t)np, s);
     launch_fill_double(B->pred6+o6, 0.0, s6, s);
     for (int q = 0; q < 4; q++) launch_fill_double(B->vel_parts+q*6*(size_t)B->M+o6, 0.0, s6, s);
     InstHost& h = B->inst[id];
@@ -1109,6 +1114,39 @@ static void fill_scalars8(const DevScalars& h, double* sc8)
     sc8[3] = h.dt; sc8[4] = h.time; sc8[5] = h.strain;
     sc8[6] = std::sqrt(__builtin_bit_cast(double, h.max_vel_bits));
     sc8[7] = std::sqrt(__builtin_bit_cast(double, h.max_slide_bits));
+}
+
+// System::getNonAffineDisp (System.h:529-532): n x 3
+static int inst_get_na_disp(lfd_batch* B, int id, double* out)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h || !out) return fail(LFD_ERR_ARG, "get_na_disp: bad argument");
+    int rc = batch_check(B, id);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(out, B->na_disp+3*(size_t)h->ic.poff, 3*(size_t)h->ic.n*sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return LFD_OK;
+}
+
+// The force columns of Simulation::outputIntFileTxt per interaction, in the
+// order of get_pairs, computed on the device from the current pairs and
+// velocities (k_pair_output); the per-pair stress buffer is the scratch.
+static int inst_get_pair_output(lfd_batch* B, int id, int cap, double* out)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h) return fail(LFD_ERR_ARG, "get_pair_output: bad instance");
+    int rc = batch_check(B, id);
+    if (rc) return rc;
+    int k0 = 0, k1 = 0;
+    if ((rc = pair_range(B, *h, k0, k1))) return rc;
+    const int np_ = k1-k0;
+    if (np_ > cap || (np_ > 0 && !out)) return fail(LFD_ERR_ARG, "get_pair_output: cap too small");
+    if ((size_t)np_*PAIR_OUT > 2*(size_t)STRESS_VALS*B->pcap) return fail(LFD_ERR_CAPACITY, "get_pair_output");
+    launch_pair_output(B->pairs[B->cur], k0, k1, B->pos, B->inst_of, B->d_ic, B->sc, B->vel6, B->na6, B->pst, B->s);
+    HIPCHK(hipGetLastError());
+    if (np_) HIPCHK(hipMemcpyAsync(out, B->pst, (size_t)np_*PAIR_OUT*sizeof(double), hipMemcpyDeviceToHost, B->s));
+    HIPCHK(hipStreamSynchronize(B->s));
+    return np_;
 }
 
 static int inst_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* sc8)
@@ -1399,6 +1437,18 @@ extern "C" int lfd_system_set_clock(lfd_system* y, double time, double strain)
     return inst_set_scalars(y, 0, nullptr, nullptr, nullptr, nullptr, &time, &strain);
 }
 
+extern "C" int lfd_system_get_na_disp(lfd_system* y, double* out3)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_get_na_disp(y, 0, out3);
+}
+
+extern "C" int lfd_system_get_pair_output(lfd_system* y, int cap, double* out10)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_get_pair_output(y, 0, cap, out10);
+}
+
 extern "C" int lfd_system_get_state(lfd_system* y, double* pos, double* vel6, double* na6, double* sc8)
 {
     if (!y) return fail(LFD_ERR_ARG, "null handle");
@@ -1636,6 +1686,18 @@ extern "C" int lfd_batch_precond_waves(lfd_batch* B, int id, int* waves)
 extern "C" int lfd_batch_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* scalars8)
 {
     return inst_get_state(B, id, pos, vel6, na6, scalars8);
+}
+
+extern "C" int lfd_batch_get_na_disp(lfd_batch* B, int id, double* out3)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    return inst_get_na_disp(B, id, out3);
+}
+
+extern "C" int lfd_batch_get_pair_output(lfd_batch* B, int id, int cap, double* out10)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    return inst_get_pair_output(B, id, cap, out10);
 }
 
 extern "C" int lfd_batch_get_pairs(lfd_batch* B, int id, int cap, int* p0, int* p1, int* state, double* geom5,

@@ -1,6 +1,9 @@
 """LF_DEM's data_ and st_ output files (Simulation::outputData,
 Simulation.cpp:641-768; OutputData.h:27-207; header of
-Simulation::createDataHeader, Simulation.cpp:812-824).
+Simulation::createDataHeader, Simulation.cpp:812-824) and the par_ / int_
+snapshot files (Simulation::outputParFileTxt / outputIntFileTxt,
+Simulation.cpp:854-1006; snapshot header of getSnapshotHeader,
+Simulation.cpp:770-794).
 
 A row is formed from plain values: the stress dict of System.stress() (group
 and total stress tensors in the internal unit, shear rate, viscosity), the
@@ -115,6 +118,91 @@ def stress_columns(stress, state):
     return cols
 
 
+def vel_difference(shear_rate, Lz, theta_shear=0.0):
+    """System::setVelocityDifference (System.cpp:1967-1970): 2 E_inf . (0, 0, Lz)."""
+    return shear_rate * Lz * np.array([math.cos(theta_shear), math.sin(theta_shear), 0.0])
+
+
+def shift_up(pos, L, shear_disp):
+    """Positions as the snapshot files print them with origin_zero_flow
+    (Simulation.cpp:866-872, shiftUpCoordinate :795-810): the origin at the
+    box centre, the particles of the upper half moved one box down through
+    the Lees-Edwards face (x and y shifted back by the shear displacement,
+    wrapped once)."""
+    L = np.asarray(L, dtype=float)
+    x = np.array(pos, dtype=float) - 0.5 * L
+    z = x[:, 2] + 0.5 * L[2]
+    up = z > 0.5 * L[2]
+    x[up, 0] -= shear_disp[0]
+    x[up, 1] -= shear_disp[1]
+    x[up & (x[:, 0] < -0.5 * L[0]), 0] += L[0]
+    x[up & (x[:, 1] < -0.5 * L[1]), 1] += L[1]
+    z[up] -= L[2]
+    x[:, 2] = z
+    return x
+
+
+def particle_rows(state, radius, L, na_disp, shear_rate, theta_shear=0.0, origin_zero_flow=True):
+    """The rows of one par_ snapshot (Simulation::outputParFileTxt,
+    Simulation.cpp:854-939; simple shear, no optional columns): index,
+    radius, position, velocity (minus the Lees-Edwards velocity difference
+    below the shifted origin), angular velocity, non-affine displacement.
+    state: System.state(), na_disp: System.na_disp(), shear_rate: the
+    current one (System.stress()["shear_rate"] or the imposed rate)."""
+    pos = np.asarray(state["pos"], dtype=float)
+    vel = np.array(state["vel"], dtype=float)
+    if origin_zero_flow:
+        pos = shift_up(pos, L, state["shear_disp"])
+        vel[pos[:, 2] < 0] -= vel_difference(shear_rate, L[2], theta_shear)
+    ang = np.asarray(state["ang_vel"], dtype=float)
+    rows = []
+    for i in range(len(radius)):
+        rows.append([("particle index", "none", 1, i),
+                     ("radius", "none", 1, radius[i]),
+                     ("position (x, y, z)", "none", 3, tuple(pos[i])),
+                     ("velocity (x, y, z)", "velocity", 3, tuple(vel[i])),
+                     ("angular velocity (x, y, z)", "none", 3, tuple(ang[i])),
+                     ("non affine displacement (x, y, z)", "none", 3, tuple(na_disp[i]))])
+    return rows
+
+
+CONTACT_STATE = ("contact state (0 = no contact, 1 = frictionless contact, 2 = non-sliding frictional, "
+                 "3 = sliding frictional)")
+
+
+def interaction_rows(pairs, pair_output):
+    """The rows of one int_ snapshot (Simulation::outputIntFileTxt,
+    Simulation.cpp:941-1006, with lubrication): pairs from System.pairs(),
+    pair_output from System.pair_output() (same order)."""
+    rows = []
+    o = np.asarray(pair_output, dtype=float)
+    for k in range(len(pairs["p0"])):
+        rows.append([("particle 1 label", "none", 1, int(pairs["p0"][k])),
+                     ("particle 2 label", "none", 1, int(pairs["p1"][k])),
+                     (CONTACT_STATE, "none", 1, int(pairs["state"][k])),
+                     ("normal vector, oriented from particle 1 to particle 2", "none", 3, tuple(pairs["nvec"][k])),
+                     ("dimensionless gap = s-2, s = 2r/(a1+a2)", "none", 1, pairs["gap"][k]),
+                     ("normal part of the lubrication force (positive for compression)", "force", 1, o[k, 0]),
+                     ("tangential part of the lubrication force", "force", 3, tuple(o[k, 1:4])),
+                     ("norm of the normal part of the contact force", "force", 1, abs(o[k, 4])),
+                     ("tangential part of the contact force", "force", 3, tuple(o[k, 5:8])),
+                     ("norm of the normal repulsive force", "force", 1, o[k, 8]),
+                     ("Viscosity contribution of contact xF", "stress", 1, o[k, 9])])
+    return rows
+
+
+def snapshot_header(strain, shear_disp_x, rate, target_stress=None):
+    """Simulation::getSnapshotHeader (Simulation.cpp:770-794), simple shear:
+    rate is Simulation::getRate (the input rate under rate control, the
+    current shear rate under stress control); target_stress the input value
+    under stress control."""
+    s = "# cumulated strain : %s\n# shear disp : %s\n# shear rate : %s\n" % (fmt(strain), fmt(shear_disp_x),
+                                                                           fmt(rate))
+    if target_stress is not None:
+        s += "# target stress : %s\n" % fmt(target_stress)
+    return s
+
+
 def fmt(x):
     """A double as std::ostream prints it with setprecision(6)."""
     x = float(x)
@@ -134,7 +222,8 @@ def data_header(np_, vf, L, flow_type="shear", version="lf_dem_amd"):
 class OutputFile:
     """OutputData: the file header once, the column header on the first
     row (`# data in <unit> units.`, then `#k: name` or `#k-m: name`), then
-    one line per row."""
+    one line per row; a snapshot file puts a snapshot header before each
+    snapshot's rows."""
 
     def __init__(self, path, header, unit, ratio=1.0):
         self.f = open(path, "w")
@@ -143,21 +232,41 @@ class OutputFile:
         self.ratio = ratio          # internal force unit / output force unit
         self.first = True
 
-    def write(self, cols):
-        if self.first:
-            self.f.write("# data in %s units.\n" % self.unit)
-            i = 1
-            for name, _, width, _ in cols:
-                self.f.write(("#%d: %s\n" % (i, name)) if width == 1 else ("#%d-%d: %s\n" % (i, i + width - 1, name)))
-                i += width
-            self.f.write("\n")
-            self.first = False
+    def _column_header(self, cols):
+        self.f.write("# data in %s units.\n" % self.unit)
+        i = 1
+        for name, _, width, _ in cols:
+            self.f.write(("#%d: %s\n" % (i, name)) if width == 1 else ("#%d-%d: %s\n" % (i, i + width - 1, name)))
+            i += width
+        self.f.write("\n")
+        self.first = False
+
+    def _line(self, cols):
         out = []
         for _, dim, width, v in cols:
             c = _CONVERT[dim](self.ratio)
             vals = [v] if width == 1 else list(v)
             out += [fmt(c * x) for x in vals]
-        self.f.write(" ".join(out) + " \n")
+        return " ".join(out) + " \n"
+
+    def write(self, cols):
+        if self.first:
+            self._column_header(cols)
+        self.f.write(self._line(cols))
+        self.f.flush()
+
+    def write_snapshot(self, header, rows):
+        """OutputData::writeToFile(header) of a snapshot file: the column
+        header before the first snapshot, then the snapshot header and one
+        line per row.  Nothing is written for an empty snapshot (the
+        interaction file without interactions, Simulation.cpp:1002-1004)."""
+        if not rows:
+            return
+        if self.first:
+            self._column_header(rows[0])
+        self.f.write(header)
+        for cols in rows:
+            self.f.write(self._line(cols))
         self.f.flush()
 
     def close(self):

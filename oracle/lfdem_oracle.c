@@ -586,6 +586,7 @@ struct orc_system {
 	double L[3];
 	double *pos, *radius;
 	double *vel, *ang, *na_vel, *na_ang, *vel_pred, *ang_pred, *u_inf;
+	double *na_disp;                /* System::na_disp (System.h:89) */
 	double shear_disp[3], vel_difference[3];
 	double E[6], omega[3];
 	double dt, time, strain;
@@ -1675,6 +1676,7 @@ orc_system *orc_create(int np, const double *pos, const double *radius, const do
 	s->vel_pred = calloc(3*(size_t)np, sizeof(double));
 	s->ang_pred = calloc(3*(size_t)np, sizeof(double));
 	s->u_inf = calloc(3*(size_t)np, sizeof(double));
+	s->na_disp = calloc(3*(size_t)np, sizeof(double));
 	s->box_of = calloc(np, sizeof(int));
 	s->snap_mem = calloc(np, sizeof(int));
 	s->friction = p->friction_model != 0;
@@ -1694,7 +1696,7 @@ void orc_destroy(orc_system *s)
 {
 	if (!s) return;
 	free(s->pos); free(s->radius); free(s->vel); free(s->ang); free(s->na_vel); free(s->na_ang);
-	free(s->vel_pred); free(s->ang_pred); free(s->u_inf); free(s->box_of); free(s->snap_mem);
+	free(s->vel_pred); free(s->ang_pred); free(s->u_inf); free(s->na_disp); free(s->box_of); free(s->snap_mem);
 	free(s->ctype); free(s->static_nb); free(s->n_static); free(s->moving_nb); free(s->n_moving);
 	free(s->snap_start); free(s->pairs); free(s->dblocks); free(s->odblocks); free(s->od_cp);
 	free(s->od_rows);
@@ -2279,6 +2281,8 @@ int orc_step_ex(orc_system *s, double strain_end, int calc_stress, int adapt_fir
 	g_phase[ORC_PH_MOVE] += ph_now()-t0;
 	PH(ORC_PH_NEIGHBOR, orc_check_new_interaction(s));
 	PH(ORC_PH_PAIRS, orc_update_interactions(s));
+	/* the corrector's non-affine velocity times dt (System.cpp:980-982) */
+	for (i = 0; i < 3*s->np; i++) s->na_disp[i] += s->na_vel[i]*s->dt;
 	return st;
 }
 
@@ -2307,6 +2311,88 @@ int orc_set_in_predictor(orc_system *s, int flag) { s->in_predictor = flag; retu
 int orc_set_dt(orc_system *s, double dt) { s->dt = dt; return 0; }
 
 int orc_num_pairs(const orc_system *s) { return s->npairs; }
+
+void orc_get_na_disp(const orc_system *s, double *out)
+{
+	memcpy(out, s->na_disp, 3*(size_t)s->np*sizeof(double));
+}
+
+/* Per interaction, in pair order, the force columns of
+ * Simulation::outputIntFileTxt (Simulation.cpp:941-1006), ORC_PAIR_OUT values:
+ *  [0]   normal part of the lubrication force on p0, positive for compression:
+ *        -F.n with F = Lubrication::getTotalForce (Lubrication.cpp:659-694) at
+ *        the non-affine velocities, zero unless the gap is positive;
+ *  [1-3] its tangential part F - (F.n) n (Lubrication.h:120-124);
+ *  [4]   normal part of the contact force F_c.n (signed; the file prints its
+ *        norm) and [5-7] its tangential part, F_c = spring + dashpot at the
+ *        total velocities (Contact.cpp:235-253, 487-501), zero without contact;
+ *  [8]   norm of the repulsive force (RepulsiveForce.h:63-66);
+ *  [9]   viscosity contribution of the contact xF,
+ *        (rvec (x) F_c) : E_inf / rate^2 (Contact.cpp:439-450 at this state). */
+int orc_pair_output(const orc_system *s, double *out)
+{
+	int k, m;
+	const double rate = s->p.shear_rate;
+	for (k = 0; k < s->npairs; k++) {
+		const orc_pair *q = &s->pairs[k];
+		const double *n = q->nvec;
+		double *o = out+ORC_PAIR_OUT*(size_t)k;
+		for (m = 0; m < ORC_PAIR_OUT; m++) o[m] = 0;
+		if (q->lub_active && q->gap > 0) {
+			const double *vi = s->na_vel+3*q->p0, *vj = s->na_vel+3*q->p1;
+			double f[3], t[3], d;
+			for (m = 0; m < 3; m++) t[m] = q->XA[0]*vi[m]+q->XA[1]*vj[m];
+			d = -dot3(t, n);
+			for (m = 0; m < 3; m++) f[m] = d*n[m];
+			if (s->p.lubrication_model == 2) {
+				const double *oi = s->na_ang+3*q->p0, *oj = s->na_ang+3*q->p1;
+				double di = dot3(n, vi), dj = dot3(n, vj), ci[3], cj[3];
+				cross3(n, oi, ci);
+				cross3(n, oj, cj);
+				for (m = 0; m < 3; m++) {
+					f[m] += -q->YA[0]*(vi[m]-n[m]*di)-q->YA[1]*(vj[m]-n[m]*dj);
+					f[m] += -q->YB[0]*ci[m]-q->YB[2]*cj[m];
+				}
+			}
+			if (rate != 0) {
+				double En[3], nnE;
+				symdot(s->E, n, En);
+				nnE = dot3(n, En);
+				if (s->p.lubrication_model == 2) {
+					double yg = q->YG[0]+q->YG[2];
+					double c = (q->XG[0]+q->XG[2]-2*yg)*nnE;
+					for (m = 0; m < 3; m++) f[m] += c*n[m]+2*yg*En[m];
+				} else {
+					double c = (q->XG[0]+q->XG[2])*nnE;
+					for (m = 0; m < 3; m++) f[m] += c*n[m];
+				}
+			}
+			d = dot3(f, n);
+			o[0] = -d;
+			for (m = 0; m < 3; m++) o[1+m] = f[m]-d*n[m];
+		}
+		if (q->contact_active) {
+			double fd[3], fc[3], d, st[6];
+			dashpot_force(s, q, s->vel+3*q->p0, s->vel+3*q->p1, s->ang+3*q->p0, s->ang+3*q->p1, 0, fd);
+			for (m = 0; m < 3; m++) fc[m] = q->f_total[m]+fd[m];
+			d = dot3(fc, n);
+			o[4] = d;
+			for (m = 0; m < 3; m++) o[5+m] = fc[m]-d*n[m];
+			/* Sym2Tensor outer_sym (xx, xy, xz, yz, yy, zz) and its double contraction */
+			st[0] = q->rvec[0]*fc[0];
+			st[1] = 0.5*(q->rvec[0]*fc[1]+q->rvec[1]*fc[0]);
+			st[2] = 0.5*(q->rvec[0]*fc[2]+q->rvec[2]*fc[0]);
+			st[3] = 0.5*(q->rvec[1]*fc[2]+q->rvec[2]*fc[1]);
+			st[4] = q->rvec[1]*fc[1];
+			st[5] = q->rvec[2]*fc[2];
+			if (rate != 0)
+				o[9] = (st[0]*s->E[0]+2*st[1]*s->E[1]+2*st[2]*s->E[2]+2*st[3]*s->E[3]+st[4]*s->E[4]
+				        +st[5]*s->E[5])/(rate*rate);
+		}
+		o[8] = q->rep_norm;
+	}
+	return s->npairs;
+}
 
 int orc_get_pairs(const orc_system *s, int *p0, int *p1, int *cs, double *geom, double *dtan)
 {

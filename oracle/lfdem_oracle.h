@@ -122,6 +122,13 @@ int orc_set_dt(orc_system *s, double dt);
 
 /* state access */
 int orc_num_pairs(const orc_system *s);
+/* System::getNonAffineDisp (System.h:529-532): N x 3 */
+void orc_get_na_disp(const orc_system *s, double *out);
+/* Per interaction (pair order) the force columns of Simulation::outputIntFileTxt:
+ * lubrication normal part, lubrication tangential (3), contact normal norm,
+ * contact tangential (3), repulsive norm, contact xF viscosity contribution */
+#define ORC_PAIR_OUT 10
+int orc_pair_output(const orc_system *s, double *out);
 int orc_get_pairs(const orc_system *s, int *p0, int *p1, int *contact_state,
                   double *geom /* P x 5: nx ny nz r gap */, double *disp_tan /* P x 3 */);
 int orc_set_pair_history(orc_system *s, int n, const int *p0, const int *p1,

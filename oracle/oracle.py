@@ -92,6 +92,9 @@ def lib():
         L.orc_gu_stress.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         L.orc_step_ex.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.orc_time_evolution.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_get_na_disp.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_pair_output.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_pair_output.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -343,6 +346,22 @@ class System:
                             cs.ctypes.data_as(_ip), geom.ctypes.data_as(_dp), dt.ctypes.data_as(_dp))
         return dict(p0=p0, p1=p1, state=cs, nvec=geom[:, :3], r=geom[:, 3], gap=geom[:, 4],
                     disp_tan=dt)
+
+    # columns of pair_output() (orc_pair_output)
+    PAIR_OUT = ("lub_normal", "lub_tan_x", "lub_tan_y", "lub_tan_z", "contact_normal", "contact_tan_x",
+                "contact_tan_y", "contact_tan_z", "repulsion", "contact_xF_viscosity")
+
+    def pair_output(self):
+        """(P, 10) per-interaction force columns of outputIntFileTxt, pair order."""
+        out = np.zeros((max(self.npairs, 1), len(self.PAIR_OUT)))
+        n = lib().orc_pair_output(self.h, out.ctypes.data_as(_dp))
+        return out[:n].copy()
+
+    def na_disp(self):
+        """(N, 3) accumulated non-affine displacement (System::getNonAffineDisp)."""
+        out = np.zeros((self.np, 3))
+        lib().orc_get_na_disp(self.h, out.ctypes.data_as(_dp))
+        return out
 
     def state(self):
         n = self.np
