@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "liblfdem_amd.so")
-SOURCES = ["lfd_kernels.hip", "lfd_agg.hip", "lfd_noalloc.hip", "lfd_system.cpp"]
+SOURCES = ["lfd_kernels.hip", "lfd_agg.hip", "lfd_agg_large.hip", "lfd_noalloc.hip", "lfd_system.cpp"]
 ARCH = "gfx950"
 
 

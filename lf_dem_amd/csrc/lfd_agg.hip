@@ -83,9 +83,28 @@ __global__ void k_agg_weight(BsrView A, const int* __restrict__ inst_of, const D
     wq[k] = agg_weight_q(A, k);
 }
 
-__global__ void k_agg_adjacency(BsrView A, const unsigned* __restrict__ wq, const int* __restrict__ inst_of,
+// Positions of the particles of chunked instances that are due: their places
+// in the cell bins (cells in label order, members in ascending particle
+// order), counted from the instance's particle offset.
+__global__ void k_agg_order(const int* __restrict__ home_start, const int* __restrict__ home_mem, int ncell,
+                            const int* __restrict__ inst_of, const InstConst* __restrict__ IC,
+                            const DevScalars* __restrict__ scs, int interval, AggView G)
+{
+    const int q = blockIdx.x*blockDim.x+threadIdx.x;
+    if (q >= home_start[ncell]) return;
+    const int i = home_mem[q];
+    const int inst = inst_of[i];
+    if (inst < 0 || G.chunk[inst] <= 0 || !agg_due(scs+inst, interval)) return;
+    const int p = IC[inst].poff+(q-home_start[IC[inst].coff]);
+    G.order[p] = i;
+    G.rank[i] = p;
+}
+
+// Adjacency rows are indexed by particle, or by position for a chunked
+// instance, whose couplings to other chunks are left out.
+__global__ void k_agg_adjacency(BsrView A, AggView G, const int* __restrict__ inst_of,
                                 const InstConst* __restrict__ IC, const DevScalars* __restrict__ scs, int interval,
-                                unsigned* __restrict__ adj, int nslots)
+                                int nslots)
 {
     const int i = blockIdx.x*blockDim.x+threadIdx.x;
     if (i >= nslots) return;
@@ -96,12 +115,24 @@ __global__ void k_agg_adjacency(BsrView A, const unsigned* __restrict__ wq, cons
         poff = IC[inst].poff;
     }
     if (!agg_due(scs+inst, interval)) return;
+    const int cs = inst_of ? G.chunk[inst] : 0;
+    int row = i, cbase = poff, cidx = 0;
+    if (cs > 0) {
+        row = G.rank[i];
+        cidx = (row-poff)/cs;
+        cbase = poff+cidx*cs;
+    }
     unsigned top[AG_K];
 #pragma unroll
     for (int j = 0; j < AG_K; j++) top[j] = 0;
     auto offer = [&](unsigned w, int partner) {          // insertion into the descending list
         if (!w) return;
-        unsigned a = w << 14 | (unsigned)(partner-poff);
+        int pl = partner;
+        if (cs > 0) {
+            pl = G.rank[partner];
+            if ((pl-poff)/cs != cidx) return;
+        }
+        unsigned a = w << 14 | (unsigned)(pl-cbase);
 #pragma unroll
         for (int j = 0; j < AG_K; j++) {
             const unsigned t = top[j];
@@ -111,12 +142,19 @@ __global__ void k_agg_adjacency(BsrView A, const unsigned* __restrict__ wq, cons
     };
     for (int e = A.tptr[i]; e < A.tptr[i+1]; e++) {
         const int k = A.tidx[e];
-        offer(wq[k], A.p0[k]);
+        offer(G.wq[k], A.p0[k]);
     }
-    for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) offer(wq[k], A.p1[k]);
-    uint4* out = reinterpret_cast<uint4*>(adj+(size_t)AG_K*i);
+    for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) offer(G.wq[k], A.p1[k]);
+    uint4* out = reinterpret_cast<uint4*>(G.adj+(size_t)AG_K*row);
 #pragma unroll
     for (int j = 0; j < AG_K/4; j++) out[j] = make_uint4(top[4*j], top[4*j+1], top[4*j+2], top[4*j+3]);
+}
+
+// The chunked instances that were due are grouped now.
+__global__ void k_agg_done(DevScalars* sc, const int* __restrict__ chunk, int ninst, int interval)
+{
+    const int k = blockIdx.x*blockDim.x+threadIdx.x;
+    if (k < ninst && chunk[k] > 0 && agg_due(sc+k, interval)) sc[k].group_age = 0;
 }
 
 // Inclusive scan over the workgroup of two 64-bit words (four 16-bit counters each).
@@ -141,8 +179,52 @@ __device__ inline int cls_count(unsigned long long lo, unsigned long long hi, in
     return s <= 4 ? (int)((lo >> (16*(s-1))) & 0xFFFF) : (int)((hi >> (16*(s-5))) & 0xFFFF);
 }
 
+// Best-fit decreasing on the counts cnt[1..8] of groups of each size
+// (consumed): open a wave with the largest remaining group and fill it with
+// the largest groups that fit; runs of identical waves are recorded once in
+// the run table.  Returns the number of waves, or -1 when the run table is
+// full.
+__device__ inline int bfd_counts(int* cnt, unsigned* run_comp, int* run_rep, int* run_bin0, int& nr_out)
+{
+    int nr = 0, bins = 0;
+    for (int s = 8; s >= 1; s--) {
+        while (cnt[s] > 0) {
+            int take[9], before[9];
+            for (int t = 0; t <= 8; t++) { take[t] = 0; before[t] = cnt[t]; }
+            take[s] = 1;
+            cnt[s]--;
+            int rem = 8-s, limited = 0;
+            for (int t = min(rem, s); t >= 1; t--) {
+                if (rem < t) continue;
+                const int want = rem/t, k = min(cnt[t], want);
+                if (k < want && k > 0) limited = 1;
+                take[t] += k;
+                cnt[t] -= k;
+                rem -= k*t;
+            }
+            int rep = limited ? 1 : 1 << 30;
+            if (!limited)
+                for (int t = 1; t <= 8; t++) if (take[t]) rep = min(rep, before[t]/take[t]);
+            unsigned comp = 0;
+            for (int t = 1; t <= 8; t++) {
+                cnt[t] = before[t]-rep*take[t];
+                comp |= (unsigned)take[t] << (4*(t-1));
+            }
+            if (nr == AG_RMAX) { nr_out = nr; return -1; }
+            run_comp[nr] = comp;
+            run_rep[nr] = rep;
+            run_bin0[nr] = bins;
+            nr++;
+            bins += rep;
+        }
+    }
+    nr_out = nr;
+    return bins;
+}
+
 // ------------------------------------------------------------------ grouping and packing
-// One workgroup per instance.  Local indices l = particle - poff.
+// One workgroup per instance, or per spatial chunk of a large instance.
+// Local indices l = particle - poff, or position - poff for a chunk.
 __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G, const AggJob* __restrict__ jobs,
                                                           DevScalars* scs, int interval)
 {
@@ -154,10 +236,11 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
     __shared__ short nxt[AGG_NMAX];            // member lists
     __shared__ short gbin[AGG_NMAX];           // wave of a group (at the leader)
     __shared__ unsigned char goff[AGG_NMAX];   // first position of the group in its wave
-    __shared__ unsigned long long sc_lds[2*AGB_THREADS/WAVE];
+    __shared__ unsigned long long sc_lds[2*AGB_THREADS/WAVE+2];
     __shared__ unsigned run_comp[AG_RMAX];     // per run: count of each size class in one wave (nibbles)
     __shared__ int run_rep[AG_RMAX], run_bin0[AG_RMAX];
-    __shared__ int s_nruns, s_bins, s_fallback;
+    __shared__ int s_nruns, s_bins, s_fallback, s_ndet;
+    __shared__ int s_split[9];
     const AggJob J = jobs[blockIdx.x];
     if (!agg_due(scs+J.inst, interval)) return;
     const int n = J.n, poff = J.poff;
@@ -229,62 +312,89 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
             __syncthreads();
         }
     }
-    // ---- counts and ranks of the groups by size, in leader order
-    unsigned long long lo = 0, hi = 0;
-    for (int l = c0; l < c1; l++)
-        if (grp[l] == l) {
-            const int s = gsz[l];
-            if (s <= 4) lo += 1ull << (16*(s-1));
-            else hi += 1ull << (16*(s-5));
-        }
-    unsigned long long plo = lo, phi = hi;
-    block_scan2(plo, phi, sc_lds);
-    if (tid == T-1) { sc_lds[0] = plo; sc_lds[1] = phi; }
-    plo -= lo; phi -= hi;                      // exclusive prefix of this chunk
+    // member lists (list heads reuse `best`)
+    int* head = reinterpret_cast<int*>(best);
+    for (int l = tid; l < n; l += T) head[l] = -1;
     __syncthreads();
+    for (int l = tid; l < n; l += T) nxt[l] = (short)atomicExch(&head[grp[l]], l);
+    __syncthreads();
+    // ---- counts and ranks of the groups by size, in leader order
+    unsigned long long lo = 0, hi = 0, plo = 0, phi = 0;
+    auto count_groups = [&]() {
+        lo = hi = 0;
+        for (int l = c0; l < c1; l++)
+            if (grp[l] == l) {
+                const int s = gsz[l];
+                if (s <= 4) lo += 1ull << (16*(s-1));
+                else hi += 1ull << (16*(s-5));
+            }
+        plo = lo; phi = hi;
+        block_scan2(plo, phi, sc_lds);
+        if (tid == T-1) { sc_lds[2*AGB_THREADS/WAVE] = plo; sc_lds[2*AGB_THREADS/WAVE+1] = phi; }
+        plo -= lo; phi -= hi;                  // exclusive prefix of this chunk
+        __syncthreads();
+    };
+    count_groups();
     if (tid == 0) {
-        // best-fit decreasing on the counts: open a wave with the largest
-        // remaining group, fill it with the largest groups that fit; runs of
-        // identical waves are recorded once
-        int cnt[9];
-        for (int s = 1; s <= 8; s++) cnt[s] = cls_count(sc_lds[0], sc_lds[1], s);
-        int nr = 0, bins = 0, ok = 1;
-        for (int s = 8; s >= 1 && ok; s--) {
-            while (cnt[s] > 0) {
-                int take[9], before[9];
-                for (int t = 0; t <= 8; t++) { take[t] = 0; before[t] = cnt[t]; }
-                take[s] = 1;
-                cnt[s]--;
-                int rem = 8-s, limited = 0;
-                for (int t = min(rem, s); t >= 1; t--) {
-                    if (rem < t) continue;
-                    const int want = rem/t, k = min(cnt[t], want);
-                    if (k < want && k > 0) limited = 1;
-                    take[t] += k;
-                    cnt[t] -= k;
-                    rem -= k*t;
-                }
-                int rep = limited ? 1 : 1 << 30;
-                if (!limited)
-                    for (int t = 1; t <= 8; t++) if (take[t]) rep = min(rep, before[t]/take[t]);
-                unsigned comp = 0;
-                for (int t = 1; t <= 8; t++) {
-                    cnt[t] = before[t]-rep*take[t];
-                    comp |= (unsigned)take[t] << (4*(t-1));
-                }
-                if (nr == AG_RMAX) { ok = 0; break; }
-                run_comp[nr] = comp;
-                run_rep[nr] = rep;
-                run_bin0[nr] = bins;
-                nr++;
-                bins += rep;
+        int cnt0[9], cnt[9];
+        for (int s = 1; s <= 8; s++) cnt[s] = cnt0[s] = cls_count(sc_lds[2*AGB_THREADS/WAVE], sc_lds[2*AGB_THREADS/WAVE+1], s);
+        int nr = 0;
+        int bins = bfd_counts(cnt, run_comp, run_rep, run_bin0, nr);
+        // A chunk of a large instance must fit its budget: members of the
+        // largest groups below 8 are set apart as single particles, which
+        // fill any wave, until best-fit decreasing fits (decided on the
+        // counts, then done by the leaders)
+        int ndet = 0;
+        for (int t = 0; t <= 8; t++) s_split[t] = 0;
+        if (J.chunked) {
+            int sim[9], avail[9];
+            for (int t = 0; t <= 8; t++) { sim[t] = cnt0[t]; avail[t] = cnt0[t]; }
+            for (int guard = 0; guard < 64 && (bins < 0 || bins > J.budget); guard++) {
+                int t = 7;
+                while (t >= 2 && avail[t] == 0) t--;
+                if (t < 2) break;
+                const int k = min(avail[t], max(4, 2*(bins < 0 ? 8 : bins-J.budget)));
+                avail[t] -= k; sim[t] -= k; sim[t-1] += k; sim[1] += k;
+                s_split[t] += k;
+                ndet += k;
+                for (int q = 0; q <= 8; q++) cnt[q] = sim[q];
+                bins = bfd_counts(cnt, run_comp, run_rep, run_bin0, nr);
             }
         }
+        s_ndet = ndet;
         s_nruns = nr;
         s_bins = bins;
-        s_fallback = !ok || bins > J.budget;
+        s_fallback = bins < 0 || bins > J.budget;
     }
     __syncthreads();
+    if (s_ndet > 0 && !s_fallback) {
+        // the first s_split[s] groups of size s (leader order) give up one
+        // member: decided on the grouping as it is, then done
+        int det[AG_CH];
+        int nd = 0, run_c[9];
+        for (int t = 0; t <= 8; t++) run_c[t] = 0;
+        for (int l = c0; l < c1; l++) {
+            if (grp[l] != l) continue;
+            const int sz = gsz[l];
+            const int r = cls_count(plo, phi, sz)+run_c[sz];
+            run_c[sz]++;
+            if (sz >= 2 && sz <= 7 && r < s_split[sz] && nd < AG_CH) det[nd++] = l;
+        }
+        __syncthreads();
+        for (int q = 0; q < nd; q++) {
+            const int l = det[q];
+            int m = head[l];
+            if (m == l) { m = nxt[l]; nxt[l] = nxt[m]; }
+            else head[l] = nxt[m];
+            grp[m] = (short)m;
+            gsz[m] = 1;
+            gsz[l] = (unsigned char)(gsz[l]-1);
+            head[m] = m;
+            nxt[m] = -1;
+        }
+        __syncthreads();
+        count_groups();
+    }
     if (!s_fallback) {
         // wave and position of every group from its size class and rank
         int run_c[9];
@@ -317,24 +427,21 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
     }
     __syncthreads();
     const int fallback = s_fallback;
-    // member lists (list heads reuse `best`)
-    int* head = reinterpret_cast<int*>(best);
-    for (int l = tid; l < n; l += T) head[l] = -1;
-    __syncthreads();
-    for (int l = tid; l < n; l += T) nxt[l] = (short)atomicExch(&head[grp[l]], l);
-    __syncthreads();
+    // particle of local index l
+    auto part = [&](int l) { return J.chunked ? G.order[poff+l] : poff+l; };
     // slots: every used slot and every padding slot of the waves in use is
     // written exactly once (perm) and every particle gets its slot (iperm)
     for (int l = tid; l < n; l += T) {
         if (fallback) {
-            G.iperm[poff+l] = J.soff+l;
-            G.perm[J.soff+l] = poff+l;
+            const int p = part(l);
+            G.iperm[p] = J.soff+l;
+            G.perm[J.soff+l] = p;
             continue;
         }
         if (grp[l] != l) continue;
         int mem[AGG_CAP], m = 0;
         for (int v = head[l]; v >= 0 && m < AGG_CAP; v = nxt[v]) mem[m++] = v;
-        for (int a = 1; a < m; a++) {            // ascending particle order
+        for (int a = 1; a < m; a++) {            // ascending local index
             const int v = mem[a];
             int b = a-1;
             while (b >= 0 && mem[b] > v) { mem[b+1] = mem[b]; b--; }
@@ -342,12 +449,14 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
         }
         const int base = J.soff+8*gbin[l]+goff[l];
         for (int a = 0; a < m; a++) {
-            G.iperm[poff+mem[a]] = base+a;
-            G.perm[base+a] = poff+mem[a];
+            const int p = part(mem[a]);
+            G.iperm[p] = base+a;
+            G.perm[base+a] = p;
         }
     }
+    const int used = fallback ? (n+AGG_CAP-1)/AGG_CAP : s_bins;
     if (fallback) {
-        for (int q = n+tid; q < AGG_CAP*((n+AGG_CAP-1)/AGG_CAP); q += T) G.perm[J.soff+q] = -1;
+        for (int q = n+tid; q < AGG_CAP*used; q += T) G.perm[J.soff+q] = -1;
     } else {
         for (int b = tid; b < s_bins; b += T) {
             int fill = AGG_CAP;
@@ -360,19 +469,31 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
             for (int q = fill; q < AGG_CAP; q++) G.perm[J.soff+AGG_CAP*b+q] = -1;
         }
     }
+    // a chunk's waves beyond those it uses are empty (the solver of a
+    // chunked instance runs every wave of the instance)
+    if (J.chunked)
+        for (int q = AGG_CAP*used+tid; q < AGG_CAP*J.budget; q += T) G.perm[J.soff+q] = -1;
     if (tid == 0) {
-        G.waves[J.inst] = fallback ? (n+AGG_CAP-1)/AGG_CAP : s_bins;
-        scs[J.inst].group_age = 0;
+        if (J.chunked) {
+            G.waves[J.inst] = J.total;
+        } else {
+            G.waves[J.inst] = used;
+            scs[J.inst].group_age = 0;
+        }
     }
 }
 
 void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, int njobs, const int* inst_of,
-                      const InstConst* IC, DevScalars* sc, int interval, int nslots, hipStream_t s)
+                      const InstConst* IC, DevScalars* sc, int interval, int nslots, int ninst,
+                      const int* home_start, const int* home_mem, int ncell, hipStream_t s)
 {
     if (njobs <= 0) return;
+    if (home_start && inst_of)
+        k_agg_order<<<(nslots+255)/256, 256, 0, s>>>(home_start, home_mem, ncell, inst_of, IC, sc, interval, G);
     k_agg_weight<<<(G.pcap+255)/256, 256, 0, s>>>(A, inst_of, sc, interval, G.wq, G.pcap);
-    k_agg_adjacency<<<(nslots+255)/256, 256, 0, s>>>(A, G.wq, inst_of, IC, sc, interval, G.adj, nslots);
+    k_agg_adjacency<<<(nslots+255)/256, 256, 0, s>>>(A, G, inst_of, IC, sc, interval, nslots);
     k_agg_group<<<njobs, AGB_THREADS, 0, s>>>(A, G, jobs, sc, interval);
+    if (inst_of) k_agg_done<<<(ninst+63)/64, 64, 0, s>>>(sc, G.chunk, ninst, interval);
 }
 
 void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s)
@@ -489,6 +610,7 @@ __global__ __launch_bounds__(AI_THREADS) void k_agg_inverse(BsrView A, AggView G
     }
     float* out = G.inv+(size_t)w*AGG_PACK;
     const bool live_r = mem[ti] >= 0, live_c = mem[tj] >= 0;
+    if (t == 0) G.wexact[w] = bad ? 0 : 1;
     if (bad) {
         // 6x6 block-Jacobi inverses on the diagonal, zero elsewhere
         const int i = mem[ti];

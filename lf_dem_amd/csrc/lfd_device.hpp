@@ -234,6 +234,16 @@ __device__ inline bool poll_records(const unsigned long long* pq, int nrec, unsi
     }
     return done;
 }
+// A workgroup-uniform double held in scalar registers (several right-hand
+// sides need every vector register the loop can spare).
+__device__ inline double uni(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // the 6 components of row j of an exchanged vector (8-byte aligned)
 __device__ inline void load6(const float* p, float* o)
 {

@@ -120,12 +120,21 @@ struct AggView {
     double* r0buf;         // [6 * solver slots] b - A x0
     double* part;          // partial records, AGG_PART_PER_BLOCK per 128-slot block
     size_t part_len;
+    int scap;              // solver slots (stride between the right-hand sides of zb and qb)
+    int* wexact;           // [solver waves] 1 when inv is the inverse of the wave's whole block
+    // instances grouped in spatial chunks (more particles than one grouping
+    // workgroup takes): positions are the particles' places in cell order
+    int* chunk;            // [instances] particles per chunk, 0 when grouped whole
+    int* order;            // [particle slots] particle at each position
+    int* rank;             // [particle slots] position of each particle
 };
 
 // One instance of the grouping kernel.
 struct AggJob {
     int inst, poff, n, soff, budget;   // budget: waves available to the instance
-    int pad[3];
+    int chunked;                       // 1: a spatial chunk of a large instance, poff is a position
+    int total;                         // chunked: waves of the whole instance
+    int pad;
 };
 
 // 6x6-block sparse symmetric matrix.  Off-diagonal blocks are stored once per
@@ -264,8 +273,11 @@ int lds_solver_max_groups();
 // aggregate preconditioner (lfd_agg.hip)
 // Grouping of the instances in jobs that are due (never grouped, or grouped
 // `interval` or more steps ago); inst_of / IC null: one instance at slot 0.
+// Chunked instances take their cell order from the cell bins (home_start
+// over ncell cells, home_mem), which they need (null otherwise).
 void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, int njobs, const int* inst_of,
-                      const InstConst* IC, DevScalars* sc, int interval, int nslots, hipStream_t s);
+                      const InstConst* IC, DevScalars* sc, int interval, int nslots, int ninst,
+                      const int* home_start, const int* home_mem, int ncell, hipStream_t s);
 // one step more since each instance's grouping
 void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s);
 void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStream_t s);
@@ -273,6 +285,14 @@ void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStrea
 hipError_t solve_agg(const BsrView& A, const AggView& G, const double* b, double* x, SolverWork& w,
                      DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
 int agg_solver_max_groups(int waves);
+// Large instances (lfd_agg_large.hip): AGG_LARGE_WAVES waves per workgroup,
+// one or two right-hand sides per launch, the waves of the instance dealt
+// to its workgroups in turn.
+static constexpr int AGG_LARGE_WAVES = 16;
+hipError_t solve_agg_large(const BsrView& A, const AggView& G, const double* b, double* x, int nrhs, size_t rs,
+                           SolverWork& w, DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o,
+                           hipStream_t s);
+int agg_large_max_groups();
 hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
 void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);

@@ -2234,16 +2234,6 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
     if (threadIdx.x == 0 && s_fail) atomicOr(&sc->error, E_SYNC_TIMEOUT);
 }
 
-// A workgroup-uniform double held in scalar registers (several right-hand
-// sides need every vector register the loop can spare).
-__device__ inline double uni(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // k_solve_lds for NR >= 2 right-hand sides in one launch (one right-hand
 // side: k_solve_lds_one, whose register allocation measured faster for it):
 // every grid sum carries 3 NR values and every SpMV reads each staged block

@@ -163,6 +163,11 @@ struct Solver {
     SolveJob* ajobs = nullptr;
     int ajobs_cap = 0;
     std::vector<std::pair<int, int>> agg_launches;
+    // large instances (more waves than the 8-wavefront aggregate solver holds
+    // at once): grouped in spatial chunks, one launch of the 16-wave solver each
+    std::vector<std::pair<int, int>> large_launches;
+    std::vector<int> h_chunk;
+    bool allow_large = false; // the chunked grouping takes its cell order from a batch's cell bins
     int regroup_interval = 4; // steps between an instance's groupings (the inverses follow every assembly)
     long long gen = 0;        // bumped whenever the solver's launch sequence changes
     hipError_t alloc(int n_, int pcap_, int icap = 0)
@@ -198,7 +203,12 @@ struct Solver {
             A(dalloc(&agg.wq, pcap));
             agg.pcap = pcap;
             A(dalloc(&agg.inv, (size_t)nwv*AGG_PACK));
-            A(dalloc(&agg.zb, 6*(size_t)scap)); A(dalloc(&agg.qb, 6*(size_t)scap));
+            // zb, qb: two right-hand sides of the large-instance solver
+            A(dalloc(&agg.zb, 12*(size_t)scap)); A(dalloc(&agg.qb, 12*(size_t)scap));
+            agg.scap = scap;
+            A(dalloc(&agg.wexact, nwv));
+            A(dalloc(&agg.chunk, std::max(icap, 1))); A(dalloc(&agg.order, n)); A(dalloc(&agg.rank, n));
+            h_chunk.assign(std::max(icap, 1), 0);
             A(dalloc(&agg.db, 6*(size_t)scap));
             A(dalloc(&agg.dsum[0], 6*(size_t)scap)); A(dalloc(&agg.dsum[1], 6*(size_t)scap));
             A(dalloc(&agg.r0buf, 6*(size_t)scap));
@@ -216,6 +226,8 @@ struct Solver {
                 launch_fill_int(wave_inst, -1, nwv, nullptr);
                 launch_fill_int(agg.iperm, -1, n, nullptr);
                 launch_fill_int(agg.perm, -1, scap, nullptr);
+                launch_fill_int(agg.chunk, 0, std::max(icap, 1), nullptr);
+                launch_fill_int(agg.wexact, 1, nwv, nullptr);
                 e = hipDeviceSynchronize();
             }
         }
@@ -227,7 +239,8 @@ struct Solver {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
                         w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials,
                         w.barrier, jobs, agg.perm, agg.iperm, agg.waves, wave_inst, wave_local, agg.adj, agg.wq, agg.inv, agg.zb,
-                        agg.qb, agg.db, agg.dsum[0], agg.dsum[1], agg.r0buf, agg.part, agg_jobs, ajobs};
+                        agg.qb, agg.db, agg.dsum[0], agg.dsum[1], agg.r0buf, agg.part, agg_jobs, ajobs,
+                        agg.wexact, agg.chunk, agg.order, agg.rank};
         for (void* p : ptrs) if (p) (void)hipFree(p);
         for (auto& e : prof) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         prof.clear();
@@ -240,6 +253,12 @@ struct Solver {
         return v;
     }
 };
+
+// Any persistent solver launch in the job table.
+static bool has_persistent(const Solver& S)
+{
+    return !(S.launches.empty() && S.agg_launches.empty() && S.large_launches.empty());
+}
 
 struct InstSpan { int inst, row0, n; };
 
@@ -254,13 +273,16 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     S.gen++;
     S.launches.clear();
     S.agg_launches.clear();
+    S.large_launches.clear();
+    std::fill(S.h_chunk.begin(), S.h_chunk.end(), 0);
     S.n_agg_jobs = 0;
     S.single = spans.size() == 1 ? spans[0].inst : -1;
     if (spans.empty()) return LFD_OK;
     const int maxg = std::min(lds_solver_max_groups(), S.w.max_groups);
     const int aw = S.opts.agg_waves;
     const int amaxg = S.scap > 0 ? agg_solver_max_groups(aw) : 0;
-    std::vector<SolveJob> jobs, ajobs;
+    const int lmaxg = S.scap > 0 && S.allow_large ? agg_large_max_groups() : 0;
+    std::vector<SolveJob> jobs, ajobs, ljobs;
     std::vector<AggJob> gjobs;
     std::fill(S.h_wave_inst.begin(), S.h_wave_inst.end(), -1);
     int first = 0, afirst = 0;
@@ -290,6 +312,46 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
             }
             continue;
         }
+        if (S.scap > 0 && S.precond == LFD_PRECOND_AGGREGATE && S.kind == LFD_SOLVER_MIXED && lmaxg > 0 &&
+            sp.inst < (int)S.h_chunk.size()) {
+            // a large instance: grouped in spatial chunks of at most AGG_NMAX
+            // particles (cell order), each packed into its own budget of waves;
+            // the instance's waves are about 4% more than its particles / 8
+            // (packing with whole groups), at most what one launch holds
+            const int nch = (sp.n+AGG_NMAX-1)/AGG_NMAX, cs = (sp.n+nch-1)/nch;
+            std::vector<int> bud(nch);
+            int need = 0;
+            for (int c = 0; c < nch; c++) {
+                bud[c] = (std::min(cs, sp.n-c*cs)+AGG_CAP-1)/AGG_CAP;
+                need += bud[c];
+            }
+            const int want = (int)std::ceil(sp.n/(AGG_CAP*0.96));
+            const int W = std::min({AGG_LARGE_WAVES*lmaxg, std::max(want, need), bw});
+            if (W >= need) {
+                for (int c = 0, spare = W-need; spare > 0; c = (c+1)%nch, spare--) bud[c]++;
+                const int soff = sp.row0/LDS_ROWS*AGG_SLOTS_PER_BLOCK;
+                const int Gl = (W+AGG_LARGE_WAVES-1)/AGG_LARGE_WAVES;
+                S.large_launches.emplace_back((int)ljobs.size(), Gl);
+                for (int g = 0; g < Gl; g++) {
+                    SolveJob j{};
+                    j.inst = sp.inst; j.lwg = g; j.G = Gl; j.row0 = soff; j.row_end = soff+AGG_CAP*W;
+                    j.key = sp.row0/LDS_ROWS;
+                    ljobs.push_back(j);
+                }
+                for (int c = 0, so = soff; c < nch; so += AGG_CAP*bud[c], c++) {
+                    AggJob a{};
+                    a.inst = sp.inst; a.poff = sp.row0+c*cs; a.n = std::min(cs, sp.n-c*cs); a.soff = so;
+                    a.budget = bud[c]; a.chunked = 1; a.total = W;
+                    gjobs.push_back(a);
+                }
+                for (int v = 0; v < W; v++) {
+                    S.h_wave_inst[soff/AGG_CAP+v] = sp.inst;
+                    S.h_wave_local[soff/AGG_CAP+v] = v;
+                }
+                S.h_chunk[sp.inst] = cs;
+                continue;
+            }
+        }
         const int G = (sp.n+LDS_ROWS-1)/LDS_ROWS;
         if (G > maxg) {
             if (spans.size() == 1) return LFD_OK;
@@ -308,6 +370,9 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     }
     if ((int)jobs.size() > first) S.launches.emplace_back(first, (int)jobs.size()-first);
     if ((int)ajobs.size() > afirst) S.agg_launches.emplace_back(afirst, (int)ajobs.size()-afirst);
+    // the large instances' jobs follow the aggregate solver's
+    for (auto& L : S.large_launches) L.first += (int)ajobs.size();
+    ajobs.insert(ajobs.end(), ljobs.begin(), ljobs.end());
     if ((int)jobs.size() > S.jobs_cap || (int)ajobs.size() > S.ajobs_cap)
         return fail(LFD_ERR_CAPACITY, "solver job table capacity exceeded");
     if (!jobs.empty())
@@ -317,6 +382,7 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
         HIPCHK(hipMemcpyAsync(S.agg_jobs, gjobs.data(), gjobs.size()*sizeof(AggJob), hipMemcpyHostToDevice, s));
     }
     if (S.scap > 0) {
+        HIPCHK(hipMemcpyAsync(S.agg.chunk, S.h_chunk.data(), S.h_chunk.size()*sizeof(int), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(S.wave_inst, S.h_wave_inst.data(), S.h_wave_inst.size()*sizeof(int),
                               hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(S.wave_local, S.h_wave_local.data(), S.h_wave_local.size()*sizeof(int),
@@ -355,16 +421,16 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
                      hipStream_t s, int nrhs = 1, size_t rs = 0)
 {
     BsrView A = S.view(npairs_dev);
-    const bool lds_only = S.kind == LFD_SOLVER_MIXED && S.agg_launches.empty() && !S.launches.empty();
-    if (nrhs > 1 && !(lds_only && S.opts.max_rhs > 1)) {
+    // two right-hand sides per launch: the block-Jacobi and large-instance solvers
+    const bool pairs = S.kind == LFD_SOLVER_MIXED && S.agg_launches.empty() && has_persistent(S);
+    if (nrhs > 1 && !(pairs && S.opts.max_rhs > 1)) {
         for (int k = 0; k < nrhs; k++) {
             int rc = run_solve(S, npairs_dev, b+k*rs, x+k*rs, sc, s);
             if (rc) return rc;
         }
         return LFD_OK;
     }
-    if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL ||
-        (S.launches.empty() && S.agg_launches.empty())) {
+    if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL || !has_persistent(S)) {
         if (S.single < 0) return fail(LFD_ERR_UNSUPPORTED, "this solver form takes a single instance");
         sc += S.single;
     }
@@ -387,9 +453,15 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, s));
     }
-    if (S.kind == LFD_SOLVER_MIXED && !(S.launches.empty() && S.agg_launches.empty())) {
+    if (S.kind == LFD_SOLVER_MIXED && has_persistent(S)) {
         for (auto& L : S.agg_launches)
             HIPCHK(solve_agg(A, S.agg, b, x, S.w, sc, S.ajobs+L.first, L.second, S.opts, s));
+        for (auto& L : S.large_launches)
+            for (int k = 0; k < nrhs; k += 2) {
+                const int nk = std::min(2, nrhs-k);
+                HIPCHK(solve_agg_large(A, S.agg, b+k*rs, x+k*rs, nk, rs, S.w, sc, S.ajobs+L.first, L.second,
+                                       S.opts, s));
+            }
         for (auto& L : S.launches)
             for (int k = 0; k < nrhs; k += 2) {
                 const int nk = std::min(2, nrhs-k);
@@ -536,6 +608,7 @@ static int batch_init(lfd_batch* B, int slot_cap, int cell_cap, int inst_cap, co
     A(dalloc(&B->ca.nb, 64*(size_t)ccap)); A(dalloc(&B->ca.nnb, ccap)); A(dalloc(&B->ca.nstatic, ccap));
     A(dalloc(&B->ca.ctype, ccap)); A(dalloc(&B->ca.cell_inst, ccap));
     A(dalloc(&B->d_ic, B->icap)); A(dalloc(&B->sc, B->icap)); A(dalloc(&B->bs, 1));
+    B->S.allow_large = true;
     A(B->S.alloc(M, pcap, B->icap));
 #undef A
     if (e != hipSuccess) return fail(LFD_ERR_HIP, std::string("batch allocation: ")+hipGetErrorString(e));
@@ -743,7 +816,7 @@ static void bat_assemble(lfd_batch* B, int nparts = 0)
     if (B->S.n_agg_jobs > 0) {
         const BsrView A = B->S.view(&B->bs->npairs);
         launch_agg_group(A, B->S.agg, B->S.agg_jobs, B->S.n_agg_jobs, B->inst_of, B->d_ic, B->sc,
-                         B->S.regroup_interval, B->M, B->s);
+                         B->S.regroup_interval, B->M, B->ninst, B->home_start, B->home_mem, B->ccap, B->s);
         launch_agg_inverse(A, B->S.agg, B->S.scap/AGG_CAP, B->s);
     }
 }
@@ -825,11 +898,12 @@ static long long graph_key(const lfd_batch* B)
 static bool capturable(const lfd_batch* B)
 {
     const Solver& S = B->S;
-    if (S.kind != LFD_SOLVER_MIXED || S.profiling || (S.launches.empty() && S.agg_launches.empty())) return false;
+    if (S.kind != LFD_SOLVER_MIXED || S.profiling || !has_persistent(S)) return false;
     auto plain = [&](int nwg, int cap) { return S.opts.launch == LFD_LAUNCH_PLAIN ||
                                                 (S.opts.launch == LFD_LAUNCH_AUTO && nwg <= cap); };
     for (auto& L : S.agg_launches) if (!plain(L.second, agg_solver_max_groups(S.opts.agg_waves))) return false;
     for (auto& L : S.launches) if (!plain(L.second, lds_solver_max_groups())) return false;
+    for (auto& L : S.large_launches) if (!plain(L.second, agg_large_max_groups())) return false;
     return true;
 }
 
@@ -1718,7 +1792,8 @@ extern "C" int lfd_batch_info(lfd_batch* B, int* live, int* slot_cap, int* cell_
     if (slot_cap) *slot_cap = B->M;
     if (cell_cap) *cell_cap = B->ccap;
     if (inst_cap) *inst_cap = B->icap;
-    if (solver_launches) *solver_launches = (int)(B->S.launches.size()+B->S.agg_launches.size());
+    if (solver_launches)
+        *solver_launches = (int)(B->S.launches.size()+B->S.agg_launches.size()+B->S.large_launches.size());
     if (bytes) {
         size_t ps = 0, pc = 0;
         batch_unit_bytes(&ps, &pc);
@@ -1911,7 +1986,8 @@ extern "C" int lfd_stokes_fill_done(lfd_stokes* st)
     if ((rc = set_jobs(st->S, std::vector<InstSpan>{{0, 0, n}}, s))) return rc;
     if (st->S.n_agg_jobs > 0) {
         const BsrView A = st->S.view(st->d_npairs);
-        launch_agg_group(A, st->S.agg, st->S.agg_jobs, st->S.n_agg_jobs, nullptr, nullptr, st->sc, 1, n, s);
+        launch_agg_group(A, st->S.agg, st->S.agg_jobs, st->S.n_agg_jobs, nullptr, nullptr, st->sc, 1, n, 1,
+                         nullptr, nullptr, 0, s);
         launch_agg_inverse(A, st->S.agg, st->S.scap/AGG_CAP, s);
     }
     st->filled = true;
