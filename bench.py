@@ -52,6 +52,18 @@ def solver_bytes_per_iter_rhs(n, p, nrhs):
     return spmv + precond + vec
 
 
+def solver_bytes_per_iter_large(n, p_off, waves, nrhs):
+    """As solver_bytes_per_iter for the solver of a large instance (16 waves
+    per workgroup) with nrhs right-hand sides: its product is A u = r + E u
+    for a preconditioned u, so it reads only the p_off couplings between
+    different waves (both directions); the wave inverses are read once for
+    every right-hand side, the vectors once per right-hand side."""
+    spmv = 2 * p_off * (72 + 4) + nrhs * 6 * 4 * n
+    precond = AGG_PACK * 4 * waves + nrhs * 2 * 6 * 4 * n
+    vec = nrhs * (n * 6 * 4 * (4 + 3) + n * 6 * 4 * 4)
+    return spmv + precond + vec
+
+
 def solver_bytes_per_outer(n, p):
     """FP64 residual r = b - R x: packed FP64 blocks both ways, diag, x, b, r."""
     return 2 * p * (144 + 4) + n * (120 + 8) + n * 6 * 8 * 3
@@ -245,17 +257,28 @@ def run_c5(args):
     sr = S.stress()
     err_final = solve_error_at_state(S, opts, capi)
     sps = args.steps/elapsed
-    # roofline of the solver launch (block-Jacobi LDS solver, one or two
-    # right-hand sides per launch as they ran)
+    # roofline of the solver launch (the aggregate solver of a large instance
+    # or the block-Jacobi LDS solver, one or two right-hand sides per launch
+    # as they ran)
     nrhs = max(1, int(round(4 * args.steps / launches))) if launches else 1
     avg_ms = solve_ms / max(launches, 1)
     iters = (st1["total_inner"] - st0["total_inner"]) / max(launches, 1)
     npairs = S.npairs
-    alg = iters * solver_bytes_per_iter_rhs(n, npairs, nrhs) + \
-        max(st1["iters_outer"], 1) * nrhs * solver_bytes_per_outer(n, npairs)
+    waves = S.precond_waves()
+    pair_suffix = "_pair" if nrhs == 2 else ""
+    p_off = None
+    if waves > 0:
+        wave_of = S.solver_slots() // 8
+        pr = S.pairs()
+        p_off = int(np.count_nonzero(wave_of[pr["p0"]] != wave_of[pr["p1"]]))
+        per_iter = solver_bytes_per_iter_large(n, p_off, waves, nrhs)
+        kernel = "k_solve_agg_large" + pair_suffix
+    else:
+        per_iter = solver_bytes_per_iter_rhs(n, npairs, nrhs)
+        kernel = "k_solve_lds" + pair_suffix
+    alg = iters * per_iter + max(st1["iters_outer"], 1) * nrhs * solver_bytes_per_outer(n, npairs)
     achieved = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    kernel = "k_solve_lds_pair" if nrhs == 2 else "k_solve_lds"
-    traffic = measured_traffic(kernel, "n32000_phi058") if (args.seed == 1 and S.precond_waves() == 0) else None
+    traffic = measured_traffic(kernel, "n32000_phi058") if args.seed == 1 else None
     out = {
         "metric": "steps/s at N=32000 phi=0.58 stress control (2 RHS per solve)", "value": round(sps, 3),
         "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -265,8 +288,9 @@ def run_c5(args):
                    "seed": args.seed, "pairs": S.npairs, "target_stress": target, "tol": args.tol, "launch": args.launch,
                    "parallelism": "single GPU"},
         "solver_form": {0: "mixed, LDS-staged", 1: "plain FP64", 2: "mixed, HBM-streamed"}[opts.kind],
-        "precond_in_use": "aggregate" if S.precond_waves() > 0 else "6x6 block-Jacobi",
-        # the LDS-staged block-Jacobi solver pairs the two right-hand sides
+        "precond_in_use": ("aggregate, %d waves, %d of %d pairs between waves" % (waves, p_off, npairs))
+                          if waves > 0 else "6x6 block-Jacobi",
+        # the large-instance and block-Jacobi LDS solvers pair the two right-hand sides
         # from the launches that ran: a stress-controlled step solves 4 right-hand sides
         "rhs_per_launch": round(4 * args.steps / launches, 2) if launches else None,
         "shear_rate_end": sr["shear_rate"], "viscosity_end": sr["viscosity"],

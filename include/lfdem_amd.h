@@ -56,7 +56,12 @@ typedef enum lfd_solver_kind {
 typedef enum lfd_precond {
 	LFD_PRECOND_BLOCK_JACOBI = 0,  /* inverse of each 6x6 diagonal block */
 	LFD_PRECOND_AGGREGATE = 1      /* inverse of the 48x48 diagonal block of groups of up to 8 strongly
-	                                  coupled particles (instances of up to 8192 particles) */
+	                                  coupled particles.  Up to 8192 particles an instance is grouped
+	                                  whole; a larger one of a system or batch (not the StokesSolver
+	                                  drop-in) is grouped in cell-ordered chunks and solved by a
+	                                  solver launch of its own, with 16 waves per workgroup (up to
+	                                  16 x 8 x the device's compute units particles, 32768 on MI355X);
+	                                  a larger instance still runs 6x6 block-Jacobi */
 } lfd_precond;
 
 /* How the persistent solver kernels are launched.  They synchronise their
@@ -87,8 +92,8 @@ typedef struct lfd_solve_opts {
 	int regroup_interval;   /* LFD_PRECOND_AGGREGATE: steps between regroupings (the block inverses
 	                           are recomputed at every assembly) */
 	int launch;             /* lfd_launch */
-	int max_rhs;            /* right-hand sides one launch of the LDS-staged solver (block-Jacobi)
-	                           takes together when a step solves several with one matrix (the rate
+	int max_rhs;            /* right-hand sides one launch of the LDS-staged solver (block-Jacobi,
+	                           or the aggregate solver of a large instance) takes together when a step solves several with one matrix (the rate
 	                           independent and rate proportional parts of stress control, the four
 	                           parts of a stress evaluation): 0 = two, 1 = one */
 	int agg_waves;          /* LFD_PRECOND_AGGREGATE: waves of 8 particles per workgroup of the solver,
@@ -245,6 +250,11 @@ int lfd_system_get_solve_stats(lfd_system *s, lfd_solve_stats *st, long long *to
  * of the aggregate preconditioner; 0 under 6x6 block-Jacobi.  No counterpart
  * in the reference (its factorisation is CHOLMOD's). */
 int lfd_system_precond_waves(lfd_system *s, int *waves);
+/* Solver slot of each of the n particles under the aggregate preconditioner,
+ * counted from the system's first slot: the particles of wave w hold slots
+ * 8w .. 8w+7 and share one inverted diagonal block; -1 for every particle
+ * under 6x6 block-Jacobi.  No counterpart in the reference. */
+int lfd_system_solver_slots(lfd_system *s, int *slots);
 int lfd_system_set_opts(lfd_system *s, const lfd_solve_opts *o);
 /* Solver-launch timing with HIP events on the library's stream: returns the
  * summed duration (ms) and count of the launches recorded since the previous
@@ -334,6 +344,8 @@ int lfd_batch_get_pair_output(lfd_batch *b, int id, int cap, double *out);
 int lfd_batch_get_na_disp(lfd_batch *b, int id, double *out3);
 /* As lfd_system_precond_waves, for instance id of a batch. */
 int lfd_batch_precond_waves(lfd_batch *b, int id, int *waves);
+/* As lfd_system_solver_slots, for instance id of a batch. */
+int lfd_batch_solver_slots(lfd_batch *b, int id, int *slots);
 int lfd_batch_get_pairs(lfd_batch *b, int id, int cap, int *p0, int *p1, int *state, double *geom5,
                         double *disp_tan);
 int lfd_batch_num_pairs(lfd_batch *b, int id);

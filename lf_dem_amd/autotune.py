@@ -50,17 +50,24 @@ SYSTEM_CANDIDATES = [
     ("hbm blocks, 4 lanes per row", dict(kind=2, precond=0, lanes_per_row=4)),
     ("hbm blocks, 8 lanes per row", dict(kind=2, precond=0, lanes_per_row=8)),
 ]
-# largest instance the aggregate preconditioner groups (LFD_PRECOND_AGGREGATE
-# in include/lfdem_amd.h); a larger one runs 6x6 block-Jacobi whatever is asked
-AGGREGATE_MAX_N = 8192
+# largest instance the aggregate preconditioner groups whole, with the 8- or
+# 12-wave solver; above it (LFD_PRECOND_AGGREGATE in include/lfdem_amd.h) the
+# instance is grouped in spatial chunks and solved with 16 waves per
+# workgroup, up to 16 x 8 particles per compute unit of the device (256 on
+# MI355X); a larger one runs 6x6 block-Jacobi whatever is asked
+AGGREGATE_WHOLE_MAX_N = 8192
+AGGREGATE_MAX_N = 16 * 8 * 256
 
 
 def system_candidates(n):
-    """SYSTEM_CANDIDATES that differ for an instance of n particles: without
-    the aggregate ones above AGGREGATE_MAX_N, where they would all time the
-    block-Jacobi solver."""
-    if n <= AGGREGATE_MAX_N:
+    """SYSTEM_CANDIDATES that differ for an instance of n particles: above
+    AGGREGATE_WHOLE_MAX_N without the waves-per-workgroup choice (the large
+    instances' solver has one form), above AGGREGATE_MAX_N without the
+    aggregate ones, which would all time the block-Jacobi solver."""
+    if n <= AGGREGATE_WHOLE_MAX_N:
         return list(SYSTEM_CANDIDATES)
+    if n <= AGGREGATE_MAX_N:
+        return [c for c in SYSTEM_CANDIDATES if "agg_waves" not in c[1]]
     return [c for c in SYSTEM_CANDIDATES if c[1].get("precond", 1) != 1]
 
 

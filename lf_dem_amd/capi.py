@@ -74,6 +74,8 @@ SIGNATURES = {
     "lfd_system_get_solve_stats": ([_vp, _vp, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "lfd_system_precond_waves": ([_vp, _ip], ctypes.c_int),
     "lfd_batch_precond_waves": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+    "lfd_system_solver_slots": ([_vp, _ip], ctypes.c_int),
+    "lfd_batch_solver_slots": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
     "lfd_system_set_opts": ([_vp, _vp], ctypes.c_int),
     "lfd_system_profile": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
     "lfd_system_set_control": ([_vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
@@ -513,6 +515,13 @@ class System:
         _check(lib().lfd_system_precond_waves(self.h, ctypes.byref(w)))
         return w.value
 
+    def solver_slots(self):
+        """Solver slot of each particle under the aggregate preconditioner
+        (wave = slot // 8); all -1 under 6x6 block-Jacobi."""
+        out = np.zeros(self.np, dtype=np.int32)
+        _check(lib().lfd_system_solver_slots(self.h, _i(out)))
+        return out
+
     def solve_stats(self):
         st = SolveStats()
         tot = ctypes.c_longlong(0)
@@ -623,6 +632,12 @@ class Batch:
         out = np.zeros((cap, len(System.PAIR_OUT)))
         n = _check(lib().lfd_batch_get_pair_output(self.h, int(iid), cap, _d(out)))
         return out[:n].copy()
+
+    def solver_slots(self, iid):
+        """System.solver_slots of instance iid."""
+        out = np.zeros(self.np[iid], dtype=np.int32)
+        _check(lib().lfd_batch_solver_slots(self.h, int(iid), _i(out)))
+        return out
 
     def na_disp(self, iid):
         """System.na_disp of instance iid."""

@@ -1613,6 +1613,32 @@ extern "C" int lfd_system_precond_waves(lfd_system* y, int* waves)
     return inst_precond_waves(y, 0, waves);
 }
 
+// Solver slot of each particle of an instance under the aggregate
+// preconditioner, counted from the instance's first slot (wave = slot / 8);
+// -1 for every particle under 6x6 block-Jacobi.
+static int inst_solver_slots(lfd_batch* B, int id, int* slots)
+{
+    const InstHost* h = live_inst(B, id);
+    if (!h || !slots) return fail(LFD_ERR_ARG, "solver_slots: bad argument");
+    int waves = 0, rc = inst_precond_waves(B, id, &waves);
+    if (rc) return rc;
+    const int n = h->ic.n;
+    if (waves == 0) {
+        std::fill(slots, slots+n, -1);
+        return LFD_OK;
+    }
+    HIPCHK(hipMemcpy(slots, B->S.agg.iperm+h->ic.poff, n*sizeof(int), hipMemcpyDeviceToHost));
+    const int soff = h->ic.poff/LDS_ROWS*AGG_SLOTS_PER_BLOCK;
+    for (int k = 0; k < n; k++) slots[k] -= soff;
+    return LFD_OK;
+}
+
+extern "C" int lfd_system_solver_slots(lfd_system* y, int* slots)
+{
+    if (!y) return fail(LFD_ERR_ARG, "null handle");
+    return inst_solver_slots(y, 0, slots);
+}
+
 // ===================================================================== batch API
 extern "C" int lfd_batch_create(int slot_cap, int cell_cap, int inst_cap, int control, const lfd_solve_opts* o,
                                 lfd_batch** out)
@@ -1755,6 +1781,12 @@ extern "C" int lfd_batch_precond_waves(lfd_batch* B, int id, int* waves)
 {
     if (!B) return fail(LFD_ERR_ARG, "null handle");
     return inst_precond_waves(B, id, waves);
+}
+
+extern "C" int lfd_batch_solver_slots(lfd_batch* B, int id, int* slots)
+{
+    if (!B) return fail(LFD_ERR_ARG, "null handle");
+    return inst_solver_slots(B, id, slots);
 }
 
 extern "C" int lfd_batch_get_state(lfd_batch* B, int id, double* pos, double* vel6, double* na6, double* scalars8)

@@ -48,16 +48,22 @@ def test_signature_groups_similar_workloads():
     assert a != autotune._signature("system", [700], [0.51], "rate")
 
 
-def test_aggregate_candidates_dropped_above_the_grouping_limit():
-    """Above 8192 particles the aggregate preconditioner is unavailable, so
-    its candidates would time block-Jacobi under another name."""
+def test_aggregate_candidates_by_instance_size():
+    """Up to 8192 particles every candidate; above, the aggregate ones without
+    the waves-per-workgroup choice (one solver form for large instances);
+    above the large instances' limit none of them, since they would time
+    block-Jacobi under another name."""
     from lf_dem_amd import autotune
     small = autotune.system_candidates(4000)
-    large = autotune.system_candidates(32000)
+    mid = autotune.system_candidates(32000)
+    large = autotune.system_candidates(autotune.AGGREGATE_MAX_N + 1)
     assert small == autotune.SYSTEM_CANDIDATES
+    assert autotune.system_candidates(autotune.AGGREGATE_WHOLE_MAX_N) == small
+    assert any(c[1].get("precond") == 1 for c in mid)
+    assert all("agg_waves" not in c[1] for c in mid)
+    assert {c[0] for c in mid} == {c[0] for c in small if "agg_waves" not in c[1]}
     assert all(c[1].get("precond") == 0 for c in large)
     assert {c[0] for c in large} == {c[0] for c in small if c[1].get("precond") == 0}
-    assert autotune.system_candidates(autotune.AGGREGATE_MAX_N) == small
 
 
 def test_near_ties_keep_the_candidate_listed_first():

@@ -10,7 +10,8 @@
   side within FP64 rounding of the oracle's; FP64 residual of the device
   solution on the host below the solver target; after stress-controlled
   steps the shear stress equals the target and both right-hand sides per
-  launch agree with one launch per right-hand side.
+  launch agree with one launch per right-hand side (the aggregate solver of
+  large instances).
 * 64-instance sweep (N in [250, 2000]): two steps of the whole batch, every
   instance bit-identical to its lone run; three instances against the
   oracle's pair sets and direct solve at t=0.
@@ -159,6 +160,8 @@ def test_n32000_stress_control(c5, capi):
         print("N=32000 stress-controlled step", k, flush=True)
         for s in (P, Q):
             s.step(1)
+            # the aggregate preconditioner of large instances (chunked grouping, 16-wave solver)
+            assert s.precond_waves() > 0
             st = s.solve_stats()
             assert st["res_norm"] <= 1e-12 * st["rhs_norm"] * 1.0000001
         a, b = P.state(), Q.state()

@@ -1,0 +1,126 @@
+"""GPU checks of the aggregate preconditioner on instances above 8192
+particles (chunked grouping, 16-wave solver with Eisenstat's product).
+
+* the grouping is a valid packing: every particle one solver slot, at most 8
+  particles per wave, the waves within the instance's budget, and no wave
+  mixing particles of two chunks (cell-ordered positions, chunks of at most
+  8192);
+* the velocities reach the FP64 residual target (checked on the host from the
+  device's own blocks) and equal the 6x6 block-Jacobi solver's to 1e-9; a few
+  steps later the trajectories still agree;
+* steps replayed from captured step graphs equal eager steps bit for bit;
+* a large instance in a batch with a small one: each equal to its lone run.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import conf
+from tests.test_gpu_configs import _host_residual, _rel
+
+pytestmark = pytest.mark.gpu
+
+N, PHI = 9000, 0.50
+CHUNK_MAX = 8192
+
+
+@pytest.fixture(scope="module")
+def c9k():
+    print("generating N=%d phi=%.2f" % (N, PHI), flush=True)
+    return conf(N, PHI, 1)
+
+
+def _cell_order_chunks(S, n):
+    """Chunk of each particle: positions in cell order (cells by label,
+    members by index), split into ceil(n / 8192) chunks of equal size."""
+    labels = S.cells()["labels"]
+    order = np.lexsort((np.arange(n), labels))
+    rank = np.empty(n, dtype=np.int64)
+    rank[order] = np.arange(n)
+    nch = -(-n // CHUNK_MAX)
+    cs = -(-n // nch)
+    return rank // cs
+
+
+def test_grouping_is_a_valid_chunked_packing(c9k, capi):
+    pos, rad, L = c9k
+    S = capi.System(pos, rad, L, opts=capi.default_opts(precond=1))
+    chunk = _cell_order_chunks(S, N)
+    S.first_update()
+    S.compute_velocities(True)
+    waves = S.precond_waves()
+    slots = S.solver_slots()
+    print("waves", waves, "particles per wave %.3f" % (N / waves), flush=True)
+    assert -(-N // 8) <= waves <= 16 * 8 * 256 // 8
+    assert slots.min() >= 0 and slots.max() < 8 * waves
+    assert len(np.unique(slots)) == N
+    wave_of = slots // 8
+    for w in np.unique(wave_of):
+        assert len(np.unique(chunk[wave_of == w])) == 1, w
+    # the strongest couplings are mostly inside waves: a third or more of the pairs
+    pr = S.pairs()
+    inside = np.count_nonzero(wave_of[pr["p0"]] == wave_of[pr["p1"]])
+    print("pairs inside waves %d of %d" % (inside, len(pr["p0"])), flush=True)
+    assert inside >= len(pr["p0"]) // 4
+
+
+def test_velocities_against_block_jacobi_and_host_residual(c9k, capi, oracle):
+    pos, rad, L = c9k
+    out = {}
+    for pc in (1, 0):
+        S = capi.System(pos, rad, L, opts=capi.default_opts(precond=pc))
+        S.first_update()
+        S.assemble()
+        db, od, p0, p1 = S.blocks()
+        b = S.rhs()
+        S.compute_velocities(True)
+        st = S.solve_stats()
+        x = np.concatenate([S.state()["na_vel"], S.state()["na_ang_vel"]], axis=1).ravel()
+        res = _host_residual(oracle, N, db, od, p0, p1, x, b)
+        print("precond", pc, "waves", S.precond_waves(), "host residual", res, st, flush=True)
+        assert res <= 2e-12
+        out[pc] = S
+    for key in ("na_vel", "na_ang_vel"):
+        assert _rel(out[1].state()[key], out[0].state()[key]) <= 1e-9, key
+    for s in out.values():
+        s.step(5)                      # through a regrouping (every 4 steps)
+    a, b = out[1].state(), out[0].state()
+    assert np.abs(a["pos"] - b["pos"]).max() <= 1e-8 * L.max()
+    for key in ("na_vel", "na_ang_vel"):
+        assert _rel(a[key], b[key]) <= 1e-8, key
+
+
+def test_graph_replay_equals_eager(c9k, capi):
+    pos, rad, L = c9k
+    E = capi.System(pos, rad, L)
+    G = capi.System(pos, rad, L)
+    for s in (E, G):
+        s.first_update()
+    E.step(3)
+    G.step(3, graph=True)
+    a, b = E.state(), G.state()
+    for key in ("pos", "vel", "ang_vel", "na_vel", "na_ang_vel", "shear_disp"):
+        assert np.array_equal(a[key], b[key]), key
+    assert (a["dt"], a["strain"]) == (b["dt"], b["strain"])
+
+
+def test_large_instance_in_a_batch_equals_its_lone_run(c9k, capi):
+    pos, rad, L = c9k
+    small = conf(500, 0.52, 7)
+    confs = [(pos, rad, L), small]
+    slots = cells = 0
+    for p, r, l in confs:
+        sl, ce, _ = capi.footprint(len(r), r, l)
+        slots += sl
+        cells += ce
+    B = capi.Batch(slots, cells, 2)
+    ids = [B.add(p, r, l) for p, r, l in confs]
+    B.step(3)
+    B.sync()
+    assert B.solver_slots(ids[0]).min() >= 0
+    for k, (p, r, l) in enumerate(confs):
+        S = capi.System(p, r, l)
+        S.first_update()
+        S.step(3)
+        a, b = B.state(ids[k]), S.state()
+        for key in ("pos", "vel", "ang_vel", "na_vel", "na_ang_vel", "shear_disp"):
+            assert np.array_equal(a[key], b[key]), (k, key)
