@@ -1059,7 +1059,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             if (own) mbuf[1][gs] = u;
             double s_r2 = (double)r*r, z0 = 0, z1 = 0;
             gsum3(s_r2, z0, z1);
-            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            const double stop2 = inner_stop2(o.inner_tol, s_r2, o.tol, bb, rr64);
             float wv = spmv32(mbuf[1], u);
             float zv = 0, qv = 0, sv2 = 0, pv = 0;
             double gamma_old = 1, alpha_old = 1;
@@ -1108,7 +1108,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             double s_rz = (double)r*z, s_r2 = (double)r*r, z0 = 0;
             gsum3(s_rz, s_r2, z0);
             double rz = s_rz;
-            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            const double stop2 = inner_stop2(o.inner_tol, s_r2, o.tol, bb, rr64);
             float q = spmv32(Gv.zb, z);
             double s_pq = (double)p*q, z1 = 0, z2 = 0;
             gsum3(s_pq, z1, z2);

@@ -374,7 +374,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
             }
             gsum(v);
             for (int k = 0; k < NR; k++) {
-                stop2[k] = uni(o.inner_tol*o.inner_tol*v[3*k]);
+                stop2[k] = uni(inner_stop2(o.inner_tol, v[3*k], o.tol, bb[k], rr64[k]));
                 zv[k] = qv[k] = sv[k] = pv[k] = 0;
                 gamma_old[k] = alpha_old[k] = 1;
             }
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
             gsum(v);
             for (int k = 0; k < NR; k++) {
                 rz[k] = v[3*k];
-                stop2[k] = o.inner_tol*o.inner_tol*v[3*k+1];
+                stop2[k] = inner_stop2(o.inner_tol, v[3*k+1], o.tol, bb[k], rr64[k]);
             }
             spmv_e(Gv.zb, r, q);                 // A z = r + E z
             for (int k = 0; k < NR; k++) { v[3*k] = (double)p[k]*q[k]; v[3*k+1] = 0; v[3*k+2] = 0; }

@@ -244,6 +244,16 @@ __device__ inline double uni(double v)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Squared stop of an inner FP32 solve, in units of the pass's scaled
+// residual (||r|| = 1 before the coarse correction; s_r2 after it): a drop
+// of inner_tol, or less when that is all the pass needs to bring the FP64
+// residual to half the refinement target tol ||b|| (bb = ||b||^2, rr64 the
+// pass's ||r||^2), so the last pass does not overshoot the target.
+__device__ inline double inner_stop2(double inner_tol, double s_r2, double tol, double bb, double rr64)
+{
+    return fmax(inner_tol*inner_tol*s_r2, 0.25*tol*tol*bb/rr64);
+}
+
 // the 6 components of row j of an exchanged vector (8-byte aligned)
 __device__ inline void load6(const float* p, float* o)
 {

@@ -1800,7 +1800,7 @@ __global__ __launch_bounds__(PS_BLOCK) void k_solve_mixed(BsrView A, const doubl
         }
         grid_sum2(grid, s_pq, dummy, part, buf, G, lds);
         double pq = s_pq;
-        const double stop2 = o.inner_tol*o.inner_tol*r20;
+        const double stop2 = inner_stop2(o.inner_tol, r20, o.tol, bb, rr64);
         int it;
         for (it = 0; it < o.max_inner; it++) {
             if (!(pq > 0)) break;
@@ -2147,7 +2147,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
             if (own) mbuf[1][gi] = u;
             double s_r2 = (double)r*r, z0 = 0, z1 = 0;
             gsum3(s_r2, z0, z1);
-            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            const double stop2 = inner_stop2(o.inner_tol, s_r2, o.tol, bb, rr64);
             float wv = spmv32(mbuf[1], u);
             float zv = 0, qv = 0, sv = 0, pv = 0;
             double gamma_old = 1, alpha_old = 1;
@@ -2193,7 +2193,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
             double s_rz = (double)r*z, s_r2 = (double)r*r, z0 = 0;
             gsum3(s_rz, s_r2, z0);
             double rz = s_rz;
-            const double stop2 = o.inner_tol*o.inner_tol*s_r2;
+            const double stop2 = inner_stop2(o.inner_tol, s_r2, o.tol, bb, rr64);
             float q = spmv32(w.z, z);
             double s_pq = (double)p*q, z1 = 0, z2 = 0;
             gsum3(s_pq, z1, z2);
@@ -2538,7 +2538,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             }
             gsum(v);
             for (int k = 0; k < NR; k++) {
-                stop2[k] = uni(o.inner_tol*o.inner_tol*v[3*k]);
+                stop2[k] = uni(inner_stop2(o.inner_tol, v[3*k], o.tol, bb[k], rr64[k]));
                 zv[k] = qv[k] = sv[k] = pv[k] = 0;
                 gamma_old[k] = alpha_old[k] = 1;
             }
@@ -2600,7 +2600,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             gsum(v);
             for (int k = 0; k < NR; k++) {
                 rz[k] = v[3*k];
-                stop2[k] = o.inner_tol*o.inner_tol*v[3*k+1];
+                stop2[k] = inner_stop2(o.inner_tol, v[3*k+1], o.tol, bb[k], rr64[k]);
             }
             spmv32(w.z, z, q);
             for (int k = 0; k < NR; k++) { v[3*k] = (double)p[k]*q[k]; v[3*k+1] = 0; v[3*k+2] = 0; }
