@@ -31,11 +31,12 @@ HBM_PEAK_GBS = 8000.0
 
 
 def solver_bytes_per_iter(n, p, waves=0):
-    """Algorithmic bytes of one inner FP32 PCG iteration: SpMV with LF_DEM's
-    packed 6x6 blocks applied in both directions (18 floats + index per pair
-    block, 15 floats + index per diagonal block, neighbour 6-vectors), the
-    preconditioner (6x6 FP32 inverses per particle, or the packed 48x48 FP32
-    inverse of each 8-particle wave), and the pipelined-PCG vector updates."""
+    """Algorithmic bytes of one inner FP32 PCG iteration with the 6x6
+    block-Jacobi preconditioner: SpMV with LF_DEM's packed 6x6 blocks applied
+    in both directions (18 floats + index per pair block, 15 floats + index
+    per diagonal block, the row's own 6-vector), the 6x6 FP32 inverses, and
+    the pipelined-PCG vector updates.  With the aggregate preconditioner
+    (waves > 0) see solver_bytes_per_iter_agg."""
     spmv = 2 * p * (72 + 4) + n * (60 + 8) + 6 * 4 * n
     precond = (AGG_PACK * 4 * waves if waves else 144 * n) + 2 * 6 * 4 * n
     vec = n * 6 * 4 * (4 + 3) + n * 6 * 4 * 4
@@ -52,16 +53,25 @@ def solver_bytes_per_iter_rhs(n, p, nrhs):
     return spmv + precond + vec
 
 
-def solver_bytes_per_iter_large(n, p_off, waves, nrhs):
-    """As solver_bytes_per_iter for the solver of a large instance (16 waves
-    per workgroup) with nrhs right-hand sides: its product is A u = r + E u
-    for a preconditioned u, so it reads only the p_off couplings between
-    different waves (both directions); the wave inverses are read once for
-    every right-hand side, the vectors once per right-hand side."""
+def solver_bytes_per_iter_agg(n, p_off, waves, nrhs=1):
+    """As solver_bytes_per_iter for the aggregate-preconditioned solvers with
+    nrhs right-hand sides: their product is Eisenstat's, A u = r + E u for a
+    preconditioned u, so they read only the p_off couplings between different
+    waves (both directions) and no diagonal block; the packed FP32 wave
+    inverses are read once for every right-hand side, the vectors once per
+    right-hand side."""
     spmv = 2 * p_off * (72 + 4) + nrhs * 6 * 4 * n
     precond = AGG_PACK * 4 * waves + nrhs * 2 * 6 * 4 * n
     vec = nrhs * (n * 6 * 4 * (4 + 3) + n * 6 * 4 * 4)
     return spmv + precond + vec
+
+
+def pairs_between_waves(S):
+    """Pairs of a system whose particles sit in different waves of the
+    aggregate preconditioner."""
+    wave_of = S.solver_slots() // 8
+    pr = S.pairs()
+    return int(np.count_nonzero(wave_of[pr["p0"]] != wave_of[pr["p1"]]))
 
 
 def solver_bytes_per_outer(n, p):
@@ -170,8 +180,13 @@ def run(args):
     # roofline of the dominant kernel (the persistent solver launch)
     avg_ms = solve_ms / max(launches, 1)
     outer_per_solve = float(max(st1["iters_outer"], 1))     # refinement passes of the last solve
-    alg_bytes = iters_per_solve * solver_bytes_per_iter(args.n, npairs, waves) + \
-        outer_per_solve * solver_bytes_per_outer(args.n, npairs)
+    if waves > 0:
+        p_off = pairs_between_waves(S)
+        per_iter = solver_bytes_per_iter_agg(args.n, p_off, waves)
+    else:
+        p_off = None
+        per_iter = solver_bytes_per_iter(args.n, npairs)
+    alg_bytes = iters_per_solve * per_iter + outer_per_solve * solver_bytes_per_outer(args.n, npairs)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     kernel = ["k_solve_lds", "k_solve_agg"][args.precond]
     default_workload = (args.n, args.phi, args.seed) == (4000, 0.56, 1)
@@ -193,7 +208,7 @@ def run(args):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
         "solver": {"inner_iters_per_solve": round(iters_per_solve, 1), "solve_launches": launches,
-                   "avg_solve_ms": round(avg_ms, 4), "precond_waves": waves,
+                   "avg_solve_ms": round(avg_ms, 4), "precond_waves": waves, "pairs_between_waves": p_off,
                    "last_solve": {"outer_passes": st1["iters_outer"], "inner_iters": st1["iters_inner"],
                                   "rel_residual": st1["res_norm"] / st1["rhs_norm"] if st1["rhs_norm"] else None}},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
@@ -268,10 +283,8 @@ def run_c5(args):
     pair_suffix = "_pair" if nrhs == 2 else ""
     p_off = None
     if waves > 0:
-        wave_of = S.solver_slots() // 8
-        pr = S.pairs()
-        p_off = int(np.count_nonzero(wave_of[pr["p0"]] != wave_of[pr["p1"]]))
-        per_iter = solver_bytes_per_iter_large(n, p_off, waves, nrhs)
+        p_off = pairs_between_waves(S)
+        per_iter = solver_bytes_per_iter_agg(n, p_off, waves, nrhs)
         kernel = "k_solve_agg_large" + pair_suffix
     else:
         per_iter = solver_bytes_per_iter_rhs(n, npairs, nrhs)

@@ -669,7 +669,6 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
     constexpr int AS_ROWS = AS_BLOCK/AS_LPR;        // solver slots per workgroup
     constexpr bool INV_REG = AS_WAVES <= 8;         // inverse rows in registers (else LDS)
     __shared__ int ent_j[AS_ENTRY_CAP];             // partner slot; ~slot for the transposed block
-    __shared__ int ent_k[AS_ENTRY_CAP];             // pair index (FP64 block)
     __shared__ float ent_b[AS_ENTRY_CAP*18];
     __shared__ float s_inv[INV_REG ? 1 : AS_WAVES*AGG_PACK];
     __shared__ AsRow rows[AS_ROWS];
@@ -697,18 +696,16 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
     const int kk = c < 6 ? 6*(int)(lane/AS_LPR)+c : 0;
     unsigned gen = ld_acquire(bar+32);
     if (threadIdx.x == 0) s_fail = 0;
-    // ---- stage the blocks of this workgroup's rows and its waves' inverses
-    if (threadIdx.x < AS_ROWS) {
-        const int ss = sbase+threadIdx.x;
-        const int ii = ss < send ? Gv.perm[ss] : -1;
-        rows[threadIdx.x].ne = ii >= 0 ? (A.tptr[ii+1]-A.tptr[ii])+(A.row_ptr[ii+1]-A.row_ptr[ii]) : 0;
-    }
+    // this wavefront's wave (global index); exact: its inverse is the
+    // inverse of the wave's whole block (not the 6x6 fallback)
+    const bool wlive = (int)wid < nw-(sbase-J.row0)/AGG_CAP;
+    const int wave = sbase/AGG_CAP+(int)wid;
+    const bool exact = wlive && Gv.wexact[wave] != 0;
     // row kk of this wavefront's inverse (register form; zero for lanes 6, 7
     // of a row group and for waves beyond the instance's)
     float Mr[INV_REG ? AGG_DIM : 1];
     if constexpr (INV_REG) {
-        const bool wlive = (int)wid < nw-(sbase-J.row0)/AGG_CAP;
-        const float* src = Gv.inv+(size_t)(sbase/AGG_CAP+(wlive ? wid : 0))*AGG_PACK;
+        const float* src = Gv.inv+(size_t)(wlive ? wave : sbase/AGG_CAP)*AGG_PACK;
         const int kb = kk*(kk+1)/2;
 #pragma unroll
         for (int m = 0; m < AGG_DIM; m++) Mr[m] = (wlive && c < 6) ? src[m <= kk ? kb+m : m*(m+1)/2+kk] : 0.f;
@@ -717,6 +714,35 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
         const int nload = min(AS_WAVES, nw-(sbase-J.row0)/AGG_CAP)*AGG_PACK;
         for (int a = threadIdx.x; a < AS_WAVES*AGG_PACK; a += AS_BLOCK) s_inv[a] = a < nload ? src[a] : 0.f;
     }
+    // ---- stage the couplings of each row to other waves (all of them when
+    // the wave's inverse is the 6x6 fallback): the product is Eisenstat's,
+    // A u = r + E u for u = W^-1 r, W the waves' diagonal blocks
+    int l0 = 0, nl = 0, u0 = 0, nt = 0;
+    if (live) {
+        l0 = A.tptr[i]; nl = A.tptr[i+1]-l0; u0 = A.row_ptr[i];
+        nt = nl+(A.row_ptr[i+1]-u0);
+    }
+    // entry e of row i: the pair index and the partner's slot (~slot when
+    // the row applies the transposed block)
+    auto entry = [&](int e, int& k) -> int {
+        if (e < nl) { k = A.tidx[l0+e]; return Gv.iperm[A.p0[k]]; }
+        k = u0+(e-nl);
+        return ~Gv.iperm[A.p1[k]];
+    };
+    auto kept = [&](int jj) { return !exact || (jj >= 0 ? jj : ~jj)/AGG_CAP != wave; };
+    int mine = 0;
+    for (int e = c; e < nt; e += AS_LPR) {
+        int k;
+        if (kept(entry(e, k))) mine++;
+    }
+    int pre = mine;                                  // inclusive prefix over the row's 8 lanes
+    for (int d = 1; d < AS_LPR; d <<= 1) {
+        const int t = __shfl_up(pre, d, AS_LPR);
+        if (c >= d) pre += t;
+    }
+    const int rowtot = __shfl(pre, AS_LPR-1, AS_LPR);
+    pre -= mine;
+    if (c == 0) rows[lr].ne = live ? rowtot : 0;
     __syncthreads();
     if (threadIdx.x == 0) {
         int used = 0;
@@ -727,35 +753,23 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
         }
     }
     __syncthreads();
-    if (live && rows[lr].ne > 0) {
-        const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
-        for (int e = c; e < rows[lr].ne; e += AS_LPR) {
-            int k, js;
-            if (e < nl) { k = A.tidx[l0+e]; js = Gv.iperm[A.p0[k]]; }
-            else { k = u0+(e-nl); js = ~Gv.iperm[A.p1[k]]; }
-            const int s = rows[lr].e0+e;
-            ent_j[s] = js;
-            ent_k[s] = k;
-            for (int m = 0; m < 18; m++) ent_b[18*s+m] = A.od32[18*(size_t)k+m];
-        }
-    }
-    // row c of D in registers (lanes 0..5)
-    float Dr[6] = {0, 0, 0, 0, 0, 0};
-    if (own) {
-        for (int m = 0; m < 6; m++) {
-            float e6[6] = {0, 0, 0, 0, 0, 0};
-            e6[m] = 1.f;
-            float y[6] = {0, 0, 0, 0, 0, 0};
-            dmul<float, float>(A.diag32+15*(size_t)i, e6, y);
-            Dr[m] = y[c];
+    const int ne = live ? rows[lr].ne : 0, e0 = live ? rows[lr].e0 : 0;
+    if (ne > 0) {
+        int sidx = e0+pre;
+        for (int e = c; e < nt; e += AS_LPR) {
+            int k;
+            const int jj = entry(e, k);
+            if (!kept(jj)) continue;
+            ent_j[sidx] = jj;
+            for (int m = 0; m < 18; m++) ent_b[18*sidx+m] = A.od32[18*(size_t)k+m];
+            sidx++;
         }
     }
     __syncthreads();
-    const int ne = live ? rows[lr].ne : 0, e0 = live ? rows[lr].e0 : 0;
 
-    // component c of A32 v for this row: neighbours' v from vbuf (slot order),
-    // own v from the group's lanes
-    auto spmv32 = [&](const float* vbuf, float vown) -> float {
+    // component c of yin + E v for this row: neighbours' v from vbuf (slot
+    // order); with v = W^-1 yin this is A v
+    auto spmv_e = [&](const float* vbuf, float yin) -> float {
         float y[6] = {0, 0, 0, 0, 0, 0};
         if (live) {
             if (ne >= 0) {
@@ -768,28 +782,23 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
                     else omul_t<float, float>(bl, xj, y);
                 }
             } else {
-                const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
-                const int nt = nl+(A.row_ptr[i+1]-u0);
                 for (int e = c; e < nt; e += AS_LPR) {
+                    int k;
+                    const int jj = entry(e, k);
+                    if (!kept(jj)) continue;
                     float xj[6];
-                    if (e < nl) {
-                        const int k = A.tidx[l0+e];
-                        load6(vbuf+6*(size_t)Gv.iperm[A.p0[k]], xj);
-                        omul<float, float>(A.od32+18*(size_t)k, xj, y);
-                    } else {
-                        const int k = u0+(e-nl);
-                        load6(vbuf+6*(size_t)Gv.iperm[A.p1[k]], xj);
-                        omul_t<float, float>(A.od32+18*(size_t)k, xj, y);
-                    }
+                    load6(vbuf+6*(size_t)(jj >= 0 ? jj : ~jj), xj);
+                    if (jj >= 0) omul<float, float>(A.od32+18*(size_t)k, xj, y);
+                    else omul_t<float, float>(A.od32+18*(size_t)k, xj, y);
                 }
             }
         }
-        for (int m = 0; m < 6; m++) y[m] = sum8f(y[m]);
-        float vv[6];
-        bcast8_6(vown, c, vv);
-        float yc = 0, mine = 0;
-        for (int m = 0; m < 6; m++) { yc += Dr[m]*vv[m]; if (m == c) mine = y[m]; }
-        return own ? mine+yc : 0.f;
+        float me = 0;
+        for (int m = 0; m < 6; m++) {
+            const float t = sum8f(y[m]);
+            if (m == c) me = t;
+        }
+        return own ? yin+me : 0.f;
     };
     // z = W^-1 r on the lane's wave: component m of r is read from its lane
     // (8 (m / 6) + m % 6) into a scalar register, so LDS serves only the
@@ -961,29 +970,13 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             }
         };
         if (live) {
-            if (ne >= 0) {
-                for (int e = c; e < ne; e += AS_LPR) {
-                    const int jj = ent_j[e0+e];
-                    double xj[6];
-                    dval(jj >= 0 ? jj : ~jj, xj);
-                    if (jj >= 0) omul<double, double>(A.od64+18*(size_t)ent_k[e0+e], xj, y);
-                    else omul_t<double, double>(A.od64+18*(size_t)ent_k[e0+e], xj, y);
-                }
-            } else {
-                const int l0 = A.tptr[i], nl = A.tptr[i+1]-l0, u0 = A.row_ptr[i];
-                const int nt = nl+(A.row_ptr[i+1]-u0);
-                for (int e = c; e < nt; e += AS_LPR) {
-                    double xj[6];
-                    if (e < nl) {
-                        const int k = A.tidx[l0+e];
-                        dval(Gv.iperm[A.p0[k]], xj);
-                        omul<double, double>(A.od64+18*(size_t)k, xj, y);
-                    } else {
-                        const int k = u0+(e-nl);
-                        dval(Gv.iperm[A.p1[k]], xj);
-                        omul_t<double, double>(A.od64+18*(size_t)k, xj, y);
-                    }
-                }
+            for (int e = c; e < nt; e += AS_LPR) {
+                int k;
+                const int jj = entry(e, k);
+                double xj[6];
+                dval(jj >= 0 ? jj : ~jj, xj);
+                if (jj >= 0) omul<double, double>(A.od64+18*(size_t)k, xj, y);
+                else omul_t<double, double>(A.od64+18*(size_t)k, xj, y);
             }
             if (c == 0) {
                 double xi[6];
@@ -1060,7 +1053,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             double s_r2 = (double)r*r, z0 = 0, z1 = 0;
             gsum3(s_r2, z0, z1);
             const double stop2 = inner_stop2(o.inner_tol, s_r2, o.tol, bb, rr64);
-            float wv = spmv32(mbuf[1], u);
+            float wv = spmv_e(mbuf[1], r);          // A u = r + E u
             float zv = 0, qv = 0, sv2 = 0, pv = 0;
             double gamma_old = 1, alpha_old = 1;
             int par = 0;
@@ -1070,7 +1063,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
                 publish3((double)r*u, (double)wv*u, (double)r*r);
                 double g, dl, rho;
                 gather3_start(g, dl, rho);
-                float nv = spmv32(mbuf[par], mv);
+                float nv = spmv_e(mbuf[par], wv);   // A m = w + E m
                 gather3_finish(g, dl, rho);
                 if (s_fail || rho <= stop2) break;
                 double beta, alpha;
@@ -1109,7 +1102,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             gsum3(s_rz, s_r2, z0);
             double rz = s_rz;
             const double stop2 = inner_stop2(o.inner_tol, s_r2, o.tol, bb, rr64);
-            float q = spmv32(Gv.zb, z);
+            float q = spmv_e(Gv.zb, r);             // A z = r + E z
             double s_pq = (double)p*q, z1 = 0, z2 = 0;
             gsum3(s_pq, z1, z2);
             double pq = s_pq;
@@ -1125,7 +1118,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
                 if (s_rr2 <= stop2) { it++; break; }
                 const float beta = (float)(s_rz2/rz);
                 rz = s_rz2;
-                float wz = spmv32(Gv.zb, z);
+                float wz = spmv_e(Gv.zb, r);
                 p = z+beta*p;
                 q = wz+beta*q;
                 double s_pq2 = (double)p*q, z4 = 0, z5 = 0;
