@@ -97,10 +97,13 @@ typedef struct lfd_solve_opts {
 	                           independent and rate proportional parts of stress control, the four
 	                           parts of a stress evaluation): 0 = two, 1 = one */
 	int agg_waves;          /* LFD_PRECOND_AGGREGATE: waves of 8 particles per workgroup of the solver,
-	                           8 or 12 (0 = 8).  With 8 (two wavefronts per SIMD) every lane keeps
+	                           8, 12 or 16 (0 = 8).  With 8 (two wavefronts per SIMD) every lane keeps
 	                           its row of the wave's 48x48 inverse in registers; with 12 the
 	                           inverses are staged in LDS and one launch takes 1.5x the waves
-	                           (fewer cooperative launches for a large batch). */
+	                           (fewer cooperative launches for a large batch).  16: every instance
+	                           takes the large-instance form (inverses in LDS, only the couplings
+	                           between waves staged, two right-hand sides per launch), which
+	                           instances above 8192 particles take whatever is asked. */
 } lfd_solve_opts;
 
 typedef struct lfd_solve_stats {

@@ -42,6 +42,9 @@ SYSTEM_CANDIDATES = [
     ("aggregate, regroup 4, inner 3e-3", dict(precond=1, regroup_interval=4, inner_tol=3e-3)),
     # 12 waves per solver workgroup (inverses in LDS, fewer workgroups)
     ("aggregate, regroup 4, 12 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=12)),
+    # 16 waves per workgroup (the large-instance form: couplings between waves
+    # only, both right-hand sides of stress control in one launch)
+    ("aggregate, regroup 4, 16 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=16)),
     ("block-jacobi", dict(precond=0)),
     ("block-jacobi, inner 3e-3", dict(precond=0, inner_tol=3e-3)),
     # blocks streamed from HBM every iteration (no LDS staging), with the
@@ -76,6 +79,7 @@ BATCH_CANDIDATES = [
     ("block-jacobi", dict(precond=0)),
     ("aggregate, regroup 4", dict(precond=1, regroup_interval=4)),
     ("aggregate, regroup 4, 12 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=12)),
+    ("aggregate, regroup 4, 16 waves per workgroup", dict(precond=1, regroup_interval=4, agg_waves=16)),
 ]
 
 

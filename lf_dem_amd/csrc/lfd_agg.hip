@@ -9,8 +9,8 @@
 // coupled particles instead removes most of them (about 0.65x the iterations).
 //
 // Pipeline, per regrouping (every few steps of an instance) and per assembly:
-//   k_agg_weight     coupling weight of every pair from its FP64 blocks;
-//   k_agg_adjacency  each particle's 8 strongest couplings, sorted;
+//   k_agg_adjacency  each particle's 8 strongest couplings (weights from the
+//                    pairs' FP64 blocks), sorted;
 //   k_agg_group      one workgroup per instance, everything in LDS: three
 //                    levels of parallel matching (each group merges with its
 //                    mutually strongest neighbour group while the size stays
@@ -74,15 +74,6 @@ __global__ void k_agg_age(DevScalars* sc, int ninst)
     if (k < ninst && sc[k].group_age >= 0) sc[k].group_age++;
 }
 
-__global__ void k_agg_weight(BsrView A, const int* __restrict__ inst_of, const DevScalars* __restrict__ scs,
-                             int interval, unsigned* __restrict__ wq, int pcap)
-{
-    const int k = blockIdx.x*blockDim.x+threadIdx.x;
-    if (k >= pcap || k >= *A.npairs) return;
-    if (!agg_due(scs+(inst_of ? inst_of[A.p0[k]] : 0), interval)) return;
-    wq[k] = agg_weight_q(A, k);
-}
-
 // Positions of the particles of chunked instances that are due: their places
 // in the cell bins (cells in label order, members in ascending particle
 // order), counted from the instance's particle offset.
@@ -142,9 +133,9 @@ __global__ void k_agg_adjacency(BsrView A, AggView G, const int* __restrict__ in
     };
     for (int e = A.tptr[i]; e < A.tptr[i+1]; e++) {
         const int k = A.tidx[e];
-        offer(G.wq[k], A.p0[k]);
+        offer(agg_weight_q(A, k), A.p0[k]);
     }
-    for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) offer(G.wq[k], A.p1[k]);
+    for (int k = A.row_ptr[i]; k < A.row_ptr[i+1]; k++) offer(agg_weight_q(A, k), A.p1[k]);
     uint4* out = reinterpret_cast<uint4*>(G.adj+(size_t)AG_K*row);
 #pragma unroll
     for (int j = 0; j < AG_K/4; j++) out[j] = make_uint4(top[4*j], top[4*j+1], top[4*j+2], top[4*j+3]);
@@ -488,12 +479,12 @@ void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, in
                       const int* home_start, const int* home_mem, int ncell, hipStream_t s)
 {
     if (njobs <= 0) return;
-    if (home_start && inst_of)
+    const bool chunked = home_start && inst_of;
+    if (chunked)
         k_agg_order<<<(nslots+255)/256, 256, 0, s>>>(home_start, home_mem, ncell, inst_of, IC, sc, interval, G);
-    k_agg_weight<<<(G.pcap+255)/256, 256, 0, s>>>(A, inst_of, sc, interval, G.wq, G.pcap);
     k_agg_adjacency<<<(nslots+255)/256, 256, 0, s>>>(A, G, inst_of, IC, sc, interval, nslots);
     k_agg_group<<<njobs, AGB_THREADS, 0, s>>>(A, G, jobs, sc, interval);
-    if (inst_of) k_agg_done<<<(ninst+63)/64, 64, 0, s>>>(sc, G.chunk, ninst, interval);
+    if (chunked) k_agg_done<<<(ninst+63)/64, 64, 0, s>>>(sc, G.chunk, ninst, interval);
 }
 
 void launch_agg_age(DevScalars* sc, int ninst, hipStream_t s)

@@ -110,8 +110,6 @@ struct AggView {
     const int* wave_inst;  // [solver waves] owning instance, -1 when free
     const int* wave_local; // [solver waves] wave index within the owning instance
     unsigned* adj;         // [AGG_ADJ_K * slots] per particle: strongest couplings as (weight << 14 | partner)
-    unsigned* wq;          // [pair capacity] quantised coupling weight of each pair
-    int pcap;
     float* inv;            // [solver waves * AGG_PACK] wave block inverse, packed lower triangle (FP32)
     float* zb;             // [6 * solver slots] exchanged Krylov vectors
     float* qb;
@@ -274,7 +272,8 @@ int lds_solver_max_groups();
 // Grouping of the instances in jobs that are due (never grouped, or grouped
 // `interval` or more steps ago); inst_of / IC null: one instance at slot 0.
 // Chunked instances take their cell order from the cell bins (home_start
-// over ncell cells, home_mem), which they need (null otherwise).
+// over ncell cells, home_mem): pass them when some instance of the batch is
+// chunked, null otherwise.
 void launch_agg_group(const BsrView& A, const AggView& G, const AggJob* jobs, int njobs, const int* inst_of,
                       const InstConst* IC, DevScalars* sc, int interval, int nslots, int ninst,
                       const int* home_start, const int* home_mem, int ncell, hipStream_t s);

@@ -102,7 +102,8 @@ static SolveOpts to_opts(const lfd_solve_opts* o)
     so.tol = o->tol; so.inner_tol = o->inner_tol; so.max_outer = o->max_outer; so.max_inner = o->max_inner;
     so.lanes_per_row = o->lanes_per_row; so.block = 256; so.groups = o->groups; so.launch = o->launch;
     so.max_rhs = o->max_rhs == 1 ? 1 : 2;
-    so.agg_waves = o->agg_waves == AGG_WAVES_WIDE ? AGG_WAVES_WIDE : AGG_WAVES_REG;
+    so.agg_waves = o->agg_waves == AGG_WAVES_WIDE ? AGG_WAVES_WIDE
+                 : o->agg_waves == AGG_LARGE_WAVES ? AGG_LARGE_WAVES : AGG_WAVES_REG;
     return so;
 }
 
@@ -200,8 +201,6 @@ struct Solver {
             const int nwv = scap/AGG_CAP;
             A(dalloc(&agg.perm, scap)); A(dalloc(&agg.iperm, n)); A(dalloc(&agg.waves, std::max(icap, rb)+1));
             A(dalloc(&wave_inst, nwv)); A(dalloc(&wave_local, nwv)); A(dalloc(&agg.adj, AGG_ADJ_K*(size_t)n));
-            A(dalloc(&agg.wq, pcap));
-            agg.pcap = pcap;
             A(dalloc(&agg.inv, (size_t)nwv*AGG_PACK));
             // zb, qb: two right-hand sides of the large-instance solver
             A(dalloc(&agg.zb, 12*(size_t)scap)); A(dalloc(&agg.qb, 12*(size_t)scap));
@@ -238,7 +237,7 @@ struct Solver {
     {
         void* ptrs[] = {diag64, diag32, minv64, minv32, od64, od32, p0, p1, tidx, row_ptr, tptr,
                         w.r64, w.x64, w.r, w.z, w.p, w.q, w.w, w.d, w.pr, w.pz, w.pp, w.pq, w.partials,
-                        w.barrier, jobs, agg.perm, agg.iperm, agg.waves, wave_inst, wave_local, agg.adj, agg.wq, agg.inv, agg.zb,
+                        w.barrier, jobs, agg.perm, agg.iperm, agg.waves, wave_inst, wave_local, agg.adj, agg.inv, agg.zb,
                         agg.qb, agg.db, agg.dsum[0], agg.dsum[1], agg.r0buf, agg.part, agg_jobs, ajobs,
                         agg.wexact, agg.chunk, agg.order, agg.rank};
         for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -279,19 +278,25 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     S.single = spans.size() == 1 ? spans[0].inst : -1;
     if (spans.empty()) return LFD_OK;
     const int maxg = std::min(lds_solver_max_groups(), S.w.max_groups);
-    const int aw = S.opts.agg_waves;
-    const int amaxg = S.scap > 0 ? agg_solver_max_groups(aw) : 0;
+    // waves per workgroup of the 8- or 12-wave aggregate solver (16 asks for
+    // the large-instance form, which solve_agg runs as 8 where it is not
+    // available)
+    const int aw = S.opts.agg_waves == AGG_WAVES_WIDE ? AGG_WAVES_WIDE : AGG_WAVES_REG;
+    // agg_waves 16: every aggregate instance takes the large-instance form
+    // (where it is available; otherwise the 8-wave form)
+    const bool all_large = S.opts.agg_waves == AGG_LARGE_WAVES && S.allow_large;
+    const int amaxg = S.scap > 0 && !all_large ? agg_solver_max_groups(aw) : 0;
     const int lmaxg = S.scap > 0 && S.allow_large ? agg_large_max_groups() : 0;
     std::vector<SolveJob> jobs, ajobs, ljobs;
     std::vector<AggJob> gjobs;
     std::fill(S.h_wave_inst.begin(), S.h_wave_inst.end(), -1);
-    int first = 0, afirst = 0;
+    int first = 0, afirst = 0, lfirst = 0;
     for (const InstSpan& sp : spans) {
         const int m = (sp.n+LDS_ROWS-1)/LDS_ROWS;
         const int bw = m*AGG_SLOTS_PER_BLOCK/AGG_CAP;
         const int bg = (bw+aw-1)/aw;
         if (S.scap > 0 && S.precond == LFD_PRECOND_AGGREGATE && S.kind == LFD_SOLVER_MIXED && sp.n <= AGG_NMAX &&
-            bg <= amaxg) {
+            !all_large && bg <= amaxg) {
             const int soff = sp.row0/LDS_ROWS*AGG_SLOTS_PER_BLOCK;
             if ((int)ajobs.size()-afirst+bg > amaxg) {
                 S.agg_launches.emplace_back(afirst, (int)ajobs.size()-afirst);
@@ -314,10 +319,11 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
         }
         if (S.scap > 0 && S.precond == LFD_PRECOND_AGGREGATE && S.kind == LFD_SOLVER_MIXED && lmaxg > 0 &&
             sp.inst < (int)S.h_chunk.size()) {
-            // a large instance: grouped in spatial chunks of at most AGG_NMAX
-            // particles (cell order), each packed into its own budget of waves;
-            // the instance's waves are about 4% more than its particles / 8
-            // (packing with whole groups), at most what one launch holds
+            // a large instance (or any, with agg_waves 16): grouped in spatial
+            // chunks of at most AGG_NMAX particles (cell order), each packed
+            // into its own budget of waves; the instance's waves are about 4%
+            // more than its particles / 8 (packing with whole groups), at most
+            // what one launch holds.  Instances share a launch while it fits.
             const int nch = (sp.n+AGG_NMAX-1)/AGG_NMAX, cs = (sp.n+nch-1)/nch;
             std::vector<int> bud(nch);
             int need = 0;
@@ -331,7 +337,10 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
                 for (int c = 0, spare = W-need; spare > 0; c = (c+1)%nch, spare--) bud[c]++;
                 const int soff = sp.row0/LDS_ROWS*AGG_SLOTS_PER_BLOCK;
                 const int Gl = (W+AGG_LARGE_WAVES-1)/AGG_LARGE_WAVES;
-                S.large_launches.emplace_back((int)ljobs.size(), Gl);
+                if ((int)ljobs.size()-lfirst+Gl > lmaxg) {
+                    S.large_launches.emplace_back(lfirst, (int)ljobs.size()-lfirst);
+                    lfirst = (int)ljobs.size();
+                }
                 for (int g = 0; g < Gl; g++) {
                     SolveJob j{};
                     j.inst = sp.inst; j.lwg = g; j.G = Gl; j.row0 = soff; j.row_end = soff+AGG_CAP*W;
@@ -370,6 +379,7 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     }
     if ((int)jobs.size() > first) S.launches.emplace_back(first, (int)jobs.size()-first);
     if ((int)ajobs.size() > afirst) S.agg_launches.emplace_back(afirst, (int)ajobs.size()-afirst);
+    if ((int)ljobs.size() > lfirst) S.large_launches.emplace_back(lfirst, (int)ljobs.size()-lfirst);
     // the large instances' jobs follow the aggregate solver's
     for (auto& L : S.large_launches) L.first += (int)ajobs.size();
     ajobs.insert(ajobs.end(), ljobs.begin(), ljobs.end());
@@ -815,8 +825,10 @@ static void bat_assemble(lfd_batch* B, int nparts = 0)
     // the wave inverses of this matrix
     if (B->S.n_agg_jobs > 0) {
         const BsrView A = B->S.view(&B->bs->npairs);
+        const bool chunked = std::any_of(B->S.h_chunk.begin(), B->S.h_chunk.end(), [](int c) { return c > 0; });
         launch_agg_group(A, B->S.agg, B->S.agg_jobs, B->S.n_agg_jobs, B->inst_of, B->d_ic, B->sc,
-                         B->S.regroup_interval, B->M, B->ninst, B->home_start, B->home_mem, B->ccap, B->s);
+                         B->S.regroup_interval, B->M, B->ninst, chunked ? B->home_start : nullptr,
+                         chunked ? B->home_mem : nullptr, B->ccap, B->s);
         launch_agg_inverse(A, B->S.agg, B->S.scap/AGG_CAP, B->s);
     }
 }

@@ -9,7 +9,10 @@ particles (chunked grouping, 16-wave solver with Eisenstat's product).
   device's own blocks) and equal the 6x6 block-Jacobi solver's to 1e-9; a few
   steps later the trajectories still agree;
 * steps replayed from captured step graphs equal eager steps bit for bit;
-* a large instance in a batch with a small one: each equal to its lone run.
+* a large instance in a batch with a small one: each equal to its lone run;
+* the same form asked for on small instances (agg_waves = 16): batch equal
+  to lone runs, t=0 velocities within 1e-8 of the oracle's direct solve, two
+  right-hand sides per launch equal to one per launch within 1e-9.
 """
 import numpy as np
 import pytest
@@ -124,3 +127,51 @@ def test_large_instance_in_a_batch_equals_its_lone_run(c9k, capi):
         a, b = B.state(ids[k]), S.state()
         for key in ("pos", "vel", "ang_vel", "na_vel", "na_ang_vel", "shear_disp"):
             assert np.array_equal(a[key], b[key]), (k, key)
+
+
+def test_sixteen_wave_form_on_small_instances(capi, oracle):
+    """agg_waves = 16 puts instances of any size on the large-instance form:
+    a batch of three equals their lone runs bit for bit, the velocities at
+    t=0 are within 1e-8 of the oracle's FP64 direct solve, and under stress
+    control both right-hand sides in one launch agree with one per launch."""
+    confs = [conf(500, 0.50, 11), conf(1500, 0.56, 12), conf(3000, 0.55, 13)]
+    opts = capi.default_opts(precond=1, agg_waves=16)
+    slots = cells = 0
+    for p, r, l in confs:
+        sl, ce, _ = capi.footprint(len(r), r, l)
+        slots += sl
+        cells += ce
+    B = capi.Batch(slots, cells, len(confs), opts=opts)
+    ids = [B.add(p, r, l) for p, r, l in confs]
+    B.step(3)
+    B.sync()
+    for k, (p, r, l) in enumerate(confs):
+        S = capi.System(p, r, l, opts=opts)
+        S.first_update()
+        S.step(3)
+        assert S.precond_waves() > 0
+        a, b = B.state(ids[k]), S.state()
+        for key in ("pos", "vel", "ang_vel", "na_vel", "na_ang_vel", "shear_disp"):
+            assert np.array_equal(a[key], b[key]), (k, key)
+    p, r, l = confs[1]
+    S = capi.System(p, r, l, opts=opts)
+    R = oracle.System(p, r, l)
+    S.first_update()
+    R.first_update()
+    S.compute_velocities(True)
+    R.set_in_predictor(True)
+    R.compute_velocities()
+    for key in ("na_vel", "na_ang_vel"):
+        assert _rel(S.state()[key], R.state()[key]) <= 1e-8, key
+    out = []
+    for m in (0, 1):
+        S = capi.System(p, r, l, opts=capi.default_opts(precond=1, agg_waves=16, max_rhs=m))
+        S.first_update()
+        if m == 0:
+            S.compute_stress(False)
+            target = S.stress()["total"][2]
+        S.set_control("stress", target)
+        S.step(2)
+        out.append(S.state())
+    for key in ("na_vel", "na_ang_vel"):
+        assert np.abs(out[0][key] - out[1][key]).max() <= 1e-9 * np.abs(out[1][key]).max(), key
