@@ -127,7 +127,7 @@ def run(args):
     pos, rad, L = workloads.generate(args.n, args.phi, args.seed)
     gen_s = time.time() - t0
     params = capi.default_params()
-    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond, agg_waves=args.agg_waves)
     tuned = None
     if args.autotune:
         from lf_dem_amd import autotune
@@ -243,7 +243,7 @@ def run_c5(args):
     print("configuration generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     params = capi.default_params()
     opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond,
-                             max_rhs=args.max_rhs)
+                             max_rhs=args.max_rhs, agg_waves=args.agg_waves)
     if args.inner_tol is not None:
         opts.inner_tol = args.inner_tol
 
@@ -334,7 +334,7 @@ def run_c3(args):
     confs = workloads.generate_many(specs)
     gen_s = time.time() - t0
     print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
-    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond, agg_waves=args.agg_waves)
     params = [capi.default_params(repulsion=s[3]) for s in specs]
     tuned = None
     if args.autotune:
@@ -431,7 +431,7 @@ def _serve_phase(args, rate, duration, drain, seed, pregen):
     gen_s = time.time() - t0
     print("%d of %d request configurations generated in %.1f s" % (k, len(arr), gen_s), file=sys.stderr,
           flush=True)
-    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond)
+    opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond, agg_waves=args.agg_waves)
     reqs = []
     for i, a in enumerate(arr):
         if i < k:
@@ -620,6 +620,9 @@ def main():
     ap.add_argument("--serve-pregen", type=int, default=96, help="c4: configurations prepared up front")
     ap.add_argument("--precond", type=int, default=None,
                     help="0: 6x6 block-Jacobi, 1: aggregate (default 1; c3: 0); the autotuner overrides it")
+    ap.add_argument("--agg-waves", type=int, default=None, choices=(8, 12, 16),
+                    help="aggregate preconditioner: waves of 8 particles per solver workgroup (default: the "
+                         "library's, 8; c4: 16); the autotuner overrides it")
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="c5: residual drop per FP32 inner solve (default: the library's)")
     ap.add_argument("--max-rhs", type=int, default=0,
@@ -634,6 +637,8 @@ def main():
     args = ap.parse_args()
     if args.precond is None:
         args.precond = 0 if args.workload == "c3" else 1
+    if args.agg_waves is None:
+        args.agg_waves = 16 if args.workload == "c4" else 0
     if args.gpus != 1:
         raise SystemExit("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)")
     if args.workload == "copy":

@@ -145,6 +145,7 @@ struct Solver {
     float *diag32 = nullptr, *od32 = nullptr, *minv32 = nullptr;
     int *p0 = nullptr, *p1 = nullptr, *row_ptr = nullptr, *tptr = nullptr, *tidx = nullptr;
     SolverWork w{};
+    size_t nbar = 0;          // tree-barrier words in w.barrier
     // batched LDS solver: job table and its cooperative launches (first job, count);
     // no launch means the single instance `single` is solved by the global-memory form
     SolveJob* jobs = nullptr;
@@ -191,7 +192,7 @@ struct Solver {
         w.max_groups = 2048;
         A(dalloc(&w.partials, std::max<size_t>(8*(size_t)w.max_groups, 12*(size_t)rb+64)));
         // tree-barrier words: 128 per row block for the batched solver, 16384 for the global form
-        const size_t nbar = std::max<size_t>(16384, 128*(size_t)rb+256);
+        nbar = std::max<size_t>(16384, 128*(size_t)rb+256);
         A(dalloc(&w.barrier, nbar));
         A(hipMemset(w.barrier, 0, nbar*sizeof(unsigned)));
         jobs_cap = rb+1;
@@ -385,6 +386,12 @@ static int set_jobs(Solver& S, const std::vector<InstSpan>& spans, hipStream_t s
     ajobs.insert(ajobs.end(), ljobs.begin(), ljobs.end());
     if ((int)jobs.size() > S.jobs_cap || (int)ajobs.size() > S.ajobs_cap)
         return fail(LFD_ERR_CAPACITY, "solver job table capacity exceeded");
+    // An instance's barrier words start at its first row block and may run
+    // into the next blocks (one leaf counter per 8 workgroups): a layout of
+    // the instances other than the last one's leaves other instances' words,
+    // generation counters among them, where the new one keeps its counters.
+    // All start again from zero.
+    launch_fill_int(reinterpret_cast<int*>(S.w.barrier), 0, S.nbar, s);
     if (!jobs.empty())
         HIPCHK(hipMemcpyAsync(S.jobs, jobs.data(), jobs.size()*sizeof(SolveJob), hipMemcpyHostToDevice, s));
     if (!ajobs.empty()) {

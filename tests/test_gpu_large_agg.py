@@ -175,3 +175,31 @@ def test_sixteen_wave_form_on_small_instances(capi, oracle):
         out.append(S.state())
     for key in ("na_vel", "na_ang_vel"):
         assert np.abs(out[0][key] - out[1][key]).max() <= 1e-9 * np.abs(out[1][key]).max(), key
+
+
+def test_new_instance_over_a_retired_ones_barrier_words(capi):
+    """A batch whose layout changes: P (4 row blocks), Q and R after it; R
+    steps (its barrier generation word advances), Q and R retire, and S,
+    larger, is placed right after P, where its barrier leaf counters fall on
+    R's old words.  S must solve (every barrier word starts from zero when
+    the job table is rebuilt) and equal its lone run."""
+    opts = capi.default_opts(precond=1, agg_waves=16)
+    P, Q, R, S = conf(500, 0.52, 21), conf(100, 0.50, 22), conf(2000, 0.55, 23), conf(5000, 0.55, 24)
+    fp = [capi.footprint(len(c[1]), c[1], c[2]) for c in (P, Q, R, S)]
+    slots = fp[0][0] + fp[1][0] + fp[2][0] + fp[3][0]
+    cells = fp[0][1] + fp[1][1] + fp[2][1] + fp[3][1]
+    B = capi.Batch(slots, cells, 4, opts=opts)
+    ip, iq, ir = (B.add(*c) for c in (P, Q, R))
+    B.step(3)
+    B.sync()
+    B.remove(iq)
+    B.remove(ir)
+    i_s = B.add(*S)
+    B.step(3)
+    B.sync()
+    lone = capi.System(*S, opts=opts)
+    lone.first_update()
+    lone.step(3)
+    a, b = B.state(i_s), lone.state()
+    for key in ("pos", "vel", "ang_vel", "na_vel", "na_ang_vel", "shear_disp"):
+        assert np.array_equal(a[key], b[key]), key
