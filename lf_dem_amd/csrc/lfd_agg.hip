@@ -835,7 +835,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             t0 = tag_partial(wave_sum_d(t0), tg);
             t1 = tag_partial(wave_sum_d(t1), tg);
             t2 = tag_partial(wave_sum_d(t2), tg);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_for_records();
             if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             double t = 0;
             for (int q = 0; q < AS_WAVES; q++) t += red8[8*q+m];
             t = tag_partial(t, tg);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_for_records();
             if (lane < 8) st_dev(pb+8*wg+m, t);
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
                 t[k] = lane < AL_WAVES ? red[NV*lane+k] : 0.0;
                 t[k] = tag_partial(wave_sum_d(t[k]), tg);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_for_records();
             if (lane == 0)
                 for (int k = 0; k < NV; k++) st_dev(pb+NV*wg+k, t[k]);
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);

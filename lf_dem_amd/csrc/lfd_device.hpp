@@ -195,6 +195,18 @@ __device__ inline void drain_vmem()
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
+// Release at agent scope ahead of relaxed record stores.  The fence writes
+// back this L2's dirty lines (the exchanged vectors that other workgroups read
+// once they see the records: other XCDs' L2s do not snoop this one), and the
+// record stores must wait for that write-back to complete.  The compiler
+// emits the fence's write-back (buffer_wbl2) but drops the wait for it when
+// no other vector memory operation is outstanding, so a record could reach
+// another XCD before the data it publishes; the explicit wait restores it.
+__device__ inline void release_for_records()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    drain_vmem();
+}
 // A partial sum with the low 8 mantissa bits replaced by a round tag
 // (relative perturbation below 3e-14).  Aligned 64-bit stores and loads are
 // single-copy atomic, so a record read with the expected tag is that round's.

@@ -2030,7 +2030,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
             t0 = tag_partial(wave_sum_d(t0), tg);
             t1 = tag_partial(wave_sum_d(t1), tg);
             t2 = tag_partial(wave_sum_d(t2), tg);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_for_records();
             if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
@@ -2388,7 +2388,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                 t[k] = lane < LS_WAVES ? red[NV*lane+k] : 0.0;
                 t[k] = tag_partial(wave_sum_d(t[k]), tg);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_for_records();
             if (lane == 0)
                 for (int k = 0; k < NV; k++) st_dev(pb+NV*wg+k, t[k]);
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
