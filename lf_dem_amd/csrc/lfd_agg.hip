@@ -227,11 +227,10 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
     __shared__ short nxt[AGG_NMAX];            // member lists
     __shared__ short gbin[AGG_NMAX];           // wave of a group (at the leader)
     __shared__ unsigned char goff[AGG_NMAX];   // first position of the group in its wave
-    __shared__ unsigned long long sc_lds[2*AGB_THREADS/WAVE+2];
+    __shared__ unsigned long long sc_lds[2*AGB_THREADS/WAVE];
     __shared__ unsigned run_comp[AG_RMAX];     // per run: count of each size class in one wave (nibbles)
     __shared__ int run_rep[AG_RMAX], run_bin0[AG_RMAX];
-    __shared__ int s_nruns, s_bins, s_fallback, s_ndet;
-    __shared__ int s_split[9];
+    __shared__ int s_nruns, s_bins, s_fallback;
     const AggJob J = jobs[blockIdx.x];
     if (!agg_due(scs+J.inst, interval)) return;
     const int n = J.n, poff = J.poff;
@@ -310,82 +309,67 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
     for (int l = tid; l < n; l += T) nxt[l] = (short)atomicExch(&head[grp[l]], l);
     __syncthreads();
     // ---- counts and ranks of the groups by size, in leader order
-    unsigned long long lo = 0, hi = 0, plo = 0, phi = 0;
-    auto count_groups = [&]() {
-        lo = hi = 0;
-        for (int l = c0; l < c1; l++)
-            if (grp[l] == l) {
-                const int s = gsz[l];
-                if (s <= 4) lo += 1ull << (16*(s-1));
-                else hi += 1ull << (16*(s-5));
-            }
-        plo = lo; phi = hi;
-        block_scan2(plo, phi, sc_lds);
-        if (tid == T-1) { sc_lds[2*AGB_THREADS/WAVE] = plo; sc_lds[2*AGB_THREADS/WAVE+1] = phi; }
-        plo -= lo; phi -= hi;                  // exclusive prefix of this chunk
-        __syncthreads();
-    };
-    count_groups();
-    if (tid == 0) {
-        int cnt0[9], cnt[9];
-        for (int s = 1; s <= 8; s++) cnt[s] = cnt0[s] = cls_count(sc_lds[2*AGB_THREADS/WAVE], sc_lds[2*AGB_THREADS/WAVE+1], s);
-        int nr = 0;
-        int bins = bfd_counts(cnt, run_comp, run_rep, run_bin0, nr);
-        // A chunk of a large instance must fit its budget: members of the
-        // largest groups below 8 are set apart as single particles, which
-        // fill any wave, until best-fit decreasing fits (decided on the
-        // counts, then done by the leaders)
-        int ndet = 0;
-        for (int t = 0; t <= 8; t++) s_split[t] = 0;
-        if (J.chunked) {
-            int sim[9], avail[9];
-            for (int t = 0; t <= 8; t++) { sim[t] = cnt0[t]; avail[t] = cnt0[t]; }
-            for (int guard = 0; guard < 64 && (bins < 0 || bins > J.budget); guard++) {
-                int t = 7;
-                while (t >= 2 && avail[t] == 0) t--;
-                if (t < 2) break;
-                const int k = min(avail[t], max(4, 2*(bins < 0 ? 8 : bins-J.budget)));
-                avail[t] -= k; sim[t] -= k; sim[t-1] += k; sim[1] += k;
-                s_split[t] += k;
-                ndet += k;
-                for (int q = 0; q <= 8; q++) cnt[q] = sim[q];
-                bins = bfd_counts(cnt, run_comp, run_rep, run_bin0, nr);
-            }
+    unsigned long long lo = 0, hi = 0;
+    for (int l = c0; l < c1; l++)
+        if (grp[l] == l) {
+            const int s = gsz[l];
+            if (s <= 4) lo += 1ull << (16*(s-1));
+            else hi += 1ull << (16*(s-5));
         }
-        s_ndet = ndet;
+    unsigned long long plo = lo, phi = hi;
+    block_scan2(plo, phi, sc_lds);
+    if (tid == T-1) { sc_lds[0] = plo; sc_lds[1] = phi; }
+    plo -= lo; phi -= hi;                      // exclusive prefix of this chunk
+    __syncthreads();
+    if (tid == 0) {
+        int cnt[9];
+        for (int s = 1; s <= 8; s++) cnt[s] = cls_count(sc_lds[0], sc_lds[1], s);
+        int nr = 0;
+        const int bins = bfd_counts(cnt, run_comp, run_rep, run_bin0, nr);
         s_nruns = nr;
         s_bins = bins;
-        s_fallback = bins < 0 || bins > J.budget;
+        // more waves than the budget: the last ones are dissolved below; only
+        // a full run table, or a budget that cannot hold the particles at
+        // all, falls back to consecutive particles
+        s_fallback = bins < 0 || J.budget*AGG_CAP < n;
     }
     __syncthreads();
-    if (s_ndet > 0 && !s_fallback) {
-        // the first s_split[s] groups of size s (leader order) give up one
-        // member: decided on the grouping as it is, then done
-        int det[AG_CH];
-        int nd = 0, run_c[9];
-        for (int t = 0; t <= 8; t++) run_c[t] = 0;
-        for (int l = c0; l < c1; l++) {
-            if (grp[l] != l) continue;
-            const int sz = gsz[l];
-            const int r = cls_count(plo, phi, sz)+run_c[sz];
-            run_c[sz]++;
-            if (sz >= 2 && sz <= 7 && r < s_split[sz] && nd < AG_CH) det[nd++] = l;
+    const int kept = min(s_bins, J.budget);
+    // particles in wave b (from the run table)
+    auto wave_fill = [&](int b) {
+        for (int q = 0; q < s_nruns; q++)
+            if (b >= run_bin0[q] && b < run_bin0[q]+run_rep[q]) {
+                int f = 0;
+                for (int t = 1; t <= 8; t++) f += t*(int)((run_comp[q] >> (4*(t-1))) & 0xF);
+                return f;
+            }
+        return 0;
+    };
+    // particles of the waves [b0, b1), summed run by run
+    auto fill_between = [&](int b0, int b1) {
+        int acc = 0;
+        for (int q = 0; q < s_nruns; q++) {
+            const int lo_b = max(b0, run_bin0[q]), hi_b = min(b1, run_bin0[q]+run_rep[q]);
+            if (hi_b <= lo_b) continue;
+            int f = 0;
+            for (int t = 1; t <= 8; t++) f += t*(int)((run_comp[q] >> (4*(t-1))) & 0xF);
+            acc += (hi_b-lo_b)*f;
         }
-        __syncthreads();
-        for (int q = 0; q < nd; q++) {
-            const int l = det[q];
-            int m = head[l];
-            if (m == l) { m = nxt[l]; nxt[l] = nxt[m]; }
-            else head[l] = nxt[m];
-            grp[m] = (short)m;
-            gsz[m] = 1;
-            gsz[l] = (unsigned char)(gsz[l]-1);
-            head[m] = m;
-            nxt[m] = -1;
+        return acc;
+    };
+    // the h-th free position of the kept waves, in wave order: 8 wave + pos
+    auto hole_slot = [&](int h) {
+        for (int q = 0; q < s_nruns; q++) {
+            const int b0 = run_bin0[q], nb = min(run_rep[q], kept-b0);
+            if (nb <= 0) continue;
+            int f = 0;
+            for (int t = 1; t <= 8; t++) f += t*(int)((run_comp[q] >> (4*(t-1))) & 0xF);
+            const int per = AGG_CAP-f;
+            if (h < per*nb) return AGG_CAP*(b0+h/per)+f+h%per;
+            h -= per*nb;
         }
-        __syncthreads();
-        count_groups();
-    }
+        return -1;
+    };
     if (!s_fallback) {
         // wave and position of every group from its size class and rank
         int run_c[9];
@@ -413,7 +397,7 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
                 }
                 acc += here;
             }
-            if (gbin[l] < 0 || gbin[l] >= J.budget || goff[l]+s > AGG_CAP) atomicOr(&s_fallback, 1);
+            if (gbin[l] < 0 || goff[l]+s > AGG_CAP) atomicOr(&s_fallback, 1);
         }
     }
     __syncthreads();
@@ -421,7 +405,10 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
     // particle of local index l
     auto part = [&](int l) { return J.chunked ? G.order[poff+l] : poff+l; };
     // slots: every used slot and every padding slot of the waves in use is
-    // written exactly once (perm) and every particle gets its slot (iperm)
+    // written exactly once (perm) and every particle gets its slot (iperm).
+    // The members of the waves beyond the budget (the last ones of the
+    // packing, which hold the smallest groups) move one by one into the free
+    // positions of the kept waves, in wave order.
     for (int l = tid; l < n; l += T) {
         if (fallback) {
             const int p = part(l);
@@ -438,26 +425,26 @@ __global__ __launch_bounds__(AGB_THREADS) void k_agg_group(BsrView A, AggView G,
             while (b >= 0 && mem[b] > v) { mem[b+1] = mem[b]; b--; }
             mem[b+1] = v;
         }
-        const int base = J.soff+8*gbin[l]+goff[l];
+        const int gb = gbin[l];
+        const int moved0 = gb >= kept ? fill_between(kept, gb)+goff[l] : 0;
         for (int a = 0; a < m; a++) {
             const int p = part(mem[a]);
-            G.iperm[p] = base+a;
-            G.perm[base+a] = p;
+            const int sl = gb < kept ? AGG_CAP*gb+goff[l]+a : hole_slot(moved0+a);
+            G.iperm[p] = J.soff+sl;
+            G.perm[J.soff+sl] = p;
         }
     }
-    const int used = fallback ? (n+AGG_CAP-1)/AGG_CAP : s_bins;
+    const int used = fallback ? (n+AGG_CAP-1)/AGG_CAP : kept;
     if (fallback) {
         for (int q = n+tid; q < AGG_CAP*used; q += T) G.perm[J.soff+q] = -1;
     } else {
-        for (int b = tid; b < s_bins; b += T) {
-            int fill = AGG_CAP;
-            for (int q = 0; q < s_nruns; q++)
-                if (b >= run_bin0[q] && b < run_bin0[q]+run_rep[q]) {
-                    fill = 0;
-                    for (int t = 1; t <= 8; t++) fill += t*(int)((run_comp[q] >> (4*(t-1))) & 0xF);
-                    break;
-                }
-            for (int q = fill; q < AGG_CAP; q++) G.perm[J.soff+AGG_CAP*b+q] = -1;
+        // the free positions of the kept waves beyond the moved particles
+        const int moved = fill_between(kept, s_bins);
+        for (int b = tid; b < kept; b += T) {
+            const int f = wave_fill(b);
+            const int h0 = AGG_CAP*b-fill_between(0, b);          // free positions before wave b
+            for (int q = f; q < AGG_CAP; q++)
+                if (h0+(q-f) >= moved) G.perm[J.soff+AGG_CAP*b+q] = -1;
         }
     }
     // a chunk's waves beyond those it uses are empty (the solver of a
