@@ -621,7 +621,7 @@ void launch_agg_inverse(const BsrView& A, const AggView& G, int nwaves, hipStrea
 static constexpr int AS_LPR = 8;
 static constexpr int AS_ENTRY_CAP = 1024;           // staged blocks per workgroup (78 KB)
 static constexpr int AS_POLL_ROUNDS = 3;            // records per lane per poll of the pipelined sums
-static constexpr int AS_POLL_ROUNDS8 = 12;          // entries per lane per poll of the 8-value sums
+static constexpr int AS_POLL_ROUNDS8 = 12;          // workgroups per lane per poll of the 8-value sums
 
 struct AsRow { int e0, ne; };       // staged entries of a slot row (ne < 0: not staged)
 
@@ -806,13 +806,14 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
     };
 
     double* part = Gv.part+AGG_PART_PER_BLOCK*(size_t)J.key;
+    const int rs3 = rec_stride(G, 3), rs8 = rec_stride(G, 8);   // record strides of the 3- and 8-value sums
     int buf = 0;
     auto publish3 = [&](double a, double bs, double cs) {
         a = wave_sum_d(a); bs = wave_sum_d(bs); cs = wave_sum_d(cs);
         if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bs; red[3*wid+2] = cs; }
         drain_vmem();
         __syncthreads();
-        double* pb = part+(size_t)buf*3*G;
+        double* pb = part+(size_t)buf*rs3*G;
         seq++;
         if (wid == 0) {
             double t0 = lane < AS_WAVES ? red[3*lane] : 0.0;
@@ -823,12 +824,14 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
             t1 = tag_partial(wave_sum_d(t1), tg);
             t2 = tag_partial(wave_sum_d(t2), tg);
             release_for_records();
-            if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
+            if (lane == 0) {
+                st_dev(pb+rs3*wg, t0); st_dev(pb+rs3*wg+1, t1); st_dev(pb+rs3*wg+2, t2);
+            }
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
                 double ac[3] = {0, 0, 0};
-                const bool dn = poll_records<3, AS_POLL_ROUNDS>(pq, G, tg, lane, ac);
+                const bool dn = poll_records<3, AS_POLL_ROUNDS>(pq, G, rs3, tg, lane, ac);
                 if (__all(dn)) {
                     const double a0 = wave_sum_d(ac[0]), a1 = wave_sum_d(ac[1]), a2 = wave_sum_d(ac[2]);
                     if (lane == 0) { red[3*AS_WAVES] = a0; red[3*AS_WAVES+1] = a1; red[3*AS_WAVES+2] = a2; }
@@ -870,20 +873,34 @@ __global__ __launch_bounds__(AS_WAVES*WAVE) void k_solve_agg(BsrView A, AggView 
         seq8++;
         if (wid == 0) {
             // lane L works on value L % 8 throughout: its workgroup total, then
-            // the records q = L / 8, L / 8 + 8, ... of the instance
-            double* pb = part+6*(size_t)G+(size_t)buf8*8*G;
+            // that value of the workgroups L / 8, L / 8 + 8, ... of the instance
+            // (a workgroup's record: 8 values at the start of its slot)
+            double* pb = part+2*rs3*(size_t)G+(size_t)buf8*rs8*G;
             const unsigned tg = seq8 & 0xFF;
             const int m = lane & 7;
             double t = 0;
             for (int q = 0; q < AS_WAVES; q++) t += red8[8*q+m];
             t = tag_partial(t, tg);
             release_for_records();
-            if (lane < 8) st_dev(pb+8*wg+m, t);
-            const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
+            if (lane < 8) st_dev(pb+rs8*wg+m, t);
+            const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb)+m;
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
                 double acc = 0;
-                const bool dn = poll_records<1, AS_POLL_ROUNDS8>(pq, 8*G, tg, lane, &acc);
+                bool dn = true;
+                for (int g0 = (int)(lane >> 3); g0 < G; g0 += 8*AS_POLL_ROUNDS8) {
+                    unsigned long long v[AS_POLL_ROUNDS8];
+#pragma unroll
+                    for (int r = 0; r < AS_POLL_ROUNDS8; r++) {
+                        const int g = g0+8*r;
+                        v[r] = g < G ? ld_dev(pq+(size_t)rs8*g) : (unsigned long long)tg;
+                    }
+#pragma unroll
+                    for (int r = 0; r < AS_POLL_ROUNDS8; r++) {
+                        dn &= (v[r] & 0xFF) == tg;
+                        acc += untag_partial(v[r]);
+                    }
+                }
                 if (__all(dn)) {
                     acc += __shfl_xor(acc, 8);
                     acc += __shfl_xor(acc, 16);

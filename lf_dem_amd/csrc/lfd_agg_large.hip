@@ -51,6 +51,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
     double* __restrict__ x, size_t rs, SolverWork w, DevScalars* scs, SolveOpts o, const SolveJob* __restrict__ jobs)
 {
     constexpr int NV = 3*NR;                         // values of one grid sum
+    static_assert(NV <= REC_STRIDE, "a record fits its slot");
     __shared__ int ent_j[AL_ENTRY_CAP];             // partner slot; ~slot for the transposed block
     __shared__ float ent_b[AL_ENTRY_CAP*18];
     __shared__ float s_inv[AL_WAVES*AGG_PACK];
@@ -205,6 +206,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
     };
 
     double* part = Gv.part+AGG_PART_PER_BLOCK*(size_t)J.key;
+    const int rst = rec_stride(G, NV);   // record stride (rs: the right-hand sides' stride)
     int buf = 0;
     // Grid sums of NV values: block sums (fixed order) published as tagged
     // records after every wave's vector stores have completed and a release
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
             for (int k = 0; k < NV; k++) red[NV*wid+k] = v[k];
         drain_vmem();
         __syncthreads();
-        double* pb = part+(size_t)buf*NV*G;
+        double* pb = part+(size_t)buf*rst*G;
         seq++;
         if (wid == 0) {
             const unsigned tg = seq & 0xFF;
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
             }
             release_for_records();
             if (lane == 0)
-                for (int k = 0; k < NV; k++) st_dev(pb+NV*wg+k, t[k]);
+                for (int k = 0; k < NV; k++) st_dev(pb+rst*wg+k, t[k]);
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(AL_BLOCK) void k_solve_agg_large(BsrView A, AggView
                 bool dn = true;
                 for (int q = (int)lane; q < G; q += WAVE) {
                     unsigned long long vq[NV];
-                    for (int k = 0; k < NV; k++) vq[k] = ld_dev(pq+NV*q+k);
+                    for (int k = 0; k < NV; k++) vq[k] = ld_dev(pq+(size_t)rst*q+k);
                     for (int k = 0; k < NV; k++) dn &= (vq[k] & 0xFF) == tg;
                     for (int k = 0; k < NV; k++) a[k] += untag_partial(vq[k]);
                 }

@@ -218,15 +218,23 @@ __device__ inline double untag_partial(unsigned long long v)
 {
     return __longlong_as_double((long long)(v & ~0xFFull));
 }
+// Distance in doubles between the partial records of K values of an
+// instance of G workgroups: one 256-byte slot each, or packed.
+__device__ inline int rec_stride(int G, int K)
+{
+    return G <= REC_SPREAD_MAX_G ? REC_STRIDE : K;
+}
 // One poll of tagged partial records of K values each (record q at
-// pq[K q .. K q + K)): lane L reads records L, L + 64, L + 128, ... below
+// pq[rs q .. rs q + K)): lane L reads records L, L + 64, L + 128, ... below
 // nrec, R rounds at a time with every load of the R rounds issued before any
 // is checked, so a poll of up to 64 R records waits for one memory latency
 // instead of one per round.  The untagged values are added into acc[0..K) in
 // record order; true when every record read carries tag tg.
 template <int K, int R>
-__device__ inline bool poll_records(const unsigned long long* pq, int nrec, unsigned tg, unsigned lane, double* acc)
+__device__ inline bool poll_records(const unsigned long long* pq, int nrec, int rs, unsigned tg, unsigned lane,
+                                    double* acc)
 {
+    static_assert(K <= REC_STRIDE, "a record fits its slot");
     bool done = true;
     for (int base = 0; base < nrec; base += R*64) {
         unsigned long long v[R][K];
@@ -234,7 +242,7 @@ __device__ inline bool poll_records(const unsigned long long* pq, int nrec, unsi
         for (int r = 0; r < R; r++) {
             const int q = base+64*r+(int)lane;
 #pragma unroll
-            for (int k = 0; k < K; k++) v[r][k] = q < nrec ? ld_dev(pq+(size_t)K*q+k) : (unsigned long long)tg;
+            for (int k = 0; k < K; k++) v[r][k] = q < nrec ? ld_dev(pq+(size_t)rs*q+k) : (unsigned long long)tg;
         }
 #pragma unroll
         for (int r = 0; r < R; r++)

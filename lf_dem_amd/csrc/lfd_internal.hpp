@@ -100,8 +100,18 @@ static constexpr int AGG_WAVES_WIDE = 12;          // the other choice of lfd_so
                                                    // inverses in LDS, more waves per launch
 static constexpr int AGG_ADJ_K = 8;               // strongest couplings kept per particle for the grouping
 static constexpr int AGG_NMAX = 8192;              // largest instance the grouping kernel takes
-static constexpr int AGG_PART_PER_BLOCK = 128;     // partial-record doubles per 128-slot block (22 per
-                                                   // workgroup, up to 2.5 workgroups per block plus one)
+// Partial-sum records of the persistent solvers: every workgroup of an
+// instance polls every record of it.  Records packed into a few cache lines
+// put all of those loads on a few memory channels; one 256-byte block per
+// record spreads them, which pays up to about a hundred workgroups (measured
+// per solve of k_solve_agg: -5.8% at 33 workgroups, -5.6% at 66, -3.2% at 97,
+// +1.0% at 132; +6.7% for k_solve_agg_large at 256 workgroups of six-value
+// records, where the spread records take 2.7x the cache lines per poll).
+static constexpr int REC_STRIDE = 32;              // doubles per record slot when spread
+static constexpr int REC_SPREAD_MAX_G = 112;       // spread up to this many workgroups, packed above
+static constexpr int AGG_PART_PER_BLOCK = 512;     // partial-record doubles per 128-slot block (four
+                                                   // buffers of REC_STRIDE per workgroup, up to 2.5
+                                                   // workgroups per block plus one)
 
 struct AggView {
     int* perm;             // [solver slots] particle slot, -1 for padding
@@ -262,9 +272,11 @@ hipError_t solve_mixed(const BsrView& A, const double* b, double* x, SolverWork&
 // Batched LDS-resident solver: one persistent launch of nwg workgroups
 // described by jobs[0..nwg) (all instances of the launch) for nrhs (1 or 2)
 // right-hand sides at b + k*rs (solutions at x + k*rs).  Partial records and
-// barrier words of an instance sit at 6*nrhs and 128 per row block from its
-// first row block (row0 / 128); the FP32 vectors w.z, w.q hold nrhs vectors.
+// barrier words of an instance sit at LDS_PART_PER_BLOCK and 128 per row block
+// from its first row block (row0 / 128); the FP32 vectors w.z, w.q hold nrhs
+// vectors.
 static constexpr int LDS_ROWS = 128;
+static constexpr int LDS_PART_PER_BLOCK = 2*REC_STRIDE;   // two buffers, one record per workgroup
 hipError_t solve_lds(const BsrView& A, const double* b, double* x, int nrhs, size_t rs, SolverWork& w,
                      DevScalars* sc, const SolveJob* jobs, int nwg, const SolveOpts& o, hipStream_t s);
 int lds_solver_max_groups();

@@ -2007,7 +2007,8 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
         return own ? z : 0.f;
     };
 
-    double* part = w.partials+6*(size_t)J.key;
+    double* part = w.partials+LDS_PART_PER_BLOCK*(size_t)J.key;
+    const int rs = rec_stride(G, 3);
     int buf = 0;
     // Grid sums of three values.  Block sums (DPP tree, fixed order) are
     // published as tagged partial records in the current buffer after every
@@ -2020,7 +2021,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
         if (lane == 0) { red[3*wid] = a; red[3*wid+1] = bs; red[3*wid+2] = cs; }
         drain_vmem();
         __syncthreads();
-        double* pb = part+(size_t)buf*3*G;
+        double* pb = part+(size_t)buf*rs*G;
         seq++;
         if (wid == 0) {
             double t0 = lane < LS_WAVES ? red[3*lane] : 0.0;
@@ -2031,14 +2032,17 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds_one(BsrView A, const dou
             t1 = tag_partial(wave_sum_d(t1), tg);
             t2 = tag_partial(wave_sum_d(t2), tg);
             release_for_records();
-            if (lane == 0) { st_dev(pb+3*wg, t0); st_dev(pb+3*wg+1, t1); st_dev(pb+3*wg+2, t2); }
+            if (lane == 0) {
+                st_dev(pb+rs*wg, t0); st_dev(pb+rs*wg+1, t1); st_dev(pb+rs*wg+2, t2);
+            }
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
                 double a0 = 0, a1 = 0, a2 = 0;
                 bool done = true;
                 for (int q = lane; q < G; q += WAVE) {
-                    const unsigned long long v0 = ld_dev(pq+3*q), v1 = ld_dev(pq+3*q+1), v2 = ld_dev(pq+3*q+2);
+                    const unsigned long long* r = pq+(size_t)rs*q;
+                    const unsigned long long v0 = ld_dev(r), v1 = ld_dev(r+1), v2 = ld_dev(r+2);
                     done &= (v0 & 0xFF) == tg && (v1 & 0xFF) == tg && (v2 & 0xFF) == tg;
                     a0 += untag_partial(v0); a1 += untag_partial(v1); a2 += untag_partial(v2);
                 }
@@ -2248,6 +2252,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
 {
     static_assert(NR >= 2, "one right-hand side: k_solve_lds_one");
     constexpr int NV = 3*NR;                         // values of one grid sum
+    static_assert(NV <= REC_STRIDE, "a record fits its slot");
     __shared__ int ent_j[LS_ENTRY_CAP];             // partner; ~partner for the transposed block
     __shared__ float ent_b[LS_ENTRY_CAP*18];
     __shared__ LdsRow rows[LS_ROWS];
@@ -2363,7 +2368,8 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
         return own ? z : 0.f;
     };
 
-    double* part = w.partials+2*NV*(size_t)J.key;
+    double* part = w.partials+LDS_PART_PER_BLOCK*(size_t)J.key;
+    const int rst = rec_stride(G, NV);   // record stride (rs: the right-hand sides' stride)
     int buf = 0;
     // Grid sums of NV values.  Block sums (DPP tree, fixed order) are
     // published as tagged partial records in the current buffer after every
@@ -2379,7 +2385,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             for (int k = 0; k < NV; k++) red[NV*wid+k] = v[k];
         drain_vmem();
         __syncthreads();
-        double* pb = part+(size_t)buf*NV*G;
+        double* pb = part+(size_t)buf*rst*G;
         seq++;
         if (wid == 0) {
             const unsigned tg = seq & 0xFF;
@@ -2390,7 +2396,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
             }
             release_for_records();
             if (lane == 0)
-                for (int k = 0; k < NV; k++) st_dev(pb+NV*wg+k, t[k]);
+                for (int k = 0; k < NV; k++) st_dev(pb+rst*wg+k, t[k]);
             const unsigned long long* pq = reinterpret_cast<const unsigned long long*>(pb);
             bool ok = false;
             for (long spin = 0; s_fail == 0 && spin < SPIN_LIMIT; spin++) {
@@ -2399,7 +2405,7 @@ __global__ __launch_bounds__(LS_BLOCK) void k_solve_lds(BsrView A, const double*
                 bool done = true;
                 for (int q = lane; q < G; q += WAVE) {
                     unsigned long long vq[NV];
-                    for (int k = 0; k < NV; k++) vq[k] = ld_dev(pq+NV*q+k);
+                    for (int k = 0; k < NV; k++) vq[k] = ld_dev(pq+(size_t)rst*q+k);
                     for (int k = 0; k < NV; k++) done &= (vq[k] & 0xFF) == tg;
                     for (int k = 0; k < NV; k++) a[k] += untag_partial(vq[k]);
                 }
@@ -2698,8 +2704,8 @@ hipError_t solve_lds(const BsrView& A, const double* b, double* x, int nrhs, siz
     BsrView Av = A;
     SolverWork wv = w;
     SolveOpts ov = o;
-    // no stale round tags in any instance's partial records (6 nrhs per row block)
-    launch_fill_double(w.partials, 0.0, 6*(size_t)nrhs*((A.n+LS_ROWS-1)/LS_ROWS), s);
+    // no stale round tags in any instance's partial records
+    launch_fill_double(w.partials, 0.0, LDS_PART_PER_BLOCK*(size_t)((A.n+LS_ROWS-1)/LS_ROWS), s);
     void* args[] = {&Av, (void*)&b, (void*)&x, &rs, &wv, &sc, &ov, (void*)&jobs};
     const void* fn = nrhs == 2 ? (const void*)k_solve_lds<2> : (const void*)k_solve_lds_one;
     return launch_persistent(fn, nwg, LS_BLOCK, args, o, lds_solver_max_groups(), s);

@@ -31,8 +31,8 @@ constexpr int NA_MAX_NP = LFD_NOALLOC_MAX_NP;
 constexpr int NA_MAX_BLOCKS = LFD_NOALLOC_MAX_BLOCKS;
 constexpr int NA_RB = (NA_MAX_NP+LDS_ROWS-1)/LDS_ROWS;        // row blocks (solver workgroups)
 constexpr int NA_MAX_GROUPS = 2048;                           // SolverWork::max_groups of the library
-constexpr size_t NA_PARTIALS = 8*(size_t)NA_MAX_GROUPS > 12*(size_t)NA_RB+64 ? 8*(size_t)NA_MAX_GROUPS
-                                                                               : 12*(size_t)NA_RB+64;
+constexpr size_t NA_PARTIALS = 8*(size_t)NA_MAX_GROUPS > LDS_PART_PER_BLOCK*(size_t)NA_RB+64
+                                    ? 8*(size_t)NA_MAX_GROUPS : LDS_PART_PER_BLOCK*(size_t)NA_RB+64;
 constexpr size_t NA_BARRIER = 16384 > 128*NA_RB+256 ? 16384 : 128*NA_RB+256;
 constexpr size_t NA_ERR_INPUT = 64;                           // DevScalars::error bit: bad block list
 

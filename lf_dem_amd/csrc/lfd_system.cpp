@@ -190,7 +190,7 @@ struct Solver {
         A(dalloc(&w.d, m));
         A(dalloc(&w.pr, m)); A(dalloc(&w.pz, m)); A(dalloc(&w.pp, m)); A(dalloc(&w.pq, m));
         w.max_groups = 2048;
-        A(dalloc(&w.partials, std::max<size_t>(8*(size_t)w.max_groups, 12*(size_t)rb+64)));
+        A(dalloc(&w.partials, std::max<size_t>(8*(size_t)w.max_groups, LDS_PART_PER_BLOCK*(size_t)rb+64)));
         // tree-barrier words: 128 per row block for the batched solver, 16384 for the global form
         nbar = std::max<size_t>(16384, 128*(size_t)rb+256);
         A(dalloc(&w.barrier, nbar));
