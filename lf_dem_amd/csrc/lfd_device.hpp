@@ -150,35 +150,6 @@ __device__ inline void bcast8_6(float v, int sub, float* o)
     o[3] = bcast8<3>(v, sub); o[4] = bcast8<4>(v, sub); o[5] = bcast8<5>(v, sub);
 }
 
-// barrier words: [0] root count, [32] generation, [64 + 32*leaf] leaf counts (128-byte spacing)
-__device__ inline unsigned ld_acquire(unsigned* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
-
-// Arrival and wait of one workgroup, run by a single thread after a
-// __syncthreads: release on arrival, acquire on the generation load.  The
-// wait gives up after SPIN_LIMIT polls and reports false.
-static constexpr long SPIN_LIMIT = 1L << 22;
-__device__ inline bool tree_barrier_core(unsigned* bar, int G, int wg, unsigned& gen)
-{
-    const int leaf = wg >> 3, nleaf = (G+7) >> 3;
-    const int lsize = min(8, G-(leaf << 3));
-    unsigned* lc = bar+64+32*leaf;
-    unsigned old = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == (unsigned)(lsize-1)) {
-        __hip_atomic_store(lc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned r = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (r == (unsigned)(nleaf-1)) {
-            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(bar+32, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    for (long spin = 0; ld_acquire(bar+32) == gen; spin++) {
-        if (spin >= SPIN_LIMIT) return false;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    gen++;
-    return true;
-}
-
 // Relaxed device-scope (sc1) accesses, performed at the device-coherent level.
 template <class T> __device__ inline void st_dev(T* p, T v)
 {
@@ -195,18 +166,57 @@ __device__ inline void drain_vmem()
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
-// Release at agent scope ahead of relaxed record stores.  The fence writes
-// back this L2's dirty lines (the exchanged vectors that other workgroups read
-// once they see the records: other XCDs' L2s do not snoop this one), and the
-// record stores must wait for that write-back to complete.  The compiler
-// emits the fence's write-back (buffer_wbl2) but drops the wait for it when
-// no other vector memory operation is outstanding, so a record could reach
-// another XCD before the data it publishes; the explicit wait restores it.
+// Release at agent scope ahead of relaxed record stores or barrier arrivals.
+// The fence writes back this L2's dirty lines (the vectors that other
+// workgroups read once they see the record or the barrier: other XCDs' L2s
+// do not snoop this one), and the following store must wait for that
+// write-back to complete.  The compiler emits the write-back (buffer_wbl2) of
+// a release fence or a release atomic but drops the wait for it when no other
+// vector memory operation is outstanding, so a record or an arrival could
+// reach another XCD before the data it publishes; the explicit wait restores
+// it.  Every release of this library goes through here, followed by relaxed
+// atomics.
 __device__ inline void release_for_records()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     drain_vmem();
 }
+// barrier words: [0] root count, [32] generation, [64 + 32*leaf] leaf counts (128-byte spacing)
+__device__ inline unsigned ld_acquire(unsigned* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Arrival and wait of one workgroup, run by a single thread after a
+// __syncthreads: release before the arrival (release_for_records, then a
+// relaxed increment), acquire after the last arrival of a leaf and of the
+// root, acquire on the generation load.  The wait gives up after SPIN_LIMIT
+// polls and reports false.
+static constexpr long SPIN_LIMIT = 1L << 22;
+__device__ inline bool tree_barrier_core(unsigned* bar, int G, int wg, unsigned& gen)
+{
+    const int leaf = wg >> 3, nleaf = (G+7) >> 3;
+    const int lsize = min(8, G-(leaf << 3));
+    unsigned* lc = bar+64+32*leaf;
+    release_for_records();
+    unsigned old = __hip_atomic_fetch_add(lc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == (unsigned)(lsize-1)) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __hip_atomic_store(lc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        release_for_records();
+        unsigned r = __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (r == (unsigned)(nleaf-1)) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            release_for_records();
+            __hip_atomic_fetch_add(bar+32, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    for (long spin = 0; ld_acquire(bar+32) == gen; spin++) {
+        if (spin >= SPIN_LIMIT) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    gen++;
+    return true;
+}
+
 // A partial sum with the low 8 mantissa bits replaced by a round tag
 // (relative perturbation below 3e-14).  Aligned 64-bit stores and loads are
 // single-copy atomic, so a record read with the expected tag is that round's.
