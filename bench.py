@@ -128,6 +128,8 @@ def run(args):
     gen_s = time.time() - t0
     params = capi.default_params()
     opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond, agg_waves=args.agg_waves)
+    if args.inner_tol is not None:
+        opts.inner_tol = args.inner_tol
     tuned = None
     if args.autotune:
         from lf_dem_amd import autotune
@@ -335,6 +337,8 @@ def run_c3(args):
     gen_s = time.time() - t0
     print("configurations generated in %.1f s" % gen_s, file=sys.stderr, flush=True)
     opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond, agg_waves=args.agg_waves)
+    if args.inner_tol is not None:
+        opts.inner_tol = args.inner_tol
     params = [capi.default_params(repulsion=s[3]) for s in specs]
     tuned = None
     if args.autotune:
@@ -432,6 +436,8 @@ def _serve_phase(args, rate, duration, drain, seed, pregen):
     print("%d of %d request configurations generated in %.1f s" % (k, len(arr), gen_s), file=sys.stderr,
           flush=True)
     opts = capi.default_opts(kind=0, tol=args.tol, launch=LAUNCHES[args.launch], precond=args.precond, agg_waves=args.agg_waves)
+    if args.inner_tol is not None:
+        opts.inner_tol = args.inner_tol
     reqs = []
     for i, a in enumerate(arr):
         if i < k:
