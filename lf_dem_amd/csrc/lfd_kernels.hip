@@ -1337,14 +1337,29 @@ __global__ __launch_bounds__(256) void k_stress_gather(const double* __restrict_
 // Sums over particles in fixed chunks (StressComponent::getTotalStress,
 // StressComponent.h:26-33), one partial row of 48 values per chunk.
 static constexpr int ST_CHUNK = 128;
+// acc + p[0] + p[s] + ... + p[(m-1) s], added in that order; the loads are
+// issued ST_BATCH at a time so that a sum waits for one memory latency per
+// batch instead of one per term
+static constexpr int ST_BATCH = 8;
+__device__ inline double ordered_sum(double acc, const double* __restrict__ p, size_t s, int m)
+{
+    int k = 0;
+    for (; k+ST_BATCH <= m; k += ST_BATCH) {
+        double t[ST_BATCH];
+#pragma unroll
+        for (int q = 0; q < ST_BATCH; q++) t[q] = p[s*(size_t)(k+q)];
+#pragma unroll
+        for (int q = 0; q < ST_BATCH; q++) acc += t[q];
+    }
+    for (; k < m; k++) acc += p[s*(size_t)k];
+    return acc;
+}
 __global__ void k_stress_chunks(const double* __restrict__ ps, int n, double* __restrict__ partial)
 {
     const int v = threadIdx.x;
     if (v >= ST_VALS) return;
     const int i0 = blockIdx.x*ST_CHUNK, i1 = min(n, i0+ST_CHUNK);
-    double acc = 0;
-    for (int i = i0; i < i1; i++) acc += ps[(size_t)ST_VALS*i+v];
-    partial[(size_t)ST_VALS*blockIdx.x+v] = acc;
+    partial[(size_t)ST_VALS*blockIdx.x+v] = ordered_sum(0.0, ps+(size_t)ST_VALS*i0+v, ST_VALS, i1-i0);
 }
 
 // Totals per slot, then either the stress-control shear rate
@@ -1362,11 +1377,7 @@ __global__ void k_stress_finalize(const double* __restrict__ partial, const Inst
     if (ic.n <= 0) return;
     const Consts& C = ic.C;
     const int b0 = ic.poff/ST_CHUNK, b1 = (ic.poff+ic.n+ST_CHUNK-1)/ST_CHUNK;
-    if (v < ST_VALS) {
-        double acc = 0;
-        for (int b = b0; b < b1; b++) acc += partial[(size_t)ST_VALS*b+v];
-        tot[v] = acc;
-    }
+    if (v < ST_VALS) tot[v] = ordered_sum(0.0, partial+(size_t)ST_VALS*b0+v, ST_VALS, b1-b0);
     __syncthreads();
     if (threadIdx.x != 0) return;
     const double vinv = 1.0/(C.L[0]*C.L[1]*C.L[2]);
