@@ -170,15 +170,21 @@ def run(args):
     inner = st1["total_inner"] - st0["total_inner"]
     iters_per_solve = inner / max(launches, 1)
     # plain single-instance GPU form on the same configuration
-    plain_sps = None
+    # (kind 1: host-side scalars, the form the workload is quoted against;
+    # kind 3: the same iterates with the scalars on the device, to show how
+    # much of the ratio is host round trips)
+    plain_sps = plain_dev_sps = None
     if args.plain_steps > 0:
-        P = capi.System(pos, rad, L, params=params, opts=capi.default_opts(kind=1, tol=args.tol, max_inner=20000))
-        P.first_update()
-        P.step(1)
-        t0 = time.perf_counter()
-        P.step(args.plain_steps)
-        plain_sps = args.plain_steps / (time.perf_counter() - t0)
-        del P
+        rates = []
+        for kind in (1, 3):
+            P = capi.System(pos, rad, L, params=params, opts=capi.default_opts(kind=kind, tol=args.tol, max_inner=20000))
+            P.first_update()
+            P.step(1)
+            t0 = time.perf_counter()
+            P.step(args.plain_steps)
+            rates.append(args.plain_steps / (time.perf_counter() - t0))
+            del P
+        plain_sps, plain_dev_sps = rates
     # roofline of the dominant kernel (the persistent solver launch)
     avg_ms = solve_ms / max(launches, 1)
     outer_per_solve = float(max(st1["iters_outer"], 1))     # refinement passes of the last solve
@@ -207,6 +213,8 @@ def run(args):
         "eager_steps_per_s": round(eager_sps, 3),
         "plain_gpu_steps_per_s": round(plain_sps, 3) if plain_sps else None,
         "speedup_vs_plain_gpu": round(sps / plain_sps, 2) if plain_sps else None,
+        "plain_gpu_device_scalars_steps_per_s": round(plain_dev_sps, 3) if plain_dev_sps else None,
+        "speedup_vs_plain_gpu_device_scalars": round(sps / plain_dev_sps, 2) if plain_dev_sps else None,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
         "solver": {"inner_iters_per_solve": round(iters_per_solve, 1), "solve_launches": launches,

@@ -49,7 +49,9 @@ typedef struct lfd_params {
 typedef enum lfd_solver_kind {
 	LFD_SOLVER_MIXED = 0,   /* FP64 refinement around persistent FP32 block-Jacobi PCG, blocks in LDS */
 	LFD_SOLVER_PLAIN = 1,   /* FP64 block-Jacobi PCG, one kernel per operation */
-	LFD_SOLVER_MIXED_GLOBAL = 2  /* as MIXED, blocks read from HBM each iteration */
+	LFD_SOLVER_MIXED_GLOBAL = 2, /* as MIXED, blocks read from HBM each iteration */
+	LFD_SOLVER_PLAIN_DEVICE = 3  /* as PLAIN with the scalars on the device: no host round trip per
+	                                iteration (the host checks every 16 iterations); same iterates */
 } lfd_solver_kind;
 
 /* Preconditioner of the MIXED solver on the device-resident paths. */

@@ -306,6 +306,9 @@ hipError_t solve_agg_large(const BsrView& A, const AggView& G, const double* b, 
 int agg_large_max_groups();
 hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
                        const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
+// the same iteration with its scalars on the device (no host round trip per iteration)
+hipError_t solve_plain_dev(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                       const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out);
 void launch_spmv64(const BsrView& A, const double* x, double* y, hipStream_t s);
 void launch_spmv32(const BsrView& A, const float* x, float* y, int lanes_per_row, hipStream_t s);
 

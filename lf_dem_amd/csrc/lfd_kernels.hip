@@ -2878,4 +2878,137 @@ hipError_t solve_plain(const BsrView& A, const double* b, double* x, SolverWork&
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ plain FP64 PCG, scalars on the device
+// The iteration of solve_plain with its scalars kept on the device: each dot
+// product ends in a one-thread kernel that also forms alpha or beta, and the
+// vector kernels read them from device memory, so an iteration has no host
+// round trip (8 kernels).  The host reads the state every PD_CHECK iterations;
+// once the residual is below the stop (or pq is not positive, or the
+// iteration limit is reached) a device flag turns the rest of the batch into
+// no-ops, so the iteration count and the result are those of solve_plain.
+// Scalars (doubles at ps): rz, pq, rr, alpha, beta, stop, done, it, maxit, bb.
+enum { PD_RZ, PD_PQ, PD_RR, PD_ALPHA, PD_BETA, PD_STOP, PD_DONE, PD_IT, PD_MAXIT, PD_BB, PD_NSCAL };
+static constexpr int PD_G = 64, PD_CHECK = 16;
+
+__global__ void k_dot2_partial(const double* a, const double* b, const double* c, const double* d, size_t m,
+                               double* part)
+{
+    __shared__ double lds[32];
+    double s = 0, t = 0;
+    for (size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x; k < m; k += (size_t)gridDim.x*blockDim.x) {
+        s += a[k]*b[k];
+        if (c) t += c[k]*d[k];
+    }
+    block_sum2(s, t, lds);
+    if (threadIdx.x == 0) { part[2*blockIdx.x] = s; part[2*blockIdx.x+1] = t; }
+}
+__device__ inline void pd_sums(const double* part, double& s, double& t)
+{
+    s = 0; t = 0;
+    for (int g = 0; g < PD_G; g++) { s += part[2*g]; t += part[2*g+1]; }
+}
+__global__ void k_pd_sum(const double* part, double* out)
+{
+    double s, t;
+    pd_sums(part, s, t);
+    *out = s;
+}
+// start: part holds (b.b, r.z), part2 (r.r, -)
+__global__ void k_pd_start(const double* part, const double* part2, double* ps, double tol2, double maxit)
+{
+    double bb, rz, rr, z;
+    pd_sums(part, bb, rz);
+    pd_sums(part2, rr, z);
+    ps[PD_BB] = bb; ps[PD_RZ] = rz; ps[PD_RR] = rr; ps[PD_STOP] = tol2*bb;
+    ps[PD_IT] = 0; ps[PD_MAXIT] = maxit;
+    ps[PD_DONE] = (bb == 0 || rr <= tol2*bb || maxit <= 0) ? 1.0 : 0.0;
+}
+__global__ void k_pd_alpha(const double* part, double* ps)
+{
+    if (ps[PD_DONE] != 0) return;
+    double pq, z;
+    pd_sums(part, pq, z);
+    ps[PD_PQ] = pq;
+    if (!(pq > 0)) { ps[PD_DONE] = 1; return; }
+    ps[PD_ALPHA] = ps[PD_RZ]/pq;
+}
+__global__ void k_pd_update_xr(double* x, double* r, const double* p, const double* q, const double* ps, size_t m)
+{
+    if (ps[PD_DONE] != 0) return;
+    const double a = ps[PD_ALPHA];
+    size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (k < m) { x[k] += a*p[k]; r[k] += (-a)*q[k]; }
+}
+// part holds (r.r, r.z)
+__global__ void k_pd_beta(const double* part, double* ps)
+{
+    if (ps[PD_DONE] != 0) return;
+    double rr, rz_new;
+    pd_sums(part, rr, rz_new);
+    ps[PD_RR] = rr;
+    ps[PD_IT] += 1;
+    if (rr <= ps[PD_STOP] || ps[PD_IT] >= ps[PD_MAXIT]) { ps[PD_DONE] = 1; return; }
+    ps[PD_BETA] = rz_new/ps[PD_RZ];
+    ps[PD_RZ] = rz_new;
+}
+__global__ void k_pd_update_p(double* p, const double* z, const double* ps, size_t m)
+{
+    if (ps[PD_DONE] != 0) return;
+    const double beta = ps[PD_BETA];
+    size_t k = blockIdx.x*(size_t)blockDim.x+threadIdx.x;
+    if (k < m) p[k] = z[k]+beta*p[k];
+}
+
+hipError_t solve_plain_dev(const BsrView& A, const double* b, double* x, SolverWork& w, DevScalars* sc,
+                           const SolveOpts& o, hipStream_t s, int* iters_out, double* res_out)
+{
+    size_t m = 6*(size_t)A.n;
+    unsigned gm = (unsigned)((m+255)/256);
+    double* part = w.partials;                    // 2 PD_G partial sums
+    double* part2 = w.partials+2*PD_G;
+    double* ps = w.partials+4*PD_G;               // PD_NSCAL scalars
+    const int maxit = o.max_inner*(o.max_outer > 0 ? o.max_outer : 1);
+    k_residual64<<<(A.n*4+255)/256, 256, 0, s>>>(A, b, x, w.pr);
+    k_precond64<<<(A.n+255)/256, 256, 0, s>>>(A, w.pr, w.pz);
+    (void)hipMemcpyAsync(w.pp, w.pz, m*sizeof(double), hipMemcpyDeviceToDevice, s);
+    k_dot2_partial<<<PD_G, 256, 0, s>>>(b, b, w.pr, w.pz, m, part);
+    k_dot2_partial<<<PD_G, 256, 0, s>>>(w.pr, w.pr, nullptr, nullptr, m, part2);
+    k_pd_start<<<1, 1, 0, s>>>(part, part2, ps, o.tol*o.tol, (double)maxit);
+    double h[PD_NSCAL];
+    (void)hipMemcpyAsync(h, ps, sizeof(h), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    if (h[PD_BB] == 0) {                          // zero right-hand side: the solution is zero
+        launch_fill_double(x, 0.0, m, s);
+        if (iters_out) *iters_out = 0;
+        if (res_out) *res_out = 0;
+        (void)sc;
+        return hipGetLastError();
+    }
+    while (h[PD_DONE] == 0) {
+        for (int k = 0; k < PD_CHECK; k++) {
+            launch_spmv64(A, w.pp, w.pq, s);
+            k_dot2_partial<<<PD_G, 256, 0, s>>>(w.pp, w.pq, nullptr, nullptr, m, part);
+            k_pd_alpha<<<1, 1, 0, s>>>(part, ps);
+            k_pd_update_xr<<<gm, 256, 0, s>>>(x, w.pr, w.pp, w.pq, ps, m);
+            k_precond64<<<(A.n+255)/256, 256, 0, s>>>(A, w.pr, w.pz);
+            k_dot2_partial<<<PD_G, 256, 0, s>>>(w.pr, w.pr, w.pr, w.pz, m, part);
+            k_pd_beta<<<1, 1, 0, s>>>(part, ps);
+            k_pd_update_p<<<gm, 256, 0, s>>>(w.pp, w.pz, ps, m);
+        }
+        (void)hipMemcpyAsync(h, ps, sizeof(h), hipMemcpyDeviceToHost, s);
+        (void)hipStreamSynchronize(s);
+    }
+    // true residual
+    k_residual64<<<(A.n*4+255)/256, 256, 0, s>>>(A, b, x, w.pr);
+    k_dot2_partial<<<PD_G, 256, 0, s>>>(w.pr, w.pr, nullptr, nullptr, m, part);
+    k_pd_sum<<<1, 1, 0, s>>>(part, ps+PD_NSCAL);
+    double rt = 0;
+    (void)hipMemcpyAsync(&rt, ps+PD_NSCAL, sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    if (iters_out) *iters_out = (int)h[PD_IT];
+    if (res_out) *res_out = sqrt(rt/h[PD_BB]);
+    (void)sc;
+    return hipGetLastError();
+}
+
 }  // namespace lfd

@@ -447,14 +447,16 @@ static int run_solve(Solver& S, const int* npairs_dev, const double* b, double* 
         }
         return LFD_OK;
     }
-    if (S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_MIXED_GLOBAL || !has_persistent(S)) {
+    const bool plain = S.kind == LFD_SOLVER_PLAIN || S.kind == LFD_SOLVER_PLAIN_DEVICE;
+    if (plain || S.kind == LFD_SOLVER_MIXED_GLOBAL || !has_persistent(S)) {
         if (S.single < 0) return fail(LFD_ERR_UNSUPPORTED, "this solver form takes a single instance");
         sc += S.single;
     }
-    if (S.kind == LFD_SOLVER_PLAIN) {
+    if (plain) {
         int it = 0;
         double res = 0;
-        HIPCHK(solve_plain(A, b, x, S.w, sc, S.opts, s, &it, &res));
+        if (S.kind == LFD_SOLVER_PLAIN) HIPCHK(solve_plain(A, b, x, S.w, sc, S.opts, s, &it, &res));
+        else HIPCHK(solve_plain_dev(A, b, x, S.w, sc, S.opts, s, &it, &res));
         DevScalars h;
         HIPCHK(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

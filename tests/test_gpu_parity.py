@@ -139,6 +139,26 @@ def test_plain_and_cooperative_launch_agree(kind, precond, capi):
         assert np.array_equal(out[0][key], out[1][key]), key
 
 
+
+def test_plain_form_with_device_scalars_equals_plain(capi):
+    """LFD_SOLVER_PLAIN_DEVICE runs the FP64 PCG of LFD_SOLVER_PLAIN with its
+    scalars on the device: the same iterates, so bit-identical steps and the
+    same iteration counts, and the velocities within 1e-8 of the mixed form."""
+    pos, rad, L, sd = _c1()
+    out, its = [], []
+    for kind in (1, 3, 0):
+        S = capi.System(pos, rad, L, sd, opts=capi.default_opts(kind=kind, precond=0, tol=1e-10, max_inner=20000))
+        S.first_update()
+        S.step(3)
+        out.append(S.state())
+        its.append(S.solve_stats())
+        print("kind", kind, its[-1], flush=True)
+    for key in ("pos", "vel", "ang_vel", "na_vel", "na_ang_vel"):
+        assert np.array_equal(out[0][key], out[1][key]), key
+    assert its[0]["iters_inner"] == its[1]["iters_inner"] and its[0]["total_inner"] == its[1]["total_inner"]
+    for key in ("na_vel", "na_ang_vel"):
+        assert np.abs(out[1][key] - out[2][key]).max() <= 1e-8 * np.abs(out[2][key]).max(), key
+
 def _pair_blocks(oracle, R, rad):
     """Per pair (p0, p1, DBlock of p0, DBlock of p1, ODBlock) as
     Interaction::RFU_DBlocks / RFU_ODBlock give them (Interaction.cpp:238-311):
