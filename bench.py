@@ -166,6 +166,7 @@ def run(args):
         S.step(1, graph=graph_sps is not None)
         lat.append((time.perf_counter() - t1) * 1e3)
     lat = np.array(lat) if lat else np.zeros(1)
+    long_window = run_long_window(S, args) if args.strain_window > 0 else None
     err_final = solve_error_at_state(S, opts, capi)
     inner = st1["total_inner"] - st0["total_inner"]
     iters_per_solve = inner / max(launches, 1)
@@ -217,6 +218,7 @@ def run(args):
         "speedup_vs_plain_gpu_device_scalars": round(sps / plain_dev_sps, 2) if plain_dev_sps else None,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
+        "long_window": long_window,
         "solver": {"inner_iters_per_solve": round(iters_per_solve, 1), "solve_launches": launches,
                    "avg_solve_ms": round(avg_ms, 4), "precond_waves": waves, "pairs_between_waves": p_off,
                    "last_solve": {"outer_passes": st1["iters_outer"], "inner_iters": st1["iters_inner"],
@@ -517,6 +519,39 @@ def run_c4(args):
         raise SystemExit("serving: no request completed below capacity")
 
 
+def run_long_window(S, args, chunk=100):
+    """The system stepped on (captured step graphs when available) until the
+    strain reaches args.strain_window: steps/s, inner iterations per solve and
+    pairs over segments of an eighth of that strain, to show the rate once
+    contacts have built up (the timed steps are the first few dozen)."""
+    graph = S.graphs() > 0
+    seg_len = args.strain_window / 8
+    st = S.state()
+    strain = st["strain"]
+    seg_start, seg_steps, seg_t, seg_inner = strain, 0, 0.0, S.solve_stats()["total_inner"]
+    total_steps, total_t, segs = 0, 0.0, []
+    while strain < args.strain_window - 1e-12:
+        t0 = time.perf_counter()
+        S.step(chunk, graph=graph)
+        S.sync()
+        dt = time.perf_counter() - t0
+        strain = S.state()["strain"]
+        seg_steps += chunk
+        seg_t += dt
+        total_steps += chunk
+        total_t += dt
+        if strain >= seg_start + seg_len or strain >= args.strain_window - 1e-12:
+            inner = S.solve_stats()["total_inner"]
+            segs.append({"strain": round(strain, 4), "steps": seg_steps, "steps_per_s": round(seg_steps / seg_t, 2),
+                         "inner_iters_per_solve": round((inner - seg_inner) / (2 * seg_steps), 1),
+                         "pairs": S.npairs})
+            print("long window: strain %.3f, %d steps, %.1f steps/s" % (strain, seg_steps, seg_steps / seg_t),
+                  file=sys.stderr, flush=True)
+            seg_start, seg_steps, seg_t, seg_inner = strain, 0, 0.0, inner
+    return {"strain_end": round(strain, 4), "steps": total_steps, "steps_per_s": round(total_steps / total_t, 2),
+            "stepping": "captured step graphs" if graph else "eager", "segments": segs}
+
+
 def run_copy(args):
     """Counter calibration: device-to-device copies of 256 MiB (64 Mi float32)
     whose HBM traffic is known, for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
@@ -622,6 +657,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--strain-window", type=float, default=0.0,
+                    help="c2: after the timed steps, step on to this strain and report steps/s per segment")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--rate", type=float, default=20.0, help="c4: request arrivals per second (saturation run)")
     ap.add_argument("--duration", type=float, default=25.0, help="c4: seconds of arrivals and service (saturation)")
