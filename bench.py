@@ -282,6 +282,7 @@ def run_c5(args):
     st1 = S.solve_stats()
     S.step_stress()
     sr = S.stress()
+    long_window = run_long_window(S, args) if args.strain_window > 0 else None
     err_final = solve_error_at_state(S, opts, capi)
     sps = args.steps/elapsed
     # roofline of the solver launch (the aggregate solver of a large instance
@@ -325,6 +326,7 @@ def run_c5(args):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": round(traffic) if traffic else None, "algorithmic_bytes_per_launch": round(alg)},
         "confgen_s": round(gen_s, 1),
+        "long_window": long_window,
         "rel_err_final_state": err_final,
     }
     if tuned:
@@ -658,7 +660,7 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--strain-window", type=float, default=0.0,
-                    help="c2: after the timed steps, step on to this strain and report steps/s per segment")
+                    help="c2, c5: after the timed steps, step on to this strain and report steps/s per segment")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--rate", type=float, default=20.0, help="c4: request arrivals per second (saturation run)")
     ap.add_argument("--duration", type=float, default=25.0, help="c4: seconds of arrivals and service (saturation)")
